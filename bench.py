@@ -1,0 +1,217 @@
+"""Benchmark of the gfx950 SH engine on the single-large-solve pipeline (one GPU).
+
+One step = one correlation + structure kernel of one reference station on the 2400 x 2400, order-8,
+nt = 8000 workload (fd2d_noise_amd/workloads.py: config2), all inputs resident in HBM:
+  Green run, mode 1 (G stored every 4th step)                 nx*nz*nt      GP-updates
+  correlation run, mode 1 (C stored every 4th step)           nx*nz*(2nt-1)
+  cc traveltime misfit + adjoint sources (FP64, host)         16 receivers
+  adjoint run, mode 1 against C (point sources, stf_fft)      nx*nz*(2nt-1)
+  adjoint run, mode 0 against G (distributed source, half)    nx*nz*nt
+which is the 'structure' branch of eval_objective_and_gradient.m:52-98.
+
+Prints one JSON line: GP-updates/s over the timed steps, with correlations+kernels/s, per-step
+latency p50/p99, the roofline of the dominant step kernel pair (measured with HIP events on the
+solver stream) and the FP64 CPU oracle timed on a bounded sample of the same chain.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per GP-update of a fused FP32 step (v, sigma_xy, sigma_zy, dt/rho, dt*mu
+# read, v, sigma_xy, sigma_zy written = 32 B; +8 B for u read/write; +4/fw_nth B per stored or
+# replayed FP32 snapshot).
+def alg_bytes(solver: str, fw_nth: int) -> float:
+    return {"green": 32.0, "correlation": 40.0, "adjoint": 40.0}[solver] + 4.0 / fw_nth
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nt", type=int, default=8000, help="time steps (8000 is the benchmark workload)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: up to 16 host cores)")
+    ap.add_argument("--cpu-nt", type=int, default=16, help="nt of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--timing-every", type=int, default=8)
+    return ap.parse_args()
+
+
+class Pipeline:
+    """Device buffers and the five stages of one correlation + structure-kernel step."""
+
+    def __init__(self, w):
+        import torch
+        from fd2d_noise_amd import device as D
+        self.torch, self.D, self.w = torch, D, w
+        c = self.c = w["config"]
+        cu = "cuda"
+        self.mu = D.to_device(w["mu"])
+        self.mu_obs = D.to_device(w["mu_obs"])
+        self.rho = D.to_device(w["rho"])
+        self.src = D.to_device(w["src"])
+        self.rec = D.to_device(w["rec"])
+        self.spec = D.to_device(w["spectrum"])
+        self.sd = D.to_device(w["source_dist"])
+        nr = w["rec"].shape[0]
+        self.nr = nr
+        cd, f64, f32 = torch.complex128, torch.float64, torch.float32
+        self.G_fft = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
+        self.G_out = D.empty_like_dims((c.nx, c.nz, c.n_fw_green()), f32)
+        self.seis_g = D.empty_like_dims((nr, c.nt), f64)
+        self.corr = D.empty_like_dims((nr, c.n_corr), f64)
+        self.C_out = D.empty_like_dims((c.nx, c.nz, c.n_fw_corr()), f32)
+        self.K1 = D.empty_like_dims((c.nx, c.nz, 2), f64)
+        self.K2 = D.empty_like_dims((c.nx, c.nz, 2), f64)
+        self.stf_fft = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
+        self.seis_a = D.empty_like_dims((nr, c.n_corr), f64)
+        self.t = c.t_corr()
+        self.obs = None
+
+    def observed(self):
+        """Observed correlations: the homogeneous 4 km/s model (computed once, untimed)."""
+        D, c = self.D, self.c
+        D.run_forward1_green(self.mu_obs, self.rho, self.src, None, 0, None, None, config=c, G_fft=self.G_fft)
+        D.run_forward2_correlation(self.mu_obs, self.rho, self.G_fft, self.spec, self.sd, self.rec, 0, None, None,
+                                   config=c, seismograms=self.corr)
+        self.obs = D.to_host(self.corr).copy()
+
+    def step(self):
+        from fd2d_noise_amd.measurement import make_adjoint_sources
+        D, c = self.D, self.c
+        D.run_forward1_green(self.mu, self.rho, self.src, None, 1, None, None, config=c, G_fft=self.G_fft,
+                             G_out=self.G_out)
+        D.run_forward2_correlation(self.mu, self.rho, self.G_fft, self.spec, self.sd, self.rec, 1, None, None,
+                                   config=c, seismograms=self.corr, C_out=self.C_out)
+        u = D.to_host(self.corr)
+        misfit, adstf = make_adjoint_sources(u, self.obs, None, self.t, "dis", "cc_time_shift")
+        stf = D.to_device(adstf.astype(np.complex128))
+        D.run_noise_adjoint(self.mu, self.rho, self.C_out, stf, self.rec, self.rec, None, None, None, 1, None, None,
+                            config=c, K=self.K1, stf_fft=self.stf_fft, seismograms=self.seis_a)
+        D.run_noise_adjoint(self.mu, self.rho, self.G_out, self.stf_fft, self.rec, self.rec, self.spec, self.sd, None,
+                            0, None, None, config=c, K=self.K2)
+        return float(misfit.sum())
+
+    def gp_updates(self):
+        c = self.c
+        return c.nx * c.nz * (c.nt + c.n_corr + c.n_corr + c.nt)
+
+
+def cpu_baseline(w, nt_sample, threads):
+    """The FP64 oracle on the same chain at the full grid, truncated to nt_sample Green steps."""
+    import oracle
+    c = w["config"].replace(nt=nt_sample)
+    c.absorb_width = w["config"].absorb_width
+    orc = oracle.solvers()
+    oracle.set_threads(threads)
+    rng = np.random.default_rng(0)
+    t0 = time.perf_counter()
+    G, Gout, _ = orc.run_forward1_green(w["mu"], w["rho"], w["src"], w["rec"], 1, config=c)
+    corr, C = orc.run_forward2_correlation(w["mu"], w["rho"], G, w["spectrum"], w["source_dist"], w["rec"], 1,
+                                           config=c)
+    stf = rng.standard_normal(corr.shape).astype(np.complex128)
+    _, sf, _, _ = orc.run_noise_adjoint(w["mu"], w["rho"], C, stf, w["rec"], w["rec"], None, None, None, 1, config=c)
+    orc.run_noise_adjoint(w["mu"], w["rho"], Gout, sf, w["rec"], w["rec"], w["spectrum"], w["source_dist"], None, 0,
+                          config=c)
+    el = time.perf_counter() - t0
+    gp = c.nx * c.nz * (2 * c.nt + 2 * c.n_corr)
+    return {"value": gp / el, "unit": "GP-updates/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, Green+correlation+adjoint(1)+adjoint(0) chain on the full "
+                      f"{c.nx}x{c.nz} order-{c.order} grid truncated to nt={nt_sample} "
+                      f"({2 * nt_sample + 2 * c.n_corr} steps, {gp:.3g} GP-updates, synthetic adjoint "
+                      f"sources), {el:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus N > 1 is not supported "
+                                   "(stations shard across GPUs with one all-reduce of the kernels, "
+                                   "which is not part of this build)"}))
+        sys.exit(2)
+    import torch
+    from fd2d_noise_amd import _lib
+    from fd2d_noise_amd.workloads import config2_workload
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    w = config2_workload(nt=args.nt)
+    c = w["config"]
+    print(f"# workload config2 {c.nx}x{c.nz} order {c.order} nt {c.nt} fw_nth {c.fw_nth} K {c.n_sample}",
+          file=sys.stderr, flush=True)
+    p = Pipeline(w)
+    p.observed()
+    print("# observed traces ready", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        p.step()
+        print(f"# warmup step {i} done", file=sys.stderr, flush=True)
+    _lib.timing_enable(args.timing_every)
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        s0 = time.perf_counter()
+        misfit = p.step()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - s0)
+        print(f"# step {i}: {lat[-1]:.3f} s misfit {misfit:.4g}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    tim = _lib.timing_read()
+    _lib.timing_enable(0)
+
+    gp = p.gp_updates() * args.steps
+    # dominant step-kernel pair: the solver with the most measured kernel time per solve
+    best, best_ms = None, -1.0
+    for name, d in tim.items():
+        if d["launches"]:
+            per = (d["velocity_ms"] + d["stress_ms"]) / d["launches"]
+            steps = {"green": c.nt, "correlation": c.n_corr, "adjoint": c.n_corr + c.nt}[name]
+            if per * steps > best_ms:
+                best, best_ms = name, per * steps
+    d = tim[best]
+    per_ms = (d["velocity_ms"] + d["stress_ms"]) / d["launches"]
+    bytes_step = alg_bytes(best, c.fw_nth) * c.nx * c.nz
+    achieved = bytes_step / (per_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"{best} step: k_velocity + k_stress (order {c.order})",
+            "avg_step_us": round(per_ms * 1e3, 2),
+            "avg_velocity_us": round(d["velocity_ms"] / d["launches"] * 1e3, 2),
+            "avg_stress_us": round(d["stress_ms"] / d["launches"] * 1e3, 2),
+            "alg_bytes_per_gp": alg_bytes(best, c.fw_nth)}
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(w, args.cpu_nt, threads)
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": "Gpt-updates/s, correlations+kernels/s, p99 latency; FP32 vs plain, L2<=1e-5",
+        "value": gp / el, "unit": "GP-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 fields, f64 DFT accumulators/kernels", "data": "synthetic",
+        "config": {"workload": "config2: 2400x2400 order-8 nt=8000 Green+correlation+cc-traveltime adjoint "
+                               "structure kernel, fw_nth=4, 121 frequencies, 16 receivers",
+                   "nx": c.nx, "nz": c.nz, "order": c.order, "nt": c.nt, "fw_nth": c.fw_nth,
+                   "n_sample": c.n_sample, "receivers": p.nr, "gp_updates_per_step": p.gp_updates()},
+        "correlations_kernels_per_s": args.steps / el,
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,75 @@
+"""Loader of the in-tree HIP library ``libfd2d_noise_amd.so``.
+
+The library is built in place by ``fd2d_noise_amd/build.py``. There is no fallback: if it is
+missing or cannot be loaded, every solver call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from . import abi
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfd2d_noise_amd.so")
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run fd2d_noise_amd/build.py (or __graft_entry__.build())")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.fd2d_last_error.restype = ctypes.c_char_p
+        lib.fd2d_last_error.argtypes = []
+        lib.fd2d_version.restype = ctypes.c_char_p
+        lib.fd2d_version.argtypes = []
+        lib.fd2d_query_output_dims.restype = ctypes.c_int
+        lib.fd2d_query_output_dims.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_int64,
+                                               ctypes.POINTER(ctypes.c_int64 * 3)]
+        for name, at in (("fd2d_run_forward1_green_device", abi.GREEN_ARGTYPES + [ctypes.c_void_p]),
+                         ("fd2d_run_forward2_correlation_device", abi.CORR_ARGTYPES + [ctypes.c_void_p]),
+                         ("fd2d_run_noise_adjoint_device", abi.ADJ_ARGTYPES + [ctypes.c_void_p])):
+            f = getattr(lib, name)
+            f.argtypes = at
+            f.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def last_error() -> str:
+    return load().fd2d_last_error().decode()
+
+
+def version() -> str:
+    return load().fd2d_version().decode()
+
+
+# Every symbol include/fd2d_noise.h declares.
+EXPORTS = (
+    "fd2d_run_forward1_green", "fd2d_run_forward2_correlation", "fd2d_run_noise_adjoint",
+    "fd2d_run_forward1_green_device", "fd2d_run_forward2_correlation_device",
+    "fd2d_run_noise_adjoint_device", "fd2d_query_output_dims", "fd2d_last_error", "fd2d_version",
+    "fd2d_timing_enable", "fd2d_timing_read",
+)
+
+
+class Timing(ctypes.Structure):
+    """fd2d_timing of include/fd2d_noise.h."""
+    _fields_ = [("launches", (ctypes.c_int64 * 2) * 3), ("ms", (ctypes.c_double * 2) * 3),
+                ("points", ctypes.c_int64 * 3)]
+
+
+def timing_enable(every: int) -> None:
+    load().fd2d_timing_enable(int(every))
+
+
+def timing_read() -> dict:
+    t = Timing()
+    load().fd2d_timing_read(ctypes.byref(t))
+    out = {}
+    for q, name in enumerate(("green", "correlation", "adjoint")):
+        out[name] = {"launches": int(t.launches[q][0]), "velocity_ms": float(t.ms[q][0]),
+                     "stress_ms": float(t.ms[q][1]), "points": int(t.points[q])}
+    return out

@@ -1,0 +1,1034 @@
+// capi.hip -- C ABI of include/fd2d_noise.h: argument checks with the reference's empty-array
+// semantics, device set-up, and the time loops of the three solvers on one HIP stream.
+//
+// The loops follow the step order of code/run_forward1_green.m:130-192,
+// code/run_forward2_correlation.m:178-251 and code/run_noise_adjoint.m:233-355; every kernel they
+// launch is in kernels.hip.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fd2d_noise.h"
+#include "engine.h"
+
+using namespace fd2d;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Failure {
+    int code;
+};
+
+[[noreturn]] void raise(int code, const char* fmt, ...)
+{
+    char buf[768];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    throw Failure{code};
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) raise(FD2D_EDEVICE, "%s failed: %s", #x, hipGetErrorString(e_));     \
+    } while (0)
+
+// Owning device buffer.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    void alloc(size_t count)
+    {
+        release();
+        if (count == 0) return;
+        if (hipMalloc((void**)&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            raise(FD2D_ENOMEM, "device allocation of %zu bytes failed", count * sizeof(T));
+        }
+        n = count;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+};
+
+int64_t numel(const fd2d_carray* a)
+{
+    if (!a || !a->data) return 0;
+    return a->dims[0] * a->dims[1] * a->dims[2];
+}
+int64_t numel(const fd2d_marray* a)
+{
+    if (!a || !a->data) return 0;
+    return a->dims[0] * a->dims[1] * a->dims[2];
+}
+// MATLAB size(A,d) with [] = 0 x 0 (size 1 along the third dimension)
+int64_t dim(const fd2d_carray* a, int d) { return numel(a) ? a->dims[d] : (d == 2 ? 1 : 0); }
+
+// Staggered first-derivative coefficients. Orders 2 and 4 are the ones of
+// code/propagation/dx_v.m:23,29; 6 and 8 are the Taylor coefficients of the same stencil family.
+const double kCoef[5][4] = {{0, 0, 0, 0},
+                            {1.0, 0, 0, 0},
+                            {9.0 / 8.0, -1.0 / 24.0, 0, 0},
+                            {75.0 / 64.0, -25.0 / 384.0, 3.0 / 640.0, 0},
+                            {1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0}};
+
+struct Geometry {
+    int nx, nz, R, K, s, f, nt;
+    double dx, dz, dt, dw;
+    size_t P;
+};
+
+Geometry geometry(const fd2d_config* cfg)
+{
+    if (!cfg) raise(FD2D_EINVAL, "config is NULL");
+    Geometry g{};
+    g.nx = cfg->nx;
+    g.nz = cfg->nz;
+    if (g.nx < 3 || g.nz < 3) raise(FD2D_EINVAL, "nx and nz must be >= 3 (got %d x %d)", g.nx, g.nz);
+    if (cfg->order != 2 && cfg->order != 4 && cfg->order != 6 && cfg->order != 8)
+        raise(FD2D_EUNSUPPORTED, "FD order %d is not one of 2, 4, 6, 8", cfg->order);
+    g.R = cfg->order / 2;
+    g.K = cfg->n_sample;
+    g.s = cfg->freq_samp;
+    g.f = cfg->fw_nth;
+    g.nt = cfg->nt;
+    if (g.s < 1 || g.f < 1 || g.nt < 1) raise(FD2D_EINVAL, "freq_samp, fw_nth and nt must be >= 1");
+    if (g.K < 0 || (g.K > 0 && !cfg->w_sample)) raise(FD2D_EINVAL, "w_sample must hold n_sample values");
+    if (!(cfg->Lx > 0) || !(cfg->Lz > 0) || !(cfg->dt > 0)) raise(FD2D_EINVAL, "Lx, Lz and dt must be positive");
+    if (cfg->n_basis_fct < 0 || cfg->n_basis_fct == 1 || (cfg->n_basis_fct != 0 && cfg->n_basis_fct > g.K))
+        raise(FD2D_EUNSUPPORTED, "n_basis_fct=%d: the band interpolation needs 0 or 2..n_sample", cfg->n_basis_fct);
+    g.dx = cfg->Lx / (cfg->nx - 1);
+    g.dz = cfg->Lz / (cfg->nz - 1);
+    g.dt = cfg->dt;
+    g.dw = cfg->dw;
+    g.P = (size_t)g.nx * (size_t)g.nz;
+    return g;
+}
+
+// Nearest grid node per axis, lowest index on ties (code/run_forward1_green.m:45-60).
+int nearest_node(double p, double h, int n)
+{
+    int best = 0;
+    double bd = std::fabs(0.0 - p);
+    for (int i = 1; i < n; ++i) {
+        const double d = std::fabs(i * h - p);
+        if (d < bd) {
+            bd = d;
+            best = i;
+        }
+    }
+    return best;
+}
+
+template <class T>
+std::vector<T> to_host(const void* dev, size_t n)
+{
+    std::vector<T> h(n);
+    if (n) HIPCHK(hipMemcpy(h.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost));
+    return h;
+}
+
+std::vector<int2> node_ids(const fd2d_carray* pts, const Geometry& g, const char* what)
+{
+    std::vector<int2> ids;
+    const int64_t n = numel(pts) ? pts->dims[0] : 0;
+    if (n == 0) return ids;
+    if (pts->dims[1] != 2 || pts->dims[2] != 1) raise(FD2D_EINVAL, "%s must be an n x 2 array", what);
+    const std::vector<double> h = to_host<double>(pts->data, (size_t)n * 2);
+    ids.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        ids[(size_t)i].x = nearest_node(h[(size_t)i], g.dx, g.nx);
+        ids[(size_t)i].y = nearest_node(h[(size_t)(i + n)], g.dz, g.nz);
+    }
+    return ids;
+}
+
+// interpol_matrix of code/run_forward2_correlation.m:121-165 with the band limits of
+// code/propagation/integration_limits.m:2-14 (K x max(nb,1), column-major).
+std::vector<double> interpol_matrix(int K, int nb)
+{
+    if (nb == 0) return std::vector<double>((size_t)K, 1.0);
+    std::vector<double> I((size_t)K * nb, 0.0);
+    const double step = (double)K / (double)nb;
+    for (int k = 1; k <= K; ++k) {
+        int ib = -1, lo = 0, hi = 0;
+        for (int b = 1; b <= nb; ++b) {
+            const int l = (int)std::floor((b - 1) * step + 1), h = (int)std::floor(b * step);
+            if (k >= l && k <= h) {
+                ib = b;
+                lo = l;
+                hi = h;
+                break;
+            }
+        }
+        if (ib < 0) raise(FD2D_EINVAL, "frequency %d lies in no band", k);
+        const int ni = hi - lo + 1;
+        const int below = ib == 1 ? 0 : ib - 1;
+        const int above = ib == 1 ? 2 : (ib == nb ? 0 : ib + 1);
+        const double i = (double)(k - lo + 1), half = ni / 2.0;
+        auto at = [&](int kk, int bb) -> double& { return I[(size_t)(kk - 1) + (size_t)K * (bb - 1)]; };
+        if (i < half) {
+            at(k, ib) = 1 - (half - i) / ni;
+            if (below) at(k, below) = (half - i) / ni;
+        } else if (i == half) {
+            at(k, ib) = 1;
+        } else {
+            at(k, ib) = 1 - (i - half) / ni;
+            if (above) at(k, above) = (i - half) / ni;
+        }
+    }
+    return I;
+}
+
+// Separable Gaussian taper of code/propagation/init_absbound.m:16-30: absbound(i,j) = ax(i)*az(j).
+void taper(const fd2d_config* cfg, const Geometry& g, std::vector<float>& ax, std::vector<float>& az)
+{
+    const double w = cfg->absorb_width, Lx = cfg->Lx, Lz = cfg->Lz;
+    auto lo_side = [w](double X) { return (double)(X > w) + std::exp(-(X - w) * (X - w) / ((2 * w) * (2 * w))) * (double)(X <= w); };
+    auto hi_side = [w](double X, double L) {
+        return (double)(X < (L - w)) + std::exp(-(X - (L - w)) * (X - (L - w)) / ((2 * w) * (2 * w))) * (double)(X >= (L - w));
+    };
+    ax.resize(g.nx);
+    az.resize(g.nz);
+    for (int i = 0; i < g.nx; ++i) {
+        const double X = i * g.dx;
+        double a = 1.0;
+        if (cfg->absorb_left == 1) a *= lo_side(X);
+        if (cfg->absorb_right == 1) a *= hi_side(X, Lx);
+        ax[i] = (float)a;
+    }
+    for (int j = 0; j < g.nz; ++j) {
+        const double Z = j * g.dz;
+        double a = 1.0;
+        if (cfg->absorb_bottom == 1) a *= lo_side(Z);
+        if (cfg->absorb_top == 1) a *= hi_side(Z, Lz);
+        az[j] = (float)a;
+    }
+}
+
+// Copies a host table to a new device buffer; waits for the copy so the host vector may go away.
+template <class T>
+void upload(DBuf<T>& d, const std::vector<T>& h, hipStream_t st)
+{
+    d.alloc(h.size());
+    if (h.empty()) return;
+    HIPCHK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+// Phase table e^{sign i w_k t_j} * scale for the listed times, rows j, columns k.
+std::vector<double2> phase_table(const fd2d_config* cfg, const std::vector<double>& t, double sign, double scale)
+{
+    const int K = cfg->n_sample;
+    std::vector<double2> c(t.size() * (size_t)K);
+    for (size_t j = 0; j < t.size(); ++j)
+        for (int k = 0; k < K; ++k) {
+            const double ph = sign * cfg->w_sample[k] * t[j];
+            c[j * K + k] = make_double2(scale * std::cos(ph), scale * std::sin(ph));
+        }
+    return c;
+}
+
+const double kInvSqrt2Pi = 1.0 / std::sqrt(2.0 * M_PI);
+
+// Time-stepping state of one solve on one stream.
+struct Solver {
+    const fd2d_config* cfg;
+    Geometry g;
+    hipStream_t st;
+    PaddedLayout lay;
+    DBuf<float> fields;  // v, sxy, szy, u
+    DBuf<float> coefs;   // bv, dtmu
+    DBuf<float> ax_d, az_d;
+    StepArgs a{};
+
+    Solver(const fd2d_config* c, const Geometry& geo, hipStream_t s, bool with_u) : cfg(c), g(geo), st(s)
+    {
+        lay = PaddedLayout::make(g.nx, g.nz);
+        const size_t E = lay.elems(), o = lay.origin();
+        const int nf = with_u ? 4 : 3;
+        fields.alloc(E * nf);
+        HIPCHK(hipMemsetAsync(fields.p, 0, E * nf * sizeof(float), st));
+        coefs.alloc(E * 2);
+        HIPCHK(hipMemsetAsync(coefs.p, 0, E * 2 * sizeof(float), st));
+        std::vector<float> ax, az;
+        taper(cfg, g, ax, az);
+        upload(ax_d, ax, st);
+        upload(az_d, az, st);
+        a.v = fields.p + o;
+        a.sxy = fields.p + E + o;
+        a.szy = fields.p + 2 * E + o;
+        a.u = with_u ? fields.p + 3 * E + o : nullptr;
+        a.bv = coefs.p + o;
+        a.dtmu = coefs.p + E + o;
+        a.ax = ax_d.p;
+        a.az = az_d.p;
+        a.nx = g.nx;
+        a.nz = g.nz;
+        a.pitch = lay.pitch;
+        for (int m = 0; m < 4; ++m) {
+            a.cx[m] = (float)(kCoef[g.R][m] / g.dx);
+            a.cz[m] = (float)(kCoef[g.R][m] / g.dz);
+        }
+        a.dt = (float)g.dt;
+    }
+    void material(const double* mu, const double* rho)
+    {
+        launch_material(mu, rho, coefs.p + lay.origin(), coefs.p + lay.elems() + lay.origin(), g.nx, g.nz,
+                        lay.pitch, g.dt, st);
+    }
+    void clear_step()
+    {
+        a.nps = 0;
+        a.ps_ij = nullptr;
+        a.ps_amp = nullptr;
+        a.ps_const = 0.f;
+        a.syn = nullptr;
+        a.snap = nullptr;
+        a.dsamp = nullptr;
+        a.nrec = 0;
+        a.rec_ij = nullptr;
+        a.seis_col = nullptr;
+        a.born_dmu = a.born_in0 = a.born_in1 = nullptr;
+        a.kmu_fwd = nullptr;
+        a.kmu = nullptr;
+        a.kmu_scale = 0.0;
+    }
+};
+
+// Blocked frequency-domain helpers: time samples are staged in FP32 slabs of up to kSlab slices.
+constexpr int kSlab = 128;
+
+struct DftAccumulator {
+    double2* acc = nullptr;
+    size_t P = 0;
+    int K = 0;
+    DBuf<float> slab;
+    DBuf<double2> coef;
+    int cnt = 0, done = 0, T = 0;
+    hipStream_t st = nullptr;
+    void init(double2* acc_, size_t P_, int K_, const std::vector<double2>& table, size_t n_samples, hipStream_t s)
+    {
+        acc = acc_;
+        P = P_;
+        K = K_;
+        st = s;
+        T = (int)std::min<size_t>(kSlab, std::max<size_t>(n_samples, 1));
+        slab.alloc((size_t)T * P);
+        upload(coef, table, st);
+        HIPCHK(hipMemsetAsync(acc, 0, P * (size_t)K * sizeof(double2), st));
+    }
+    float* next_slot() { return slab.p + (size_t)cnt * P; }
+    void commit()
+    {
+        if (++cnt == T) flush();
+    }
+    void flush()
+    {
+        if (cnt == 0) return;
+        launch_dft_flush(acc, slab.p, coef.p + (size_t)done * K, P, K, cnt, st);
+        done += cnt;
+        cnt = 0;
+    }
+};
+
+struct Synthesiser {
+    const double2* A = nullptr;
+    size_t P = 0;
+    int K = 0, T = 0, n_total = 0;
+    DBuf<float> slab;
+    DBuf<double2> coef;
+    hipStream_t st = nullptr;
+    void init(const double2* A_, size_t P_, int K_, const std::vector<double2>& table, int n_samples, hipStream_t s)
+    {
+        A = A_;
+        P = P_;
+        K = K_;
+        st = s;
+        n_total = n_samples;
+        T = std::min(kSlab, std::max(n_samples, 1));
+        slab.alloc((size_t)T * P);
+        upload(coef, table, st);
+    }
+    // Source slice of the idx-th sampled step (0-based); synthesises a new block when needed.
+    const float* slice(int idx)
+    {
+        const int slot = idx % T;
+        if (slot == 0) launch_synth(slab.p, A, coef.p + (size_t)idx * K, P, K, std::min(T, n_total - idx), st);
+        return slab.p + (size_t)slot * P;
+    }
+};
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------------------------------------
+// Step-kernel timing (fd2d_timing_enable / fd2d_timing_read)
+// ---------------------------------------------------------------------------------------------
+std::mutex g_timing_mu;
+int g_timing_every = 0;
+fd2d_timing g_timing{};
+
+struct StepTimer {
+    int solver, every;
+    size_t P;
+    hipStream_t st;
+    std::vector<hipEvent_t> ev;  // triples: before velocity, between, after stress
+    StepTimer(int solver_, size_t P_, hipStream_t s) : solver(solver_), P(P_), st(s)
+    {
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        every = g_timing_every;
+    }
+    bool on(int n) const { return every > 0 && n % every == 0; }
+    void mark(int n)
+    {
+        if (!on(n)) return;
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventRecord(e, st));
+        ev.push_back(e);
+    }
+    // Call after the stream has been synchronised.
+    void collect()
+    {
+        double ms_v = 0, ms_s = 0;
+        int64_t cnt = 0;
+        for (size_t q = 0; q + 2 < ev.size(); q += 3) {
+            float a = 0, b = 0;
+            HIPCHK(hipEventElapsedTime(&a, ev[q], ev[q + 1]));
+            HIPCHK(hipEventElapsedTime(&b, ev[q + 1], ev[q + 2]));
+            ms_v += a;
+            ms_s += b;
+            ++cnt;
+        }
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        if (cnt) {
+            std::lock_guard<std::mutex> lk(g_timing_mu);
+            g_timing.launches[solver - 1][0] += cnt;
+            g_timing.launches[solver - 1][1] += cnt;
+            g_timing.ms[solver - 1][0] += ms_v;
+            g_timing.ms[solver - 1][1] += ms_s;
+            g_timing.points[solver - 1] = (int64_t)P;
+        }
+    }
+    ~StepTimer()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+void check_out(const fd2d_marray* o, int64_t d0, int64_t d1, int64_t d2, const char* what)
+{
+    if (!o || !o->data) return;
+    if (o->dims[0] * o->dims[1] * o->dims[2] != d0 * d1 * d2)
+        raise(FD2D_EINVAL, "output %s must hold %lld x %lld x %lld elements", what, (long long)d0, (long long)d1,
+              (long long)d2);
+}
+
+int64_t green_n_fw(const Geometry& g)
+{
+    int64_t n = 0;
+    for (int64_t q = g.nt; q <= 2 * (int64_t)g.nt - 1; ++q)
+        if (q % g.f == 0) ++n;
+    return n;
+}
+
+// =============================================================================================
+// run_forward1_green (code/run_forward1_green.m)
+// =============================================================================================
+void green_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* src,
+                  const fd2d_carray* rec, double mode, const fd2d_carray* dmu, const fd2d_carray* G_in,
+                  fd2d_marray* G_fft, fd2d_marray* G_out, fd2d_marray* seis, hipStream_t st)
+{
+    const Geometry g = geometry(cfg);
+    const int64_t P = (int64_t)g.P;
+    if (numel(mu) != P || numel(rho) != P) raise(FD2D_EINVAL, "mu and rho must have nx*nz = %lld elements", (long long)P);
+    const bool born = numel(dmu) != 0;
+    if (born && numel(dmu) != P) raise(FD2D_EINVAL, "dmu must have nx*nz elements");
+    if (born && (numel(G_in) == 0 || G_in->dims[0] * G_in->dims[1] != P || dim(G_in, 2) < g.nt))
+        raise(FD2D_EINVAL, "Born mode needs G_in with nt slices of nx x nz (run_forward1_green.m:141)");
+    const std::vector<int2> sid = node_ids(src, g, "src");
+    const std::vector<int2> rid = node_ids(rec, g, "rec");
+    const int64_t nr = (int64_t)rid.size();
+    const int64_t n_fw = green_n_fw(g);
+    check_out(G_fft, g.nx, g.nz, g.K, "G_fft");
+    if (mode != 0) check_out(G_out, g.nx, g.nz, n_fw, "G_out");
+    check_out(seis, nr, g.nt, 1, "seismograms");
+    double2* Gf = (G_fft && G_fft->data && g.K > 0) ? (double2*)G_fft->data : nullptr;
+    float* Gout = (mode != 0 && G_out && G_out->data) ? (float*)G_out->data : nullptr;
+    double* sg = (seis && seis->data) ? (double*)seis->data : nullptr;
+
+    Solver S(cfg, g, st, false);
+    S.material((const double*)mu->data, (const double*)rho->data);
+    DBuf<int2> sid_d, rid_d;
+    upload(sid_d, sid, st);
+    upload(rid_d, rid, st);
+    DBuf<float> dtdmu;
+    if (born) {
+        dtdmu.alloc(g.P);
+        launch_scale_to_float((const double*)dmu->data, dtdmu.p, g.P, g.dt, st);
+    }
+    if (sg && nr) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * g.nt, st));
+
+    // sampled steps of :176 and their times t(n) = (n-1) dt (fft_coeff, :81-90)
+    std::vector<double> ts;
+    for (int n = 1; n <= g.nt; ++n)
+        if ((n + g.nt - 1) % g.s == 0) ts.push_back((n - 1) * g.dt);
+    DftAccumulator dft;
+    if (Gf) dft.init(Gf, g.P, g.K, phase_table(cfg, ts, -1.0, 1.0), ts.size(), st);
+
+    StepTimer timer(FD2D_GREEN, g.P, st);
+    int64_t i_fw_out = 0;
+    for (int n = 1; n <= g.nt; ++n) {
+        S.clear_step();
+        const bool snap_step = mode != 0 && (n + g.nt - 1) % g.f == 0;
+        if (snap_step) {
+            if (Gout) S.a.snap = Gout + (size_t)i_fw_out * g.P;
+            ++i_fw_out;
+        }
+        if (born) launch_born_green(g.R, S.a, dtdmu.p, (const float*)G_in->data + (size_t)(n - 1) * g.P, st);
+        else {
+            S.a.nps = (int)sid.size();
+            S.a.ps_ij = sid_d.p;
+            S.a.ps_const = 1.0e9f;  // stf = 1e9 every step (:64)
+        }
+        if (Gf && (n + g.nt - 1) % g.s == 0) S.a.dsamp = dft.next_slot();
+        if (sg && nr) {
+            S.a.nrec = (int)nr;
+            S.a.rec_ij = rid_d.p;
+            S.a.seis_col = sg + (size_t)nr * (n - 1);
+        }
+        timer.mark(n);
+        launch_velocity(g.R, S.a, st);
+        timer.mark(n);
+        launch_stress(g.R, S.a, st);
+        timer.mark(n);
+        if (S.a.dsamp) dft.commit();
+    }
+    if (Gf) {
+        dft.flush();
+        // G_fft = conj(sum v * (2pi)^-1/2 e^{-iwt} dt)  (:178-180, :196-198)
+        launch_finalize_dft(Gf, Gf, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 1, st);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    timer.collect();
+}
+
+// =============================================================================================
+// run_forward2_correlation (code/run_forward2_correlation.m)
+// =============================================================================================
+void correlation_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* G,
+                        const fd2d_carray* spectrum, const fd2d_carray* sd, const fd2d_carray* rec, double mode,
+                        const fd2d_carray* dmu, const fd2d_carray* C_in, fd2d_marray* seis, fd2d_marray* C_out,
+                        hipStream_t st)
+{
+    const Geometry g = geometry(cfg);
+    const int64_t P = (int64_t)g.P;
+    const int N = 2 * g.nt - 1;  // :68-69
+    const int nb = cfg->n_basis_fct, nb_cols = nb == 0 ? 1 : nb;
+    if (numel(mu) != P || numel(rho) != P) raise(FD2D_EINVAL, "mu and rho must have nx*nz = %lld elements", (long long)P);
+    if (numel(G) != P * g.K || g.K == 0) raise(FD2D_EINVAL, "G_fft must be nx x nz x n_sample");
+    const int64_t n_src = dim(spectrum, 1);
+    if (dim(spectrum, 0) != g.K || n_src < 1 || dim(spectrum, 2) != 1)
+        raise(FD2D_EINVAL, "spectrum must be n_sample x n_noise_sources");
+    if (nb == 0 && n_src != 1)
+        raise(FD2D_EINVAL, "with n_basis_fct = 0 the reference accepts one noise source (run_forward2_correlation.m:171)");
+    if (numel(sd) != P * (nb == 0 ? n_src : nb)) raise(FD2D_EINVAL, "source_dist must be nx x nz x %d", nb == 0 ? (int)n_src : nb);
+    const bool born = numel(dmu) != 0;
+    if (born && (numel(dmu) != P || numel(C_in) == 0 || C_in->dims[0] * C_in->dims[1] != P || dim(C_in, 2) < N))
+        raise(FD2D_EINVAL, "Born mode needs dmu (nx x nz) and C_in with 2nt-1 slices (run_forward2_correlation.m:236)");
+    const std::vector<int2> rid = node_ids(rec, g, "rec");
+    const int64_t nr = (int64_t)rid.size();
+    const int64_t n_fw = N / g.f;
+    if (mode != 0) check_out(C_out, g.nx, g.nz, n_fw, "C_out");
+    check_out(seis, nr, N, 1, "seismograms");
+    float* Cout = (mode != 0 && C_out && C_out->data) ? (float*)C_out->data : nullptr;
+    double* sg = (seis && seis->data) ? (double*)seis->data : nullptr;
+
+    Solver S(cfg, g, st, true);
+    S.material((const double*)mu->data, (const double*)rho->data);
+    DBuf<int2> rid_d;
+    upload(rid_d, rid, st);
+    if (sg && nr) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * N, st));
+
+    // source spectrum A = (sum_ns spectrum) * distribution .* G_fft (:168-173, :201-205)
+    const std::vector<double> spec = to_host<double>(spectrum->data, (size_t)g.K * n_src);
+    std::vector<double> spec_sum((size_t)g.K, 0.0);
+    for (int k = 0; k < g.K; ++k)
+        for (int64_t q = 0; q < n_src; ++q) spec_sum[k] += spec[(size_t)k + (size_t)g.K * q];
+    DBuf<double> I_d, ss_d;
+    upload(I_d, interpol_matrix(g.K, nb), st);
+    upload(ss_d, spec_sum, st);
+    DBuf<double2> A(g.P * (size_t)g.K);
+    launch_source_spectrum(A.p, (const double2*)G->data, 0, (const double*)sd->data, nb_cols, I_d.p, ss_d.p, g.P,
+                           g.K, st);
+    // ifft_coeff of :73-85 at t(n) = (n - nt) dt for n = s, 2s, ...
+    const int n_ftc = N / g.s;
+    std::vector<double> ts;
+    for (int q = 1; q <= n_ftc; ++q) ts.push_back((double)(q * g.s - g.nt) * g.dt);
+    Synthesiser syn;
+    syn.init(A.p, g.P, g.K, phase_table(cfg, ts, 1.0, kInvSqrt2Pi * g.dw), n_ftc, st);
+
+    DBuf<float> dmu_f;
+    if (born) {
+        dmu_f.alloc(g.P);
+        launch_scale_to_float((const double*)dmu->data, dmu_f.p, g.P, 1.0, st);
+    }
+    StepTimer timer(FD2D_CORRELATION, g.P, st);
+    int64_t i_fw_out = 0;
+    for (int n = 1; n <= N; ++n) {
+        S.clear_step();
+        if (mode != 0 && n % g.f == 0) {  // :182-186
+            if (Cout) S.a.snap = Cout + (size_t)i_fw_out * g.P;
+            ++i_fw_out;
+        }
+        if (n % g.s == 0) S.a.syn = syn.slice(n / g.s - 1);  // :195-216
+        if (born && n != N) {                                 // :232-238
+            S.a.born_dmu = dmu_f.p;
+            S.a.born_in0 = (const float*)C_in->data + (size_t)(n - 1) * g.P;
+            S.a.born_in1 = (const float*)C_in->data + (size_t)n * g.P;
+        }
+        if (sg && nr) {
+            S.a.nrec = (int)nr;
+            S.a.rec_ij = rid_d.p;
+            S.a.seis_col = sg + (size_t)nr * (n - 1);
+        }
+        timer.mark(n);
+        launch_velocity(g.R, S.a, st);
+        timer.mark(n);
+        launch_stress(g.R, S.a, st);
+        timer.mark(n);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    timer.collect();
+}
+
+// =============================================================================================
+// run_noise_adjoint (code/run_noise_adjoint.m)
+// =============================================================================================
+void adjoint_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* u_fwd,
+                    const fd2d_carray* stf, const fd2d_carray* src, const fd2d_carray* rec,
+                    const fd2d_carray* spectrum, const fd2d_carray* sd, const fd2d_carray* G, double mode,
+                    const fd2d_carray* dmu, const fd2d_carray* u_in, fd2d_marray* K_out, fd2d_marray* stf_fft,
+                    fd2d_marray* u_out, fd2d_marray* seis, hipStream_t st)
+{
+    const Geometry g = geometry(cfg);
+    const int64_t P = (int64_t)g.P;
+    const int N = 2 * g.nt - 1;  // :47-49
+    const int nb = cfg->n_basis_fct, nb_cols = nb == 0 ? 1 : nb;
+    if (numel(mu) != P || numel(rho) != P) raise(FD2D_EINVAL, "mu and rho must have nx*nz = %lld elements", (long long)P);
+    const int imode = (int)mode;
+    if ((double)imode != mode || !(imode == 0 || imode == 1 || (imode >= 10 && imode <= 13)))
+        raise(FD2D_EINVAL, "mode must be 0, 1, 10, 11, 12 or 13 (run_noise_adjoint.m:16-22)");
+    const bool fft_mode = imode == 1 || imode == 12 || imode == 13;
+    const bool save_mode = imode == 11 || imode == 13;
+
+    const int64_t n_stf = numel(stf);
+    const bool stf_point = n_stf != 0 && stf->dims[2] == 1;  // :249
+    const bool stf_dist = n_stf != 0 && stf->dims[2] != 1;   // :255
+    if (stf_dist && fft_mode)
+        raise(FD2D_EUNSUPPORTED, "a distributed stf with mode %d advances the reference's DFT counter twice per sample "
+                                 "(run_noise_adjoint.m:265,343)", imode);
+    if (stf_dist && (stf->dims[0] * stf->dims[1] != P || stf->dims[2] != g.K))
+        raise(FD2D_EINVAL, "a distributed stf must be nx x nz x n_sample");
+    const int nsteps = ((imode == 0 || imode == 1) && stf_dist) ? g.nt : N;  // :165-167
+
+    const std::vector<int2> sid = node_ids(src, g, "src");
+    const std::vector<int2> rid = node_ids(rec, g, "rec");
+    const int64_t nr = (int64_t)rid.size();
+    if (stf_point && (stf->dims[0] < (int64_t)sid.size() || stf->dims[1] < nsteps))
+        raise(FD2D_EINVAL, "a point stf must be n_src x (2nt-1)");
+
+    const bool have_sd = numel(sd) != 0;  // :112-123
+    const int64_t n_src = have_sd ? dim(spectrum, 1) : 0;
+    std::vector<double> spec;
+    if (numel(spectrum)) spec = to_host<double>(spectrum->data, (size_t)numel(spectrum));
+    if (have_sd) {
+        if (dim(spectrum, 0) != g.K) raise(FD2D_EINVAL, "spectrum must have n_sample rows");
+        if (nb == 0 && n_src != 1) raise(FD2D_EINVAL, "with n_basis_fct = 0 the reference accepts one noise source");
+        if (numel(sd) != P * (nb == 0 ? n_src : nb)) raise(FD2D_EINVAL, "source_dist size");
+    }
+    const bool src_kernel = numel(G) != 0 && dim(G, 2) == g.K && G->dims[0] * G->dims[1] == P && g.K > 0;  // :302
+    if (src_kernel && (dim(spectrum, 0) != g.K || dim(spectrum, 1) != 1))
+        raise(FD2D_EINVAL, "the source kernel needs an n_sample x 1 spectrum (run_noise_adjoint.m:312)");
+    const int64_t n_ufwd = numel(u_fwd) ? dim(u_fwd, 2) : 0;  // :324
+    if (n_ufwd && u_fwd->dims[0] * u_fwd->dims[1] != P) raise(FD2D_EINVAL, "u_fwd slices must have nx*nz elements");
+    const int64_t n_uin = numel(u_in) ? dim(u_in, 2) : 0;
+    const bool born = numel(dmu) != 0 && n_uin == N;  // :285
+    if (born && (numel(dmu) != P || u_in->dims[0] * u_in->dims[1] != P)) raise(FD2D_EINVAL, "dmu / u_in sizes");
+
+    const int64_t n_fw = N / g.f;
+    check_out(K_out, g.nx, g.nz, nb_cols + 1, "K");
+    if (fft_mode) check_out(stf_fft, g.nx, g.nz, g.K, "stf_fft");
+    if (save_mode) check_out(u_out, g.nx, g.nz, n_fw, "u_out");
+    check_out(seis, nr, N, 1, "seismograms");
+    double* Kd = (K_out && K_out->data) ? (double*)K_out->data : nullptr;
+    double2* sfft = (fft_mode && stf_fft && stf_fft->data && g.K > 0) ? (double2*)stf_fft->data : nullptr;
+    float* uo = (save_mode && u_out && u_out->data) ? (float*)u_out->data : nullptr;
+    double* sg = (seis && seis->data) ? (double*)seis->data : nullptr;
+
+    Solver S(cfg, g, st, true);
+    S.material((const double*)mu->data, (const double*)rho->data);
+    DBuf<int2> sid_d, rid_d;
+    upload(sid_d, sid, st);
+    upload(rid_d, rid, st);
+    if (sg && nr) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * N, st));
+    if (Kd) HIPCHK(hipMemsetAsync(Kd, 0, sizeof(double) * g.P * (size_t)(nb_cols + 1), st));
+
+    // point adjoint sources: real part of stf(i, n) (:249-253)
+    DBuf<float> amp;
+    if (stf_point && !sid.empty()) {
+        amp.alloc(sid.size() * (size_t)nsteps);
+        launch_stf_real(amp.p, (const double2*)stf->data, stf->dims[0], (int)sid.size(), nsteps, st);
+    }
+    // distributed adjoint source: A = (sum_ns spectrum) * distribution .* conj(stf) (:255-268)
+    const std::vector<double> I = interpol_matrix(g.K, nb);
+    DBuf<double> I_d, spec_d;
+    upload(I_d, I, st);
+    DBuf<double2> A;
+    Synthesiser syn;
+    bool dist_src = stf_dist && have_sd;
+    if (dist_src) {
+        std::vector<double> spec_sum((size_t)g.K, 0.0);
+        for (int k = 0; k < g.K; ++k)
+            for (int64_t q = 0; q < n_src; ++q) spec_sum[k] += spec[(size_t)k + (size_t)g.K * q];
+        DBuf<double> ss_d;
+        upload(ss_d, spec_sum, st);
+        A.alloc(g.P * (size_t)g.K);
+        launch_source_spectrum(A.p, (const double2*)stf->data, 1, (const double*)sd->data, nb_cols, I_d.p, ss_d.p,
+                               g.P, g.K, st);
+        HIPCHK(hipStreamSynchronize(st));  // ss_d goes out of scope
+        const int n_syn = nsteps / g.s;
+        std::vector<double> ts;
+        for (int q = 1; q <= n_syn; ++q) ts.push_back((double)(q * g.s - g.nt) * g.dt);
+        syn.init(A.p, g.P, g.K, phase_table(cfg, ts, 1.0, kInvSqrt2Pi * g.dw), n_syn, st);
+    }
+    // DFT of u at the sampled steps: stf_fft (:335-346) and the source kernel (:302-320) share it
+    const bool need_dft = (sfft != nullptr) || (src_kernel && Kd);
+    DBuf<double2> acc_tmp;
+    DftAccumulator dft;
+    if (need_dft) {
+        double2* acc = sfft;
+        if (!acc) {
+            acc_tmp.alloc(g.P * (size_t)g.K);
+            acc = acc_tmp.p;
+        }
+        std::vector<double> ts;
+        for (int n = g.s; n <= nsteps; n += g.s) ts.push_back((double)(n - g.nt) * g.dt);
+        dft.init(acc, g.P, g.K, phase_table(cfg, ts, -1.0, 1.0), ts.size(), st);
+    }
+    DBuf<float> dmu_f;
+    if (born) {
+        dmu_f.alloc(g.P);
+        launch_scale_to_float((const double*)dmu->data, dmu_f.p, g.P, 1.0, st);
+    }
+
+    StepTimer timer(FD2D_ADJOINT, g.P, st);
+    int64_t i_fw_in = 0, i_fw_out = 0;
+    for (int n = 1; n <= nsteps; ++n) {
+        S.clear_step();
+        if (save_mode && n % g.f == 0) {  // :237-241
+            if (uo) S.a.snap = uo + (size_t)i_fw_out * g.P;
+            ++i_fw_out;
+        }
+        if (stf_point && !sid.empty()) {
+            S.a.nps = (int)sid.size();
+            S.a.ps_ij = sid_d.p;
+            S.a.ps_amp = amp.p + (size_t)(n - 1) * sid.size();
+        } else if (dist_src && n % g.s == 0) {
+            S.a.syn = syn.slice(n / g.s - 1);
+        }
+        if (born && n != N) {  // :285-291
+            S.a.born_dmu = dmu_f.p;
+            S.a.born_in0 = (const float*)u_in->data + (size_t)(n - 1) * g.P;
+            S.a.born_in1 = (const float*)u_in->data + (size_t)n * g.P;
+        }
+        if (Kd && n_ufwd && n_ufwd >= i_fw_in + 1 && n % g.f == 0) {  // :324-331
+            S.a.kmu_fwd = (const float*)u_fwd->data + (size_t)(n_ufwd - 1 - i_fw_in) * g.P;
+            S.a.kmu = Kd + (size_t)nb_cols * g.P;
+            S.a.kmu_scale = (double)g.f;
+            ++i_fw_in;
+        }
+        if (need_dft && n % g.s == 0) S.a.dsamp = dft.next_slot();
+        if (sg && nr) {
+            S.a.nrec = (int)nr;
+            S.a.rec_ij = rid_d.p;
+            S.a.seis_col = sg + (size_t)nr * (n - 1);
+        }
+        timer.mark(n);
+        launch_velocity(g.R, S.a, st);
+        timer.mark(n);
+        launch_stress(g.R, S.a, st);
+        timer.mark(n);
+        if (S.a.dsamp) dft.commit();
+    }
+    if (need_dft) {
+        dft.flush();
+        if (src_kernel && Kd) {
+            // K_s(:,:,b) = sum_n Re(M_b(n) u(n)) = (2pi)^-1/2 dw sum_k I(k,b) spec_k Re(conj(G_k D_k)),
+            // D_k = sum_n u(n) e^{-i w_k t_n}
+            DBuf<double> sp_d;
+            upload(sp_d, std::vector<double>(spec.begin(), spec.begin() + g.K), st);
+            launch_source_kernel(Kd, (const double2*)G->data, dft.acc, I_d.p, sp_d.p, g.P, g.K, nb_cols,
+                                 kInvSqrt2Pi * g.dw, st);
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        if (sfft) launch_finalize_dft(sfft, sfft, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 0, st);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    timer.collect();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host-memory wrappers: copy every array in, run on the null stream, copy outputs back.
+// ---------------------------------------------------------------------------------------------
+struct Staged {
+    std::vector<std::unique_ptr<DBuf<char>>> bufs;
+    fd2d_carray in(const fd2d_carray* h, size_t esize)
+    {
+        fd2d_carray d{nullptr, {0, 0, 1}};
+        if (!h) return d;
+        d.dims[0] = h->dims[0];
+        d.dims[1] = h->dims[1];
+        d.dims[2] = h->dims[2];
+        const int64_t n = numel(h);
+        if (n == 0) return d;
+        bufs.emplace_back(new DBuf<char>((size_t)n * esize));
+        HIPCHK(hipMemcpy(bufs.back()->p, h->data, (size_t)n * esize, hipMemcpyHostToDevice));
+        d.data = bufs.back()->p;
+        return d;
+    }
+    fd2d_marray out(const fd2d_marray* h, size_t esize)
+    {
+        fd2d_marray d{nullptr, {0, 0, 1}};
+        if (!h || !h->data) return d;
+        d.dims[0] = h->dims[0];
+        d.dims[1] = h->dims[1];
+        d.dims[2] = h->dims[2];
+        const int64_t n = numel(h);
+        if (n == 0) return d;
+        bufs.emplace_back(new DBuf<char>((size_t)n * esize));
+        d.data = bufs.back()->p;
+        return d;
+    }
+    static void back(fd2d_marray* h, const fd2d_marray& d, size_t esize)
+    {
+        if (!h || !h->data || !d.data) return;
+        HIPCHK(hipMemcpy(h->data, d.data, (size_t)numel(h) * esize, hipMemcpyDeviceToHost));
+    }
+};
+
+const fd2d_carray* ptr_or_null(const fd2d_carray& a) { return a.data ? &a : nullptr; }
+
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        g_err.clear();
+        f();
+        return FD2D_OK;
+    } catch (const Failure& e) {
+        (void)hipGetLastError();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "host allocation failed";
+        return FD2D_ENOMEM;
+    } catch (...) {
+        g_err = "unexpected C++ exception";
+        return FD2D_EDEVICE;
+    }
+}
+
+void require_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        (void)hipGetLastError();
+        raise(FD2D_EDEVICE, "no HIP device is visible");
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fd2d_last_error(void) { return g_err.c_str(); }
+
+int fd2d_timing_enable(int32_t every)
+{
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_timing_every = every > 0 ? every : 0;
+    g_timing = fd2d_timing{};
+    return FD2D_OK;
+}
+
+int fd2d_timing_read(fd2d_timing* out)
+{
+    if (!out) return FD2D_EINVAL;
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    *out = g_timing;
+    g_timing = fd2d_timing{};
+    return FD2D_OK;
+}
+const char* fd2d_version(void) { return "fd2d_noise_amd 0.1 (gfx950)"; }
+
+int fd2d_run_forward1_green_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho,
+                                   const fd2d_carray* src, const fd2d_carray* rec, double mode, const fd2d_carray* dmu,
+                                   const fd2d_carray* G_in, fd2d_marray* G_fft, fd2d_marray* G_out,
+                                   fd2d_marray* seismograms, void* stream)
+{
+    return guarded([&] {
+        require_device();
+        green_device(cfg, mu, rho, src, rec, mode, dmu, G_in, G_fft, G_out, seismograms, as_stream(stream));
+    });
+}
+
+int fd2d_run_forward2_correlation_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho,
+                                         const fd2d_carray* G_fft, const fd2d_carray* spectrum,
+                                         const fd2d_carray* source_dist, const fd2d_carray* rec, double mode,
+                                         const fd2d_carray* dmu, const fd2d_carray* C_in, fd2d_marray* seismograms,
+                                         fd2d_marray* C_out, void* stream)
+{
+    return guarded([&] {
+        require_device();
+        correlation_device(cfg, mu, rho, G_fft, spectrum, source_dist, rec, mode, dmu, C_in, seismograms, C_out,
+                           as_stream(stream));
+    });
+}
+
+int fd2d_run_noise_adjoint_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho,
+                                  const fd2d_carray* u_fwd, const fd2d_carray* stf, const fd2d_carray* src,
+                                  const fd2d_carray* rec, const fd2d_carray* spectrum, const fd2d_carray* source_dist,
+                                  const fd2d_carray* G_fft, double mode, const fd2d_carray* dmu,
+                                  const fd2d_carray* u_in, fd2d_marray* K, fd2d_marray* stf_fft, fd2d_marray* u_out,
+                                  fd2d_marray* seismograms, void* stream)
+{
+    return guarded([&] {
+        require_device();
+        adjoint_device(cfg, mu, rho, u_fwd, stf, src, rec, spectrum, source_dist, G_fft, mode, dmu, u_in, K, stf_fft,
+                       u_out, seismograms, as_stream(stream));
+    });
+}
+
+int fd2d_run_forward1_green(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho,
+                            const fd2d_carray* src, const fd2d_carray* rec, double mode, const fd2d_carray* dmu,
+                            const fd2d_carray* G_in, fd2d_marray* G_fft, fd2d_marray* G_out, fd2d_marray* seismograms)
+{
+    return guarded([&] {
+        require_device();
+        Staged s;
+        fd2d_carray dmu_d = s.in(dmu, 8), mu_d = s.in(mu, 8), rho_d = s.in(rho, 8), src_d = s.in(src, 8),
+                    rec_d = s.in(rec, 8), gin_d = s.in(G_in, 4);
+        fd2d_marray gf = s.out(G_fft, 16), go = s.out(G_out, 4), sg = s.out(seismograms, 8);
+        green_device(cfg, &mu_d, &rho_d, ptr_or_null(src_d), ptr_or_null(rec_d), mode, ptr_or_null(dmu_d),
+                     ptr_or_null(gin_d), &gf, &go, &sg, nullptr);
+        Staged::back(G_fft, gf, 16);
+        if (mode != 0) Staged::back(G_out, go, 4);
+        Staged::back(seismograms, sg, 8);
+    });
+}
+
+int fd2d_run_forward2_correlation(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho,
+                                  const fd2d_carray* G_fft, const fd2d_carray* spectrum,
+                                  const fd2d_carray* source_dist, const fd2d_carray* rec, double mode,
+                                  const fd2d_carray* dmu, const fd2d_carray* C_in, fd2d_marray* seismograms,
+                                  fd2d_marray* C_out)
+{
+    return guarded([&] {
+        require_device();
+        Staged s;
+        fd2d_carray mu_d = s.in(mu, 8), rho_d = s.in(rho, 8), g_d = s.in(G_fft, 16), sp_d = s.in(spectrum, 8),
+                    sd_d = s.in(source_dist, 8), rec_d = s.in(rec, 8), dmu_d = s.in(dmu, 8), cin_d = s.in(C_in, 4);
+        fd2d_marray sg = s.out(seismograms, 8), co = s.out(C_out, 4);
+        correlation_device(cfg, &mu_d, &rho_d, &g_d, &sp_d, &sd_d, ptr_or_null(rec_d), mode, ptr_or_null(dmu_d),
+                           ptr_or_null(cin_d), &sg, &co, nullptr);
+        Staged::back(seismograms, sg, 8);
+        if (mode != 0) Staged::back(C_out, co, 4);
+    });
+}
+
+int fd2d_run_noise_adjoint(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho,
+                           const fd2d_carray* u_fwd, const fd2d_carray* stf, const fd2d_carray* src,
+                           const fd2d_carray* rec, const fd2d_carray* spectrum, const fd2d_carray* source_dist,
+                           const fd2d_carray* G_fft, double mode, const fd2d_carray* dmu, const fd2d_carray* u_in,
+                           fd2d_marray* K, fd2d_marray* stf_fft, fd2d_marray* u_out, fd2d_marray* seismograms)
+{
+    return guarded([&] {
+        require_device();
+        Staged s;
+        fd2d_carray mu_d = s.in(mu, 8), rho_d = s.in(rho, 8), uf_d = s.in(u_fwd, 4), stf_d = s.in(stf, 16),
+                    src_d = s.in(src, 8), rec_d = s.in(rec, 8), sp_d = s.in(spectrum, 8), sd_d = s.in(source_dist, 8),
+                    g_d = s.in(G_fft, 16), dmu_d = s.in(dmu, 8), uin_d = s.in(u_in, 4);
+        fd2d_marray kk = s.out(K, 8), sf = s.out(stf_fft, 16), uo = s.out(u_out, 4), sg = s.out(seismograms, 8);
+        adjoint_device(cfg, &mu_d, &rho_d, ptr_or_null(uf_d), ptr_or_null(stf_d), ptr_or_null(src_d),
+                       ptr_or_null(rec_d), ptr_or_null(sp_d), ptr_or_null(sd_d), ptr_or_null(g_d), mode,
+                       ptr_or_null(dmu_d), ptr_or_null(uin_d), &kk, &sf, &uo, &sg, nullptr);
+        const int im = (int)mode;
+        Staged::back(K, kk, 8);
+        if (im == 1 || im == 12 || im == 13) Staged::back(stf_fft, sf, 16);
+        if (im == 11 || im == 13) Staged::back(u_out, uo, 4);
+        Staged::back(seismograms, sg, 8);
+    });
+}
+
+int fd2d_query_output_dims(const fd2d_config* cfg, int32_t solver, double mode, int64_t n_rec, int64_t dims[4][3])
+{
+    int count = 0;
+    const int st = guarded([&] {
+        const Geometry g = geometry(cfg);
+        const int64_t N = 2 * (int64_t)g.nt - 1;
+        for (int q = 0; q < 4; ++q) dims[q][0] = dims[q][1] = dims[q][2] = 0;
+        auto set = [&](int q, int64_t a, int64_t b, int64_t c) {
+            dims[q][0] = a;
+            dims[q][1] = b;
+            dims[q][2] = c;
+        };
+        if (solver == FD2D_GREEN) {
+            set(0, g.nx, g.nz, g.K);
+            if (mode != 0) set(1, g.nx, g.nz, green_n_fw(g));
+            set(2, n_rec, g.nt, 1);
+            count = 3;
+        } else if (solver == FD2D_CORRELATION) {
+            set(0, n_rec, N, 1);
+            if (mode != 0) set(1, g.nx, g.nz, N / g.f);
+            count = 2;
+        } else if (solver == FD2D_ADJOINT) {
+            const int im = (int)mode;
+            const int nb = cfg->n_basis_fct;
+            set(0, g.nx, g.nz, (nb == 0 ? 1 : nb) + 1);
+            if (im == 1 || im == 12 || im == 13) set(1, g.nx, g.nz, g.K);
+            if (im == 11 || im == 13) set(2, g.nx, g.nz, N / g.f);
+            set(3, n_rec, N, 1);
+            count = 4;
+        } else {
+            raise(FD2D_EINVAL, "unknown solver %d", solver);
+        }
+    });
+    return st == FD2D_OK ? count : -st;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+"""Seeded synthetic inputs of the benchmark and parity workloads.
+
+config1: the reference-sized CPU case -- 2-D SH on 2000 x 2000 km, 300 x 300 nodes, order 4,
+         dt = 0.5 s, nt = 2400, homogeneous rho = 3000 kg/m^3, mu = 4.8e10 Pa (4 km/s), a 30-node
+         taper on all four sides, Gaussian noise spectrum centred at 0.05 Hz (sigma 0.02 Hz) on 91
+         frequencies in 0.01-0.10 Hz, a Gaussian source blob (sigma 200 km) at a seeded location
+         plus a 0.1 background, the reference station at the centre and 2 receivers 300-700 km away.
+config2: the single large solve -- 2400 x 2400 nodes on 2000 km (dx = 0.834 km), order 8,
+         dt = 0.06 s, nt = 8000, vs = 4 km/s times (1 + 0.1 g) with g a seeded Gaussian random field
+         (30 km correlation length, max |g| = 1), rho constant, forward fields stored every 4th step,
+         the default 121 frequencies (0.02:0.0015:0.2 Hz), 16 receivers 400-800 km from the centre
+         station and a cross-correlation traveltime misfit against the homogeneous model's traces.
+The taper is the reference's Gaussian taper (init_absbound.m:18).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .config import Config, matlab_colon
+
+MU0, RHO0 = 4.8e10, 3000.0
+
+
+def gaussian_random_field(rng, nx, nz, dx, corr_len):
+    """Zero-mean field with Gaussian correlation of length corr_len, normalised to max |g| = 1."""
+    w = rng.standard_normal((nx, nz))
+    kx = 2 * np.pi * np.fft.rfftfreq(nz, d=dx)[None, :]
+    kz = 2 * np.pi * np.fft.fftfreq(nx, d=dx)[:, None]
+    filt = np.exp(-(kx ** 2 + kz ** 2) * corr_len ** 2 / 8.0)
+    g = np.fft.irfft2(np.fft.rfft2(w) * filt, s=(nx, nz))
+    g -= g.mean()
+    return g / np.abs(g).max()
+
+
+def _ring(rng, centre, n, rmin, rmax):
+    ang = rng.uniform(0, 2 * np.pi, n)
+    r = rng.uniform(rmin, rmax, n)
+    return np.stack([centre[0] + r * np.cos(ang), centre[1] + r * np.sin(ang)], axis=1)
+
+
+def _blob(rng, c: Config, sigma=200e3, background=0.1):
+    x0 = rng.uniform(0.2, 0.8) * c.Lx
+    z0 = rng.uniform(0.2, 0.8) * c.Lz
+    X = c.x()[:, None]
+    Z = c.z()[None, :]
+    return np.asfortranarray(background + np.exp(-((X - x0) ** 2 + (Z - z0) ** 2) / (2 * sigma ** 2)))
+
+
+def config1_workload(seed: int = 1) -> dict:
+    rng = np.random.default_rng(seed)
+    c = Config(Lx=2.0e6, Lz=2.0e6, nx=300, nz=300, dt=0.5, nt=2400, order=4, fw_nth=1,
+               f_sample=np.linspace(0.01, 0.10, 91), freq_samp=5)
+    c.absorb_width = 30 * c.dx
+    f = np.asarray(c.f_sample)
+    centre = np.array([c.Lx / 2, c.Lz / 2])
+    return dict(
+        name="config1", config=c,
+        mu=MU0 * np.ones((c.nx, c.nz), order="F"), rho=RHO0 * np.ones((c.nx, c.nz), order="F"),
+        src=centre[None, :], rec=_ring(rng, centre, 2, 300e3, 700e3),
+        spectrum=np.exp(-((f - 0.05) ** 2) / (2 * 0.02 ** 2))[:, None],
+        source_dist=_blob(rng, c))
+
+
+def config2_workload(seed: int = 2, nt: int = 8000) -> dict:
+    rng = np.random.default_rng(seed)
+    c = Config(Lx=2.0e6, Lz=2.0e6, nx=2400, nz=2400, dt=0.06, nt=nt, order=8, fw_nth=4,
+               f_sample=matlab_colon(0.02, 0.0015, 0.2), freq_samp=5)
+    c.absorb_width = 30 * c.dx
+    g = gaussian_random_field(rng, c.nx, c.nz, c.dx, 30e3)
+    vs = 4000.0 * (1 + 0.1 * g)
+    rho = RHO0 * np.ones((c.nx, c.nz), order="F")
+    f = np.asarray(c.f_sample)
+    centre = np.array([c.Lx / 2, c.Lz / 2])
+    return dict(
+        name="config2", config=c,
+        mu=np.asfortranarray(rho * vs ** 2), rho=rho,
+        mu_obs=MU0 * np.ones((c.nx, c.nz), order="F"),
+        src=centre[None, :], rec=_ring(rng, centre, 16, 400e3, 800e3),
+        spectrum=np.exp(-((f - 0.1) ** 2) / (2 * 0.03 ** 2))[:, None],
+        source_dist=_blob(rng, c))

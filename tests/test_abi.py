@@ -1,0 +1,72 @@
+"""The C ABI library loads, exports every entry point include/fd2d_noise.h declares, and its
+pure-host logic (output sizes, the config struct layout) matches the reference's rules."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from fd2d_noise_amd import Config, _lib, abi
+from fd2d_noise_amd.config import CConfig
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(fd2d_\w+)\s*\(", text, flags=re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert {"fd2d_run_forward1_green", "fd2d_run_forward2_correlation", "fd2d_run_noise_adjoint"} <= names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(_lib.EXPORTS) == names
+    assert "gfx950" in _lib.version()
+
+
+def test_config_struct_layout():
+    # offsets of the C struct on x86-64 (natural alignment)
+    assert CConfig.Lx.offset == 0 and CConfig.nx.offset == 16 and CConfig.dt.offset == 24
+    assert CConfig.nt.offset == 32 and CConfig.n_sample.offset == 48 and CConfig.w_sample.offset == 56
+    assert CConfig.dw.offset == 64 and CConfig.absorb_width.offset == 72 and CConfig.absorb_bottom.offset == 92
+    assert ctypes.sizeof(CConfig) == 96
+
+
+def query(c, solver, mode, nr):
+    dims = (ctypes.c_int64 * 3 * 4)()
+    cc = c.to_c()
+    n = _lib.load().fd2d_query_output_dims(ctypes.byref(cc), solver, float(mode), nr,
+                                            ctypes.cast(dims, ctypes.POINTER(ctypes.c_int64 * 3)))
+    return n, [tuple(d) for d in dims]
+
+
+def test_query_output_dims_follow_reference_rules():
+    c = Config(nt=150, fw_nth=4)  # tiny setup with fw_nth = 4
+    n, d = query(c, abi.FD2D_GREEN, 1, 3)
+    # run_forward1_green.m:98-103: count of n in nt..2nt-1 with mod(n, fw_nth) == 0
+    n_fw = sum(1 for q in range(150, 300) if q % 4 == 0)
+    assert n == 3 and d[0] == (50, 50, 121) and d[1] == (50, 50, n_fw) and d[2] == (3, 150, 1)
+    n, d = query(c, abi.FD2D_GREEN, 0, 3)
+    assert d[1] == (0, 0, 0)
+    n, d = query(c, abi.FD2D_CORRELATION, 1, 2)  # floor((2nt-1)/fw_nth), run_forward2_correlation.m:94
+    assert n == 2 and d[0] == (2, 299, 1) and d[1] == (50, 50, 299 // 4)
+    n, d = query(c.replace(n_basis_fct=4), abi.FD2D_ADJOINT, 13, 2)
+    assert n == 4 and d[0] == (50, 50, 5) and d[1] == (50, 50, 121) and d[2] == (50, 50, 299 // 4)
+    n, d = query(c, abi.FD2D_ADJOINT, 0, 2)
+    assert d[0] == (50, 50, 2) and d[1] == (0, 0, 0) and d[2] == (0, 0, 0)
+    n, _ = query(c.replace(order=5), abi.FD2D_GREEN, 0, 1)
+    assert n == -abi.FD2D_EUNSUPPORTED
+    assert "order 5" in _lib.last_error()
+
+
+def test_matlab_dims():
+    assert abi.matlab_dims((5,)) == (5, 1, 1)
+    assert abi.matlab_dims((5, 2)) == (5, 2, 1)
+    assert abi.matlab_dims(np.zeros((2, 3, 4)).shape) == (2, 3, 4)
