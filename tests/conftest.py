@@ -24,5 +24,7 @@ def rel_l2(a, b):
 @pytest.fixture(scope="session")
 def orc():
     import oracle
-    oracle.oracle.build()
+    if not os.path.exists(oracle.oracle.LIB_PATH):
+        oracle.oracle.build()
+    oracle.set_threads(min(16, os.cpu_count() or 1))
     return oracle.solvers()
