@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--cpu-nt", type=int, default=16, help="nt of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-every", type=int, default=8)
+    ap.add_argument("--only", choices=["green", "correlation", "adjoint1", "adjoint0", "copy"], default=None,
+                    help="profiling aid: time one solver of the pipeline alone (or a 1 GiB device copy "
+                         "used to calibrate HBM byte counters); the default is the whole pipeline")
     return ap.parse_args()
 
 
@@ -111,6 +114,41 @@ class Pipeline:
         c = self.c
         return c.nx * c.nz * (c.nt + c.n_corr + c.n_corr + c.nt)
 
+    # ---- single-solver runs (profiling aid) ----
+    def prepare_only(self, which):
+        D, c = self.D, self.c
+        if which in ("correlation", "adjoint1", "adjoint0"):
+            D.run_forward1_green(self.mu, self.rho, self.src, None, 1, None, None, config=c, G_fft=self.G_fft,
+                                 G_out=self.G_out)
+        if which in ("adjoint1", "adjoint0"):
+            D.run_forward2_correlation(self.mu, self.rho, self.G_fft, self.spec, self.sd, self.rec, 1, None, None,
+                                       config=c, seismograms=self.corr, C_out=self.C_out)
+            from fd2d_noise_amd.measurement import make_adjoint_sources
+            _, adstf = make_adjoint_sources(D.to_host(self.corr), self.obs, None, self.t, "dis", "cc_time_shift",
+                                            window=self.window)
+            self.stf = D.to_device(adstf.astype(np.complex128))
+        if which == "adjoint0":
+            D.run_noise_adjoint(self.mu, self.rho, self.C_out, self.stf, self.rec, self.rec, None, None, None, 1, None,
+                                None, config=c, K=self.K1, stf_fft=self.stf_fft, seismograms=self.seis_a)
+
+    def run_only(self, which):
+        D, c = self.D, self.c
+        if which == "green":
+            D.run_forward1_green(self.mu, self.rho, self.src, None, 1, None, None, config=c, G_fft=self.G_fft,
+                                 G_out=self.G_out)
+            return c.nx * c.nz * c.nt
+        if which == "correlation":
+            D.run_forward2_correlation(self.mu, self.rho, self.G_fft, self.spec, self.sd, self.rec, 1, None, None,
+                                       config=c, seismograms=self.corr, C_out=self.C_out)
+            return c.nx * c.nz * c.n_corr
+        if which == "adjoint1":
+            D.run_noise_adjoint(self.mu, self.rho, self.C_out, self.stf, self.rec, self.rec, None, None, None, 1, None,
+                                None, config=c, K=self.K1, stf_fft=self.stf_fft, seismograms=self.seis_a)
+            return c.nx * c.nz * c.n_corr
+        D.run_noise_adjoint(self.mu, self.rho, self.G_out, self.stf_fft, self.rec, self.rec, self.spec, self.sd, None, 0,
+                            None, None, config=c, K=self.K2)
+        return c.nx * c.nz * c.nt
+
 
 def cpu_baseline(w, nt_sample, threads):
     """The FP64 oracle on the same chain at the full grid, truncated to nt_sample Green steps."""
@@ -150,6 +188,20 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.only == "copy":
+        n = 1 << 28  # 1 GiB of FP32 read + 1 GiB written per copy
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(args.warmup):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        print(json.dumps({"copy_bytes_per_launch": 2 * 4 * n, "GBps": 2 * 4 * n * args.steps / el / 1e9}))
+        return
     w = config2_workload(nt=args.nt)
     c = w["config"]
     print(f"# workload config2 {c.nx}x{c.nz} order {c.order} nt {c.nt} fw_nth {c.fw_nth} K {c.n_sample}",
@@ -157,6 +209,30 @@ def main():
     p = Pipeline(w)
     p.observed()
     print("# observed traces ready", file=sys.stderr, flush=True)
+    if args.only:
+        p.prepare_only(args.only)
+        torch.cuda.synchronize()
+        for _ in range(args.warmup):
+            p.run_only(args.only)
+        _lib.timing_enable(args.timing_every)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gp = 0
+        for _ in range(args.steps):
+            gp += p.run_only(args.only)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        tim = _lib.timing_read()
+        _lib.timing_enable(0)
+        name = {"green": "green", "correlation": "correlation"}.get(args.only, "adjoint")
+        d = tim[name]
+        per = d["velocity_ms"] / max(d["launches"], 1)
+        ab = alg_bytes(name, c.fw_nth)
+        print(json.dumps({"only": args.only, "step_kernel": os.environ.get("FD2D_STEP_KERNEL", "tile"),
+                          "GP_updates_per_s": gp / el, "avg_step_us": per * 1e3,
+                          "achieved_GBps": ab * c.nx * c.nz / (per * 1e-3) / 1e9, "alg_bytes_per_gp": ab,
+                          "nt": c.nt}), flush=True)
+        return
     for i in range(args.warmup):
         p.step()
         print(f"# warmup step {i} done", file=sys.stderr, flush=True)
@@ -174,6 +250,10 @@ def main():
     el = time.perf_counter() - t0
     tim = _lib.timing_read()
     _lib.timing_enable(0)
+    for name, d in tim.items():
+        if d["launches"]:
+            print(f"# {name}: step kernel {d['velocity_ms'] / d['launches'] * 1e3:.1f} us over {d['launches']} timed steps",
+                  file=sys.stderr, flush=True)
 
     gp = p.gp_updates() * args.steps
     # dominant step-kernel pair: the solver with the most measured kernel time per solve
@@ -190,10 +270,8 @@ def main():
     achieved = bytes_step / (per_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"{best} step: k_velocity + k_stress (order {c.order})",
-            "avg_step_us": round(per_ms * 1e3, 2),
-            "avg_velocity_us": round(d["velocity_ms"] / d["launches"] * 1e3, 2),
-            "avg_stress_us": round(d["stress_ms"] / d["launches"] * 1e3, 2),
+            "kernel": f"k_tile<{c.order // 2},{0 if best == 'green' else 1}> ({best} run, fused time step)",
+            "avg_launch_us": round(per_ms * 1e3, 2),
             "alg_bytes_per_gp": alg_bytes(best, c.fw_nth)}
     cpu = None
     if not args.no_cpu:

@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -106,6 +108,7 @@ Geometry geometry(const fd2d_config* cfg)
     g.nx = cfg->nx;
     g.nz = cfg->nz;
     if (g.nx < 3 || g.nz < 3) raise(FD2D_EINVAL, "nx and nz must be >= 3 (got %d x %d)", g.nx, g.nz);
+    if (g.nx > kMaxDim || g.nz > kMaxDim) raise(FD2D_EUNSUPPORTED, "nx and nz must be <= %d", kMaxDim);
     if (cfg->order != 2 && cfg->order != 4 && cfg->order != 6 && cfg->order != 8)
         raise(FD2D_EUNSUPPORTED, "FD order %d is not one of 2, 4, 6, 8", cfg->order);
     g.R = cfg->order / 2;
@@ -257,29 +260,38 @@ struct Solver {
     const fd2d_config* cfg;
     Geometry g;
     hipStream_t st;
+    bool with_u;
     PaddedLayout lay;
-    DBuf<float> fields;  // v, sxy, szy, u
+    DBuf<float> fields;  // two sets of v, sxy, szy (, u)
     DBuf<float> coefs;   // bv, dtmu
     DBuf<float> ax_d, az_d;
+    float *v[2], *sxy[2], *szy[2], *u[2];
+    int cur = 0;
+    int tiles_x = 0, tiles_z = 0, n_tiles = 0;
+    DBuf<int> ps_off_d, rec_off_d;
+    DBuf<int2> ps_ent_d, rec_ent_d;
     StepArgs a{};
 
-    Solver(const fd2d_config* c, const Geometry& geo, hipStream_t s, bool with_u) : cfg(c), g(geo), st(s)
+    Solver(const fd2d_config* c, const Geometry& geo, hipStream_t s, bool with_u_) : cfg(c), g(geo), st(s), with_u(with_u_)
     {
         lay = PaddedLayout::make(g.nx, g.nz);
         const size_t E = lay.elems(), o = lay.origin();
         const int nf = with_u ? 4 : 3;
-        fields.alloc(E * nf);
-        HIPCHK(hipMemsetAsync(fields.p, 0, E * nf * sizeof(float), st));
+        fields.alloc(E * nf * 2);
+        HIPCHK(hipMemsetAsync(fields.p, 0, E * nf * 2 * sizeof(float), st));
         coefs.alloc(E * 2);
         HIPCHK(hipMemsetAsync(coefs.p, 0, E * 2 * sizeof(float), st));
         std::vector<float> ax, az;
         taper(cfg, g, ax, az);
         upload(ax_d, ax, st);
         upload(az_d, az, st);
-        a.v = fields.p + o;
-        a.sxy = fields.p + E + o;
-        a.szy = fields.p + 2 * E + o;
-        a.u = with_u ? fields.p + 3 * E + o : nullptr;
+        for (int b = 0; b < 2; ++b) {
+            float* base = fields.p + (size_t)b * nf * E;
+            v[b] = base + o;
+            sxy[b] = base + E + o;
+            szy[b] = base + 2 * E + o;
+            u[b] = with_u ? base + 3 * E + o : nullptr;
+        }
         a.bv = coefs.p + o;
         a.dtmu = coefs.p + E + o;
         a.ax = ax_d.p;
@@ -287,6 +299,14 @@ struct Solver {
         a.nx = g.nx;
         a.nz = g.nz;
         a.pitch = lay.pitch;
+        tiles_x = (g.nx + kTX - 1) / kTX;
+        tiles_z = (g.nz + kTileZ - 1) / kTileZ;
+        n_tiles = tiles_x * tiles_z;
+        a.tiles_x = tiles_x;
+        a.n_tiles = n_tiles;
+        a.tiles_per_xcd = (n_tiles + kXCD - 1) / kXCD;
+        a.origin = (int)o;
+        a.padded_bytes = (unsigned)(E * sizeof(float));
         for (int m = 0; m < 4; ++m) {
             a.cx[m] = (float)(kCoef[g.R][m] / g.dx);
             a.cz[m] = (float)(kCoef[g.R][m] / g.dz);
@@ -298,27 +318,95 @@ struct Solver {
         launch_material(mu, rho, coefs.p + lay.origin(), coefs.p + lay.elems() + lay.origin(), g.nx, g.nz,
                         lay.pitch, g.dt, st);
     }
+    // Point sources as per-tile lists over every node whose velocity the tile computes (the tile
+    // plus its (R-1, R) halo), in ascending source order so every copy of a halo node sums
+    // identically.
+    void set_sources(const std::vector<int2>& nodes)
+    {
+        if (nodes.empty()) return;
+        std::vector<std::vector<int2>> per(n_tiles);
+        for (size_t q = 0; q < nodes.size(); ++q) {
+            const int x = nodes[q].x, z = nodes[q].y;
+            for (int t = 0; t < n_tiles; ++t) {
+                const int i0 = (t % tiles_x) * kTX, j0 = (t / tiles_x) * kTileZ;
+                if (x < i0 - (g.R - 1) || x > i0 + kTX - 1 + g.R) continue;
+                if (z < j0 - (g.R - 1) || z > j0 + kTileZ - 1 + g.R) continue;
+                per[t].push_back(make_int2(node_key_host(x, z), (int)q));
+            }
+        }
+        csr(per, ps_off_d, ps_ent_d);
+    }
+    // Receivers as per-tile lists over its own nodes.
+    void set_receivers(const std::vector<int2>& nodes)
+    {
+        if (nodes.empty()) return;
+        std::vector<std::vector<int2>> per(n_tiles);
+        for (size_t q = 0; q < nodes.size(); ++q) {
+            const int x = nodes[q].x, z = nodes[q].y;
+            per[(z / kTileZ) * tiles_x + x / kTX].push_back(make_int2(node_key_host(x, z), (int)q));
+        }
+        csr(per, rec_off_d, rec_ent_d);
+    }
+    void csr(const std::vector<std::vector<int2>>& per, DBuf<int>& off_d, DBuf<int2>& ent_d)
+    {
+        std::vector<int> off(per.size() + 1, 0);
+        std::vector<int2> ent;
+        for (size_t t = 0; t < per.size(); ++t) {
+            ent.insert(ent.end(), per[t].begin(), per[t].end());
+            off[t + 1] = (int)ent.size();
+        }
+        upload(off_d, off, st);
+        upload(ent_d, ent, st);
+    }
     void clear_step()
     {
-        a.nps = 0;
-        a.ps_ij = nullptr;
+        a.v_in = v[cur];
+        a.sxy_in = sxy[cur];
+        a.szy_in = szy[cur];
+        a.u_in = u[cur];
+        a.v_out = v[1 - cur];
+        a.sxy_out = sxy[1 - cur];
+        a.szy_out = szy[1 - cur];
+        a.u_out = u[1 - cur];
+        a.ps_off = nullptr;
+        a.ps_ent = nullptr;
         a.ps_amp = nullptr;
         a.ps_const = 0.f;
         a.syn = nullptr;
         a.snap = nullptr;
         a.dsamp = nullptr;
-        a.nrec = 0;
-        a.rec_ij = nullptr;
+        a.rec_off = nullptr;
+        a.rec_ent = nullptr;
         a.seis_col = nullptr;
         a.born_dmu = a.born_in0 = a.born_in1 = nullptr;
-        a.kmu_fwd = nullptr;
-        a.kmu = nullptr;
-        a.kmu_scale = 0.0;
     }
+    void point_sources(const float* amp, float cst)
+    {
+        a.ps_off = ps_off_d.p;
+        a.ps_ent = ps_ent_d.p;
+        a.ps_amp = amp;
+        a.ps_const = cst;
+    }
+    void receivers(double* col)
+    {
+        a.rec_off = rec_off_d.p;
+        a.rec_ent = rec_ent_d.p;
+        a.seis_col = col;
+    }
+    void born_green(const float* dtdmu, const float* gin)
+    {
+        launch_born_green(g.R, a, sxy[cur], szy[cur], dtdmu, gin, st);
+    }
+    void step()
+    {
+        launch_step(g.R, with_u, a, st);
+        cur ^= 1;
+    }
+    // structure-kernel product with the displacement the last step produced
+    void kmu(const float* fwd, double* kmu_acc, double scale) { launch_kmu(g.R, a, u[cur], fwd, kmu_acc, scale, st); }
 };
 
 // Blocked frequency-domain helpers: time samples are staged in FP32 slabs of up to kSlab slices.
-constexpr int kSlab = 128;
 
 struct DftAccumulator {
     double2* acc = nullptr;
@@ -393,7 +481,7 @@ struct StepTimer {
     int solver, every;
     size_t P;
     hipStream_t st;
-    std::vector<hipEvent_t> ev;  // triples: before velocity, between, after stress
+    std::vector<hipEvent_t> ev;  // pairs: before and after the step launch
     StepTimer(int solver_, size_t P_, hipStream_t s) : solver(solver_), P(P_), st(s)
     {
         std::lock_guard<std::mutex> lk(g_timing_mu);
@@ -413,12 +501,10 @@ struct StepTimer {
     {
         double ms_v = 0, ms_s = 0;
         int64_t cnt = 0;
-        for (size_t q = 0; q + 2 < ev.size(); q += 3) {
-            float a = 0, b = 0;
+        for (size_t q = 0; q + 1 < ev.size(); q += 2) {
+            float a = 0;
             HIPCHK(hipEventElapsedTime(&a, ev[q], ev[q + 1]));
-            HIPCHK(hipEventElapsedTime(&b, ev[q + 1], ev[q + 2]));
             ms_v += a;
-            ms_s += b;
             ++cnt;
         }
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -426,7 +512,7 @@ struct StepTimer {
         if (cnt) {
             std::lock_guard<std::mutex> lk(g_timing_mu);
             g_timing.launches[solver - 1][0] += cnt;
-            g_timing.launches[solver - 1][1] += cnt;
+            g_timing.launches[solver - 1][1] += 0;
             g_timing.ms[solver - 1][0] += ms_v;
             g_timing.ms[solver - 1][1] += ms_s;
             g_timing.points[solver - 1] = (int64_t)P;
@@ -481,9 +567,8 @@ void green_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carr
 
     Solver S(cfg, g, st, false);
     S.material((const double*)mu->data, (const double*)rho->data);
-    DBuf<int2> sid_d, rid_d;
-    upload(sid_d, sid, st);
-    upload(rid_d, rid, st);
+    S.set_sources(sid);
+    S.set_receivers(rid);
     DBuf<float> dtdmu;
     if (born) {
         dtdmu.alloc(g.P);
@@ -507,22 +592,12 @@ void green_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carr
             if (Gout) S.a.snap = Gout + (size_t)i_fw_out * g.P;
             ++i_fw_out;
         }
-        if (born) launch_born_green(g.R, S.a, dtdmu.p, (const float*)G_in->data + (size_t)(n - 1) * g.P, st);
-        else {
-            S.a.nps = (int)sid.size();
-            S.a.ps_ij = sid_d.p;
-            S.a.ps_const = 1.0e9f;  // stf = 1e9 every step (:64)
-        }
+        if (born) S.born_green(dtdmu.p, (const float*)G_in->data + (size_t)(n - 1) * g.P);
+        else if (!sid.empty()) S.point_sources(nullptr, 1.0e9f);  // stf = 1e9 every step (:64)
         if (Gf && (n + g.nt - 1) % g.s == 0) S.a.dsamp = dft.next_slot();
-        if (sg && nr) {
-            S.a.nrec = (int)nr;
-            S.a.rec_ij = rid_d.p;
-            S.a.seis_col = sg + (size_t)nr * (n - 1);
-        }
+        if (sg && nr) S.receivers(sg + (size_t)nr * (n - 1));
         timer.mark(n);
-        launch_velocity(g.R, S.a, st);
-        timer.mark(n);
-        launch_stress(g.R, S.a, st);
+        S.step();
         timer.mark(n);
         if (S.a.dsamp) dft.commit();
     }
@@ -569,8 +644,7 @@ void correlation_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2
 
     Solver S(cfg, g, st, true);
     S.material((const double*)mu->data, (const double*)rho->data);
-    DBuf<int2> rid_d;
-    upload(rid_d, rid, st);
+    S.set_receivers(rid);
     if (sg && nr) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * N, st));
 
     // source spectrum A = (sum_ns spectrum) * distribution .* G_fft (:168-173, :201-205)
@@ -610,15 +684,9 @@ void correlation_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2
             S.a.born_in0 = (const float*)C_in->data + (size_t)(n - 1) * g.P;
             S.a.born_in1 = (const float*)C_in->data + (size_t)n * g.P;
         }
-        if (sg && nr) {
-            S.a.nrec = (int)nr;
-            S.a.rec_ij = rid_d.p;
-            S.a.seis_col = sg + (size_t)nr * (n - 1);
-        }
+        if (sg && nr) S.receivers(sg + (size_t)nr * (n - 1));
         timer.mark(n);
-        launch_velocity(g.R, S.a, st);
-        timer.mark(n);
-        launch_stress(g.R, S.a, st);
+        S.step();
         timer.mark(n);
     }
     HIPCHK(hipGetLastError());
@@ -692,9 +760,8 @@ void adjoint_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_ca
 
     Solver S(cfg, g, st, true);
     S.material((const double*)mu->data, (const double*)rho->data);
-    DBuf<int2> sid_d, rid_d;
-    upload(sid_d, sid, st);
-    upload(rid_d, rid, st);
+    if (stf_point) S.set_sources(sid);
+    S.set_receivers(rid);
     if (sg && nr) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * N, st));
     if (Kd) HIPCHK(hipMemsetAsync(Kd, 0, sizeof(double) * g.P * (size_t)(nb_cols + 1), st));
 
@@ -755,9 +822,7 @@ void adjoint_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_ca
             ++i_fw_out;
         }
         if (stf_point && !sid.empty()) {
-            S.a.nps = (int)sid.size();
-            S.a.ps_ij = sid_d.p;
-            S.a.ps_amp = amp.p + (size_t)(n - 1) * sid.size();
+            S.point_sources(amp.p + (size_t)(n - 1) * sid.size(), 0.f);
         } else if (dist_src && n % g.s == 0) {
             S.a.syn = syn.slice(n / g.s - 1);
         }
@@ -766,23 +831,17 @@ void adjoint_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_ca
             S.a.born_in0 = (const float*)u_in->data + (size_t)(n - 1) * g.P;
             S.a.born_in1 = (const float*)u_in->data + (size_t)n * g.P;
         }
+        const float* kfwd = nullptr;
         if (Kd && n_ufwd && n_ufwd >= i_fw_in + 1 && n % g.f == 0) {  // :324-331
-            S.a.kmu_fwd = (const float*)u_fwd->data + (size_t)(n_ufwd - 1 - i_fw_in) * g.P;
-            S.a.kmu = Kd + (size_t)nb_cols * g.P;
-            S.a.kmu_scale = (double)g.f;
+            kfwd = (const float*)u_fwd->data + (size_t)(n_ufwd - 1 - i_fw_in) * g.P;
             ++i_fw_in;
         }
         if (need_dft && n % g.s == 0) S.a.dsamp = dft.next_slot();
-        if (sg && nr) {
-            S.a.nrec = (int)nr;
-            S.a.rec_ij = rid_d.p;
-            S.a.seis_col = sg + (size_t)nr * (n - 1);
-        }
+        if (sg && nr) S.receivers(sg + (size_t)nr * (n - 1));
         timer.mark(n);
-        launch_velocity(g.R, S.a, st);
+        S.step();
         timer.mark(n);
-        launch_stress(g.R, S.a, st);
-        timer.mark(n);
+        if (kfwd) S.kmu(kfwd, Kd + (size_t)nb_cols * g.P, (double)g.f);
         if (S.a.dsamp) dft.commit();
     }
     if (need_dft) {

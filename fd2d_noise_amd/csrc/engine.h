@@ -4,8 +4,11 @@
 // coefficient (dt/rho, dt*mu) is an FP32 nx x nz array, x fastest (the MATLAB column-major order,
 // so a wavefront reads 64 consecutive x nodes of one z row: 256 contiguous bytes). Each array is
 // padded with kGhost zero nodes on every side and a row pitch rounded up to 64 elements, so every
-// stencil neighbour of an interior node is an in-bounds load. sigma_xy and sigma_zy share the node
-// (i,j) of the MATLAB arrays sxy(i,j) ((nx-1) x nz) and szy(i,j) (nx x (nz-1)).
+// load of a tile's halo that lies inside the padding is in bounds. sigma_xy and sigma_zy share the
+// node (i,j) of the MATLAB arrays sxy(i,j) ((nx-1) x nz) and szy(i,j) (nx x (nz-1)).
+//
+// The time-stepped fields are double-buffered: step n reads the "in" set and writes the "out" set,
+// so a workgroup never reads a neighbour's already-updated values.
 //
 // Snapshots (G_out, C_out, u_out, u_fwd, G_in, C_in, u_in), DFT samples and synthesised source
 // slices are compact nx x nz arrays (the MATLAB layout, so they are copied out unchanged).
@@ -18,76 +21,92 @@
 
 namespace fd2d {
 
-constexpr int kGhost = 4;    // widest half-stencil (order 8)
-constexpr int kBX = 64;      // step-kernel tile: one wavefront per z row
-constexpr int kBZ = 4;
-constexpr int kMaxPointSources = 1 << 16;
+constexpr int kGhost = 8;   // ghost rows (>= 2R-1 for R <= 4: the sigma halo of a fused step)
+constexpr int kGhostX = 32; // ghost columns: node x = 0 starts a 128-byte line
+constexpr int kTX = 64;     // tile width: one wavefront per row
+constexpr int kXCD = 8;     // MI355X accelerator complex dies (workgroups are dealt round-robin)
+constexpr int kTileZ = 32;  // rows of a step tile
+constexpr int kMaxDim = 32000;  // node keys pack (x+8, z+8) into 16 bits each
 
 struct PaddedLayout {
     int nx = 0, nz = 0, pitch = 0, rows = 0;
     size_t elems() const { return (size_t)pitch * (size_t)rows; }
-    size_t origin() const { return (size_t)kGhost * (size_t)pitch + kGhost; }
+    size_t origin() const { return (size_t)kGhost * (size_t)pitch + kGhostX; }
     static PaddedLayout make(int nx, int nz)
     {
         PaddedLayout l;
         l.nx = nx;
         l.nz = nz;
-        l.pitch = ((nx + 2 * kGhost + 63) / 64) * 64;
+        // every tile column (the last tile runs to the next multiple of 64) plus its 2R-1 halo
+        // stays inside its own padded row
+        l.pitch = ((kGhostX + nx + 63) / 64) * 64 + 64;
         l.rows = nz + 2 * kGhost;
         return l;
     }
 };
 
-// Arguments of one time step (velocity phase and stress phase).
+// Arguments of one fused time step.
 struct StepArgs {
-    // state (origin pointers into padded arrays)
-    float* v;
-    float* sxy;
-    float* szy;
-    float* u;            // displacement (correlation / adjoint runs), else null
+    // state (origin pointers into padded arrays); in = step n-1, out = step n
+    const float* v_in;
+    const float* sxy_in;
+    const float* szy_in;
+    const float* u_in;   // displacement (correlation / adjoint runs), else null
+    float* v_out;
+    float* sxy_out;
+    float* szy_out;
+    float* u_out;
     const float* bv;     // dt / rho
     const float* dtmu;   // dt * mu
     const float* ax;     // separable taper, nx values
     const float* az;     // nz values
     int nx, nz, pitch;
+    int tiles_x, n_tiles;           // tiles across x, tiles in all
+    int tiles_per_xcd;              // tile t = (g % 8) * tiles_per_xcd + g / 8 for workgroup g
+    int origin;                     // element offset of node (0,0) in a padded array
+    unsigned padded_bytes;          // bytes of one padded array
     float cx[4], cz[4];  // staggered coefficients / dx, / dz
     float dt;
 
-    // velocity phase
-    int nps;                 // point sources
-    const int2* ps_ij;
-    const float* ps_amp;     // nps amplitudes of this step (null: ps_const for all)
+    // point sources: per-tile CSR over the nodes whose velocity the tile computes (tile plus
+    // halo); entries {node key ((z+8)<<16 | (x+8)), source}
+    const int* ps_off;
+    const int2* ps_ent;
+    const float* ps_amp;     // amplitudes of this step indexed by source (null: ps_const)
     float ps_const;
     const float* syn;        // synthesised distributed source of this step (compact), or null
     // With u present (correlation / adjoint) the snapshot is u before the update and the DFT
     // sample and receivers take u after it; without u (Green) they take v.
     float* snap;             // snapshot destination (compact) or null
     float* dsamp;            // DFT sample destination (compact) or null
-    int nrec;
-    const int2* rec_ij;
+    // receivers: per-tile CSR over its own nodes; entries {node key, receiver}
+    const int* rec_off;
+    const int2* rec_ent;
     double* seis_col;        // seismograms(:, n)
 
-    // stress phase
+    // stress-phase Born terms
     const float* born_dmu;   // dmu as FP32 (compact) for C_in / u_in Born terms, or null
     const float* born_in0;   // C_in(:,:,n)
     const float* born_in1;   // C_in(:,:,n+1)
-    const float* kmu_fwd;    // forward snapshot for the structure kernel (compact), or null
-    double* kmu;             // K_mu accumulator (compact FP64)
-    double kmu_scale;        // fw_nth
 };
 
-// Launchers (R = half stencil width: 1..4).
-void launch_velocity(int R, const StepArgs& a, hipStream_t s);
-void launch_stress(int R, const StepArgs& a, hipStream_t s);
-// Green-function Born source applied at the start of a step: sigma += (dt*dmu) .* D(G_in slice).
-void launch_born_green(int R, const StepArgs& a, const float* dtdmu, const float* gin, hipStream_t s);
+inline int node_key_host(int x, int z) { return ((z + 8) << 16) | (x + 8); }
 
-// Material coefficients: bv = dt/rho, dtmu = dt*mu into padded arrays; optional dmu conversion.
+void launch_step(int R, bool with_u, const StepArgs& a, hipStream_t s);
+// K_mu -= scale * (D u . D fwd) over the grid (u padded, fwd compact), after a step.
+void launch_kmu(int R, const StepArgs& a, const float* u, const float* fwd, double* kmu, double scale,
+                hipStream_t s);
+// Green-function Born source applied at the start of a step: sigma += (dt*dmu) .* D(G_in slice),
+// in place on the current state.
+void launch_born_green(int R, const StepArgs& a, float* sxy, float* szy, const float* dtdmu, const float* gin,
+                       hipStream_t s);
+
+// Material coefficients: bv = dt/rho, dtmu = dt*mu into padded arrays.
 void launch_material(const double* mu, const double* rho, float* bv, float* dtmu, int nx, int nz,
                      int pitch, double dt, hipStream_t s);
 void launch_scale_to_float(const double* in, float* out, size_t n, double scale, hipStream_t s);
 
-// acc[k][x] (+)= sum_j samp[j][x] * coef[j][k]   (FP64 complex accumulators, FP32 samples)
+// acc[k][x] += sum_j samp[j][x] * coef[j][k]   (FP64 complex accumulators, FP32 samples)
 void launch_dft_flush(double2* acc, const float* samp, const double2* coef, size_t P, int K, int cnt,
                       hipStream_t s);
 // out[j][x] = sum_k Re(A[k][x] * coef[j][k])      (FP64, rounded to FP32)
@@ -105,7 +124,7 @@ void launch_source_kernel(double* Ks, const double2* G, const double2* D, const 
                           hipStream_t s);
 // point-source amplitudes: amp[n][q] = (float)Re(stf(q, n)) from complex nr x N interleaved
 void launch_stf_real(float* amp, const double2* stf, int64_t d0, int nq, int nsteps, hipStream_t s);
-// Ks/Kmu copy into the MATLAB K output
-void launch_copy_f64(double* dst, const double* src, size_t n, hipStream_t s);
+
+constexpr int kSlab = 128;  // time samples per DFT flush / synthesis block
 
 }  // namespace fd2d

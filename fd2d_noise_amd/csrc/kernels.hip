@@ -14,193 +14,383 @@
 namespace fd2d {
 
 // ---------------------------------------------------------------------------------------------
-// Velocity phase
+// Fused time step
 // ---------------------------------------------------------------------------------------------
-template <int R>
-__global__ __launch_bounds__(kBX* kBZ) void k_velocity(StepArgs a)
+// One workgroup (4 wavefronts) advances one kTX x kTileZ tile by one step:
+//   phase 1 issues, in one burst, the loads of sigma over the tile plus the halo that the velocity
+//           stencil of the halo needs, the velocity operands and (structure-kernel steps) the
+//           forward snapshot, then stores them to LDS;
+//   phase 2 computes the velocity (and displacement) on the tile plus an (R-1, R) halo in both
+//           directions -- the halo is recomputed, bit-identically, by the neighbouring tiles --
+//           and writes the outputs of the tile's own nodes;
+//   phase 3 updates sigma (and K_mu) on the tile.
+// Lane l of every wavefront owns column i0 + l; wavefront w owns rows w, w+4, ... of each region,
+// so row offsets are wave-uniform and LDS row offsets fold into instruction immediates. The few
+// halo columns (2R-1 of the velocity region, 2(2R-1) of the sigma_xy region) are handled as one
+// packed list. Global loads are raw buffer loads with the whole byte offset in the VGPR, so the
+// hardware range check covers every load (out-of-range reads return 0).
+//
+// Tiles are dealt to the XCDs in contiguous bands (tile = (g % 8) * tiles_per_xcd + g / 8), so a
+// tile's x-neighbour, and a tile row later its z-neighbour, read the shared halo lines from the
+// same L2.
+//
+// MODE 0: Green run (no displacement); 1: with displacement (correlation and adjoint runs).
+__device__ __forceinline__ int node_key(int x, int z) { return ((z + 8) << 16) | (x + 8); }
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, unsigned bytes)
 {
-    const int i0 = blockIdx.x * kBX, j0 = blockIdx.y * kBZ;
-    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
-    if (i >= a.nx || j >= a.nz) return;
-    const long p = (long)j * a.pitch + i;
-    const long c = (long)j * a.nx + i;
-
-    float ds = 0.f;
-    if (i >= R && i <= a.nx - R - 1) {
-        float s = 0.f;
-#pragma unroll
-        for (int m = 1; m <= R; ++m) s += a.cx[m - 1] * (a.sxy[p + m - 1] - a.sxy[p - m]);
-        ds = s;
-    }
-    if (j >= R && j <= a.nz - R - 1) {
-        float s = 0.f;
-#pragma unroll
-        for (int m = 1; m <= R; ++m)
-            s += a.cz[m - 1] * (a.szy[p + (long)(m - 1) * a.pitch] - a.szy[p - (long)m * a.pitch]);
-        ds += s;
-    }
-    // point sources: the tile test is uniform, the node test runs only in the owning tile
-    for (int q = 0; q < a.nps; ++q) {
-        const int2 ij = a.ps_ij[q];
-        if (ij.x >= i0 && ij.x < i0 + kBX && ij.y >= j0 && ij.y < j0 + kBZ) {
-            if (ij.x == i && ij.y == j) ds += a.ps_amp ? a.ps_amp[q] : a.ps_const;
-        }
-    }
-    if (a.syn) ds += a.syn[c];
-
-    const float vo = a.v[p];
-    const float vn = (vo + a.bv[p] * ds) * (a.ax[i] * a.az[j]);
-    a.v[p] = vn;
-    float out = vn;  // recorded and sampled field: v in the Green run, u otherwise
-    if (a.u) {
-        const float uo = a.u[p];
-        if (a.snap) a.snap[c] = uo;
-        const float un = uo + vn * a.dt;
-        a.u[p] = un;
-        out = un;
-    } else if (a.snap) {
-        a.snap[c] = vo;
-    }
-    if (a.dsamp) a.dsamp[c] = out;
-    for (int q = 0; q < a.nrec; ++q) {
-        const int2 ij = a.rec_ij[q];
-        if (ij.x >= i0 && ij.x < i0 + kBX && ij.y >= j0 && ij.y < j0 + kBZ) {
-            if (ij.x == i && ij.y == j) a.seis_col[q] = (double)out;
-        }
-    }
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int off)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
 
-// ---------------------------------------------------------------------------------------------
-// Stress phase
-// ---------------------------------------------------------------------------------------------
-template <int R>
-__device__ __forceinline__ float dx_pad(const float* f, long p, const float* cx)
+template <int R, int MODE>
+__global__ __launch_bounds__(256) void k_tile(StepArgs a)
 {
-    float s = 0.f;
-#pragma unroll
-    for (int m = 1; m <= R; ++m) s += cx[m - 1] * (f[p + m] - f[p - m + 1]);
-    return s;
-}
+    constexpr bool HAS_U = MODE >= 1;
+    constexpr int TZ = kTileZ;
+    constexpr int WV = kTX + 2 * R - 1;     // velocity region: columns i0-(R-1) .. i0+63+R
+    constexpr int HV = TZ + 2 * R - 1;      //                  rows    j0-(R-1) .. j0+TZ-1+R
+    constexpr int WSX = WV + 2 * R - 1;     // sigma_xy region: columns i0-(2R-1) .. i0+63+2R-1
+    constexpr int HSZ = HV + 2 * R - 1;     // sigma_zy region: rows    j0-(2R-1) .. j0+TZ-1+2R-1
+    constexpr int HCV = 2 * R - 1;          // halo columns of the velocity / sigma_zy regions
+    constexpr int HCS = 2 * (2 * R - 1);    // halo columns of the sigma_xy region
+    __shared__ float sx[HV * WSX];
+    __shared__ float sz[HSZ * WV];
+    __shared__ float vv[HV * WV];
 
-template <int R>
-__device__ __forceinline__ float dz_pad(const float* f, long p, long pitch, const float* cz)
-{
-    float s = 0.f;
-#pragma unroll
-    for (int m = 1; m <= R; ++m) s += cz[m - 1] * (f[p + (long)m * pitch] - f[p - (long)(m - 1) * pitch]);
-    return s;
-}
+    const int g = blockIdx.x;
+    const int t = (g % kXCD) * a.tiles_per_xcd + g / kXCD;
+    if (t >= a.n_tiles) return;
+    const int i0 = (t % a.tiles_x) * kTX, j0 = (t / a.tiles_x) * TZ;
+    const int nx = a.nx, nz = a.nz, pitch = a.pitch;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int x = i0 + lane;                 // this lane's own column
+    const bool xg = x < nx;
 
-template <int R>
-__global__ __launch_bounds__(kBX* kBZ) void k_stress(StepArgs a)
-{
-    const int i = blockIdx.x * kBX + threadIdx.x, j = blockIdx.y * kBZ + threadIdx.y;
-    if (i >= a.nx || j >= a.nz) return;
-    const long p = (long)j * a.pitch + i;
-    const long c = (long)j * a.nx + i;
-    const long nxl = a.nx;
-    // dx_v is set for i = R-1 .. nx-R-1 (0-based), dz_v for j = R-1 .. nz-R-1
-    const bool xin = i >= R - 1 && i <= a.nx - R - 1;
-    const bool zin = j >= R - 1 && j <= a.nz - R - 1;
-    const float dm = a.dtmu[p];
-    if (xin) {
-        float sn = a.sxy[p] + dm * dx_pad<R>(a.v, p, a.cx);
-        if (a.born_dmu) {
+    const unsigned pb = a.padded_bytes;
+    const auto r_sxy = rsrc(a.sxy_in - a.origin, pb), r_szy = rsrc(a.szy_in - a.origin, pb);
+    const auto r_v = rsrc(a.v_in - a.origin, pb), r_bv = rsrc(a.bv - a.origin, pb);
+    const auto r_u = rsrc(HAS_U ? a.u_in - a.origin : a.v_in - a.origin, pb);
+    const auto r_dm = rsrc(a.dtmu - a.origin, pb);
+    // padded byte offset of node (xx, zz); zz >= -kGhost and xx >= -kGhostX on every use
+    auto po = [&](int xx, int zz) { return ((zz + kGhost) * pitch + xx + kGhostX) * 4; };
+    auto co = [&](int xx, int zz) { return (zz * nx + xx) * 4; };  // compact byte offset (in grid)
+    auto ingrid = [&](int xx, int zz) { return xx >= 0 && xx < nx && zz >= 0 && zz < nz; };
+    const float* syn = a.syn;
+
+    // ---------------- phase 1
+    constexpr int NRV = (HV + 3) / 4, NRZ = (HSZ + 3) / 4;
+    constexpr int NHS = (HV * HCS + 255) / 256, NHZ = (HSZ * HCV + 255) / 256, NHV = (HV * HCV + 255) / 256;
+    float msx[NRV], msz[NRZ];
+    float mv[NRV], mb[NRV], mu[NRV], msy[NRV];
+    float hsx[NHS], hsz[NHZ];
+    float hv[NHV], hb[NHV], hu[NHV], hsy[NHV], ht[NHV];
+    const float axl = xg ? a.ax[x] : 0.f;
+#pragma unroll
+    for (int k = 0; k < NRV; ++k) {
+        const int r = w + 4 * k, z = j0 - (R - 1) + r;
+        const bool live = r < HV;
+        const int o = po(x, z);
+        msx[k] = live ? bload(r_sxy, o) : 0.f;
+        const bool in = live && xg && z >= 0 && z < nz;
+        mv[k] = in ? bload(r_v, o) : 0.f;
+        mb[k] = in ? bload(r_bv, o) : 0.f;
+        mu[k] = (HAS_U && in) ? bload(r_u, o) : 0.f;
+        msy[k] = (in && syn) ? syn[co(x, z) / 4] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NRZ; ++k) {
+        const int r = w + 4 * k, z = j0 - (2 * R - 1) + r;
+        msz[k] = r < HSZ ? bload(r_szy, po(x, z)) : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NHS; ++k) {
+        const int idx = tid + 256 * k;
+        const int r = idx / HCS, h = idx - r * HCS;
+        const int c = h < 2 * R - 1 ? h : h + kTX;            // sigma_xy region column
+        const int xx = i0 - (2 * R - 1) + c, zz = j0 - (R - 1) + r;
+        hsx[k] = idx < HV * HCS ? bload(r_sxy, po(xx, zz)) : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NHZ; ++k) {
+        const int idx = tid + 256 * k;
+        const int r = idx / HCV, h = idx - r * HCV;
+        const int c = h < R - 1 ? h : h + kTX;                // velocity-region column
+        const int xx = i0 - (R - 1) + c, zz = j0 - (2 * R - 1) + r;
+        hsz[k] = idx < HSZ * HCV ? bload(r_szy, po(xx, zz)) : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NHV; ++k) {
+        const int idx = tid + 256 * k;
+        const int r = idx / HCV, h = idx - r * HCV;
+        const int c = h < R - 1 ? h : h + kTX;
+        const int xx = i0 - (R - 1) + c, zz = j0 - (R - 1) + r;
+        const bool in = idx < HV * HCV && ingrid(xx, zz);
+        const int o = po(xx, zz);
+        hv[k] = in ? bload(r_v, o) : 0.f;
+        hb[k] = in ? bload(r_bv, o) : 0.f;
+        hu[k] = (HAS_U && in) ? bload(r_u, o) : 0.f;
+        hsy[k] = (in && syn) ? syn[co(xx, zz) / 4] : 0.f;
+        ht[k] = in ? a.ax[xx] * a.az[zz] : 0.f;
+    }
+    // LDS stores
+#pragma unroll
+    for (int k = 0; k < NRV; ++k) {
+        const int r = w + 4 * k;
+        if (r < HV) sx[r * WSX + lane + 2 * R - 1] = msx[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NRZ; ++k) {
+        const int r = w + 4 * k;
+        if (r < HSZ) sz[r * WV + lane + R - 1] = msz[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NHS; ++k) {
+        const int idx = tid + 256 * k;
+        const int r = idx / HCS, h = idx - r * HCS;
+        if (idx < HV * HCS) sx[r * WSX + (h < 2 * R - 1 ? h : h + kTX)] = hsx[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NHZ; ++k) {
+        const int idx = tid + 256 * k;
+        const int r = idx / HCV, h = idx - r * HCV;
+        if (idx < HSZ * HCV) sz[r * WV + (h < R - 1 ? h : h + kTX)] = hsz[k];
+    }
+    __syncthreads();
+
+    // ---------------- phase 2
+    const int e0 = a.ps_off ? a.ps_off[t] : 0, e1 = a.ps_off ? a.ps_off[t + 1] : 0;
+    const int f0 = a.rec_off ? a.rec_off[t] : 0, f1 = a.rec_off ? a.rec_off[t + 1] : 0;
+    // velocity at (xx, zz) = region cell (r, c) from the staged sigma; the point sources of the node
+    auto divs = [&](int xx, int zz, int r, int c) {
+        float ds = 0.f;
+        if (xx >= R && xx <= nx - R - 1) {  // div_s x-part frame
+            const float* row = sx + r * WSX + c + R;
             float s = 0.f;
 #pragma unroll
-            for (int m = 1; m <= R; ++m)
-                s += a.cx[m - 1] * ((a.born_in0[c + m] - a.born_in1[c + m]) -
-                                    (a.born_in0[c - m + 1] - a.born_in1[c - m + 1]));
-            sn -= a.born_dmu[c] * s;
+            for (int m = 1; m <= R; ++m) s += a.cx[m - 1] * (row[m - 1] - row[-m]);
+            ds = s;
         }
-        a.sxy[p] = sn;
-    }
-    if (zin) {
-        float sn = a.szy[p] + dm * dz_pad<R>(a.v, p, a.pitch, a.cz);
-        if (a.born_dmu) {
+        if (zz >= R && zz <= nz - R - 1) {  // div_s z-part frame
+            const float* col = sz + (r + R) * WV + c;
             float s = 0.f;
 #pragma unroll
-            for (int m = 1; m <= R; ++m)
-                s += a.cz[m - 1] * ((a.born_in0[c + m * nxl] - a.born_in1[c + m * nxl]) -
-                                    (a.born_in0[c - (m - 1) * nxl] - a.born_in1[c - (m - 1) * nxl]));
-            sn -= a.born_dmu[c] * s;
+            for (int m = 1; m <= R; ++m) s += a.cz[m - 1] * (col[(m - 1) * WV] - col[-m * WV]);
+            ds += s;
         }
-        a.szy[p] = sn;
+        if (e1 > e0) {
+            const int key = node_key(xx, zz);
+            for (int e = e0; e < e1; ++e) {
+                const int2 en = a.ps_ent[e];
+                if (en.x == key) ds += a.ps_amp ? a.ps_amp[en.y] : a.ps_const;
+            }
+        }
+        return ds;
+    };
+#pragma unroll
+    for (int k = 0; k < NRV; ++k) {
+        const int r = w + 4 * k;
+        if (r >= HV) break;
+        const int z = j0 - (R - 1) + r, c = lane + R - 1;
+        float vn = 0.f;
+        if (xg && z >= 0 && z < nz) {
+            const float ds = divs(x, z, r, c) + msy[k];
+            vn = (mv[k] + mb[k] * ds) * (axl * a.az[z]);
+            const float un = HAS_U ? mu[k] + vn * a.dt : 0.f;
+            if (r >= R - 1 && r < R - 1 + TZ) {  // own node: outputs
+                const long p = (long)z * pitch + x;
+                const long cc = (long)z * nx + x;
+                a.v_out[p] = vn;
+                float out = vn;
+                if (HAS_U) {
+                    a.u_out[p] = un;
+                    if (a.snap) a.snap[cc] = mu[k];
+                    out = un;
+                } else if (a.snap) {
+                    a.snap[cc] = mv[k];
+                }
+                if (a.dsamp) a.dsamp[cc] = out;
+                if (f1 > f0) {
+                    const int key = node_key(x, z);
+                    for (int e = f0; e < f1; ++e) {
+                        const int2 en = a.rec_ent[e];
+                        if (en.x == key) a.seis_col[en.y] = (double)out;
+                    }
+                }
+            }
+        }
+        vv[r * WV + c] = vn;
     }
-    if (a.kmu) {
-        double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < NHV; ++k) {
+        const int idx = tid + 256 * k;
+        if (idx >= HV * HCV) break;
+        const int r = idx / HCV, h = idx - r * HCV;
+        const int c = h < R - 1 ? h : h + kTX;
+        const int xx = i0 - (R - 1) + c, zz = j0 - (R - 1) + r;
+        float vn = 0.f;
+        if (ingrid(xx, zz)) {
+            const float ds = divs(xx, zz, r, c) + hsy[k];
+            vn = (hv[k] + hb[k] * ds) * ht[k];
+        }
+        vv[r * WV + c] = vn;
+    }
+    __syncthreads();
+
+    // ---------------- phase 3
+    constexpr int NT = TZ / 4;
+    float rm[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        const int z = j0 + w + 4 * k;
+        rm[k] = (xg && z < nz) ? bload(r_dm, po(x, z)) : 0.f;
+    }
+    const bool xin = x >= R - 1 && x <= nx - R - 1;  // dx_v frame
+    const int c = lane + R - 1;
+    const long nxl = nx;
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        const int lz = w + 4 * k, z = j0 + lz;
+        if (!xg || z >= nz) continue;
+        const int r = lz + R - 1;
+        const long p = (long)z * pitch + x;
+        const long cc = (long)z * nx + x;
+        const bool zin = z >= R - 1 && z <= nz - R - 1;  // dz_v frame
+        const float dm = rm[k];
+        const float* vrow = vv + r * WV + c;
         if (xin) {
-            float sf = 0.f;
+            float e = 0.f;
 #pragma unroll
-            for (int m = 1; m <= R; ++m) sf += a.cx[m - 1] * (a.kmu_fwd[c + m] - a.kmu_fwd[c - m + 1]);
-            acc += (double)dx_pad<R>(a.u, p, a.cx) * (double)sf;
+            for (int m = 1; m <= R; ++m) e += a.cx[m - 1] * (vrow[m] - vrow[1 - m]);
+            float sn = sx[r * WSX + c + R] + dm * e;
+            if (a.born_dmu) {
+                float s = 0.f;
+#pragma unroll
+                for (int m = 1; m <= R; ++m)
+                    s += a.cx[m - 1] * ((a.born_in0[cc + m] - a.born_in1[cc + m]) -
+                                        (a.born_in0[cc - m + 1] - a.born_in1[cc - m + 1]));
+                sn -= a.born_dmu[cc] * s;
+            }
+            a.sxy_out[p] = sn;
         }
         if (zin) {
-            float sf = 0.f;
+            float e = 0.f;
 #pragma unroll
-            for (int m = 1; m <= R; ++m)
-                sf += a.cz[m - 1] * (a.kmu_fwd[c + m * nxl] - a.kmu_fwd[c - (m - 1) * nxl]);
-            acc += (double)dz_pad<R>(a.u, p, a.pitch, a.cz) * (double)sf;
+            for (int m = 1; m <= R; ++m) e += a.cz[m - 1] * (vrow[m * WV] - vrow[(1 - m) * WV]);
+            float sn = sz[(r + R) * WV + c] + dm * e;
+            if (a.born_dmu) {
+                float s = 0.f;
+#pragma unroll
+                for (int m = 1; m <= R; ++m)
+                    s += a.cz[m - 1] * ((a.born_in0[cc + m * nxl] - a.born_in1[cc + m * nxl]) -
+                                        (a.born_in0[cc - (m - 1) * nxl] - a.born_in1[cc - (m - 1) * nxl]));
+                sn -= a.born_dmu[cc] * s;
+            }
+            a.szy_out[p] = sn;
         }
-        if (xin || zin) a.kmu[c] -= acc * a.kmu_scale;
+    }
+}
+
+// Structure-kernel product of run_noise_adjoint.m:324-331 after a step that stores it:
+// K_mu -= fw_nth * (dx_v(u) .* dx_v(u_fwd) + dz_v(u) .* dz_v(u_fwd)) with u the new displacement
+// (padded) and u_fwd the forward snapshot (compact); the products are formed in FP64.
+template <int R>
+__global__ __launch_bounds__(256) void k_kmu(StepArgs a, const float* __restrict__ u, const float* __restrict__ fwd,
+                                             double* __restrict__ kmu, double scale)
+{
+    const int x = blockIdx.x * kTX + threadIdx.x, z = blockIdx.y * 4 + threadIdx.y;
+    if (x >= a.nx || z >= a.nz) return;
+    const bool xin = x >= R - 1 && x <= a.nx - R - 1, zin = z >= R - 1 && z <= a.nz - R - 1;
+    if (!xin && !zin) return;
+    const int p = z * a.pitch + x, c = z * a.nx + x, pz = a.pitch, cz = a.nx;
+    double acc = 0.0;
+    if (xin) {
+        float su = 0.f, sf = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) {
+            su += a.cx[m - 1] * (u[p + m] - u[p - m + 1]);
+            sf += a.cx[m - 1] * (fwd[c + m] - fwd[c - m + 1]);
+        }
+        acc += (double)su * (double)sf;
+    }
+    if (zin) {
+        float su = 0.f, sf = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) {
+            su += a.cz[m - 1] * (u[p + m * pz] - u[p - (m - 1) * pz]);
+            sf += a.cz[m - 1] * (fwd[c + m * cz] - fwd[c - (m - 1) * cz]);
+        }
+        acc += (double)su * (double)sf;
+    }
+    kmu[c] -= acc * scale;
+}
+
+void launch_kmu(int R, const StepArgs& a, const float* u, const float* fwd, double* kmu, double scale, hipStream_t s)
+{
+    const dim3 g((unsigned)((a.nx + kTX - 1) / kTX), (unsigned)((a.nz + 3) / 4)), b(kTX, 4);
+    switch (R) {
+    case 1: hipLaunchKernelGGL(k_kmu<1>, g, b, 0, s, a, u, fwd, kmu, scale); break;
+    case 2: hipLaunchKernelGGL(k_kmu<2>, g, b, 0, s, a, u, fwd, kmu, scale); break;
+    case 3: hipLaunchKernelGGL(k_kmu<3>, g, b, 0, s, a, u, fwd, kmu, scale); break;
+    default: hipLaunchKernelGGL(k_kmu<4>, g, b, 0, s, a, u, fwd, kmu, scale); break;
     }
 }
 
 template <int R>
-__global__ __launch_bounds__(kBX* kBZ) void k_born_green(StepArgs a, const float* dtdmu, const float* gin)
+__global__ __launch_bounds__(256) void k_born_green(StepArgs a, float* sxy, float* szy, const float* dtdmu,
+                                                    const float* gin)
 {
-    const int i = blockIdx.x * kBX + threadIdx.x, j = blockIdx.y * kBZ + threadIdx.y;
-    if (i >= a.nx || j >= a.nz) return;
-    const long p = (long)j * a.pitch + i;
-    const long c = (long)j * a.nx + i;
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long)a.nx * a.nz) return;
+    const int x = (int)(c % a.nx), z = (int)(c / a.nx);
+    const long p = (long)z * a.pitch + x;
     const long nxl = a.nx;
     const float d = dtdmu[c];
-    if (i >= R - 1 && i <= a.nx - R - 1) {
+    if (x >= R - 1 && x <= a.nx - R - 1) {
         float s = 0.f;
 #pragma unroll
         for (int m = 1; m <= R; ++m) s += a.cx[m - 1] * (gin[c + m] - gin[c - m + 1]);
-        a.sxy[p] += d * s;
+        sxy[p] += d * s;
     }
-    if (j >= R - 1 && j <= a.nz - R - 1) {
+    if (z >= R - 1 && z <= a.nz - R - 1) {
         float s = 0.f;
 #pragma unroll
         for (int m = 1; m <= R; ++m) s += a.cz[m - 1] * (gin[c + m * nxl] - gin[c - (m - 1) * nxl]);
-        a.szy[p] += d * s;
+        szy[p] += d * s;
     }
 }
 
-static dim3 step_grid(const StepArgs& a) { return dim3((a.nx + kBX - 1) / kBX, (a.nz + kBZ - 1) / kBZ); }
-
-void launch_velocity(int R, const StepArgs& a, hipStream_t s)
+void launch_step(int R, bool with_u, const StepArgs& a, hipStream_t s)
 {
-    const dim3 g = step_grid(a), b(kBX, kBZ);
+    const dim3 g((unsigned)(kXCD * a.tiles_per_xcd)), b(256);
+#define FD2D_TILE(RR)                                                                            \
+    case RR:                                                                                     \
+        if (with_u) hipLaunchKernelGGL((k_tile<RR, 1>), g, b, 0, s, a);                          \
+        else hipLaunchKernelGGL((k_tile<RR, 0>), g, b, 0, s, a);                                 \
+        break;
     switch (R) {
-    case 1: hipLaunchKernelGGL(k_velocity<1>, g, b, 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_velocity<2>, g, b, 0, s, a); break;
-    case 3: hipLaunchKernelGGL(k_velocity<3>, g, b, 0, s, a); break;
-    default: hipLaunchKernelGGL(k_velocity<4>, g, b, 0, s, a); break;
+        FD2D_TILE(1)
+        FD2D_TILE(2)
+        FD2D_TILE(3)
+        FD2D_TILE(4)
     }
+#undef FD2D_TILE
 }
 
-void launch_stress(int R, const StepArgs& a, hipStream_t s)
+void launch_born_green(int R, const StepArgs& a, float* sxy, float* szy, const float* dtdmu, const float* gin,
+                       hipStream_t s)
 {
-    const dim3 g = step_grid(a), b(kBX, kBZ);
+    const long n = (long)a.nx * a.nz;
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
     switch (R) {
-    case 1: hipLaunchKernelGGL(k_stress<1>, g, b, 0, s, a); break;
-    case 2: hipLaunchKernelGGL(k_stress<2>, g, b, 0, s, a); break;
-    case 3: hipLaunchKernelGGL(k_stress<3>, g, b, 0, s, a); break;
-    default: hipLaunchKernelGGL(k_stress<4>, g, b, 0, s, a); break;
-    }
-}
-
-void launch_born_green(int R, const StepArgs& a, const float* dtdmu, const float* gin, hipStream_t s)
-{
-    const dim3 g = step_grid(a), b(kBX, kBZ);
-    switch (R) {
-    case 1: hipLaunchKernelGGL(k_born_green<1>, g, b, 0, s, a, dtdmu, gin); break;
-    case 2: hipLaunchKernelGGL(k_born_green<2>, g, b, 0, s, a, dtdmu, gin); break;
-    case 3: hipLaunchKernelGGL(k_born_green<3>, g, b, 0, s, a, dtdmu, gin); break;
-    default: hipLaunchKernelGGL(k_born_green<4>, g, b, 0, s, a, dtdmu, gin); break;
+    case 1: hipLaunchKernelGGL(k_born_green<1>, g, b, 0, s, a, sxy, szy, dtdmu, gin); break;
+    case 2: hipLaunchKernelGGL(k_born_green<2>, g, b, 0, s, a, sxy, szy, dtdmu, gin); break;
+    case 3: hipLaunchKernelGGL(k_born_green<3>, g, b, 0, s, a, sxy, szy, dtdmu, gin); break;
+    default: hipLaunchKernelGGL(k_born_green<4>, g, b, 0, s, a, sxy, szy, dtdmu, gin); break;
     }
 }
 
@@ -237,17 +427,6 @@ void launch_scale_to_float(const double* in, float* out, size_t n, double scale,
     hipLaunchKernelGGL(k_scale_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n, scale);
 }
 
-__global__ void k_copy_f64(double* dst, const double* src, size_t n)
-{
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < n) dst[c] = src[c];
-}
-
-void launch_copy_f64(double* dst, const double* src, size_t n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_copy_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, n);
-}
-
 __global__ void k_stf_real(float* amp, const double2* stf, int64_t d0, int nq, int nsteps)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,92 +445,137 @@ void launch_stf_real(float* amp, const double2* stf, int64_t d0, int nq, int nst
 // ---------------------------------------------------------------------------------------------
 // Frequency domain
 // ---------------------------------------------------------------------------------------------
-constexpr int kDftKB = 8;      // complex frequencies per wavefront pass
-constexpr int kMaxSlab = 128;  // time samples per flush / synthesis block
+// Both contractions are tall-skinny FP64 products over the P grid nodes:
+//   flush:  acc[k][x] += sum_j samp[j][x] * coef[j][k]       (P x cnt) * (cnt x K)
+//   synth:  out[j][x]  = sum_k Re(A[k][x] * coef[j][k])      (P x K) * (K x cnt)
+// One thread owns one node and a chunk of 16 (flush: frequencies; synth: time slices) FP64
+// accumulators; the phase-table chunk is staged in LDS and read as wave-uniform broadcasts. The
+// workgroups that share a 256-node block (one per chunk) are placed on the same XCD one after the
+// other, so the block's samples (flush) or spectrum (synth) come from that XCD's L2 after the first.
+constexpr int kChunk = 16;
 
-// One workgroup = 64 nodes x all K frequencies; the 4 wavefronts split the frequencies. The
-// node's samples of the block are staged once in LDS; the phase table index is wave-uniform.
-__global__ __launch_bounds__(256) void k_dft_flush(double2* __restrict__ acc, const float* __restrict__ samp,
-                                                   const double2* __restrict__ coef, size_t P, int K, int cnt)
+// (node block, chunk) of workgroup g when n_blocks is padded to a multiple of kXCD.
+__device__ __forceinline__ void xcd_block(int g, int n_chunks, int& block, int& chunk)
 {
-    __shared__ float tile[kMaxSlab * 64];
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const size_t x0 = (size_t)blockIdx.x * 64;
-    const size_t x = x0 + lane;
-    for (int jj = w; jj < cnt; jj += 4) tile[jj * 64 + lane] = x < P ? samp[(size_t)jj * P + x] : 0.f;
+    const int xcd = g % kXCD, q = g / kXCD;
+    chunk = q % n_chunks;
+    block = (q / n_chunks) * kXCD + xcd;
+}
+
+__global__ __launch_bounds__(256) void k_dft_flush(double2* __restrict__ acc, const float* __restrict__ samp,
+                                                   const double2* __restrict__ coef, size_t P, int K, int cnt,
+                                                   int n_blocks, int n_chunks)
+{
+    __shared__ double2 tab[kSlab * kChunk];
+    int block, chunk;
+    xcd_block(blockIdx.x, n_chunks, block, chunk);
+    if (block >= n_blocks) return;
+    const int k0 = chunk * kChunk;
+    const int nk = K - k0 < kChunk ? K - k0 : kChunk;
+    for (int idx = threadIdx.x; idx < cnt * kChunk; idx += 256) {
+        const int j = idx / kChunk, kk = idx - j * kChunk;
+        tab[idx] = kk < nk ? coef[(size_t)j * K + k0 + kk] : make_double2(0.0, 0.0);
+    }
     __syncthreads();
-    for (int k0 = w * kDftKB; k0 < K; k0 += 4 * kDftKB) {
-        double re[kDftKB], im[kDftKB];
+    const size_t x = (size_t)block * 256 + threadIdx.x;
+    if (x >= P) return;
+    double re[kChunk], im[kChunk];
 #pragma unroll
-        for (int kk = 0; kk < kDftKB; ++kk) re[kk] = im[kk] = 0.0;
-        const int nk = K - k0 < kDftKB ? K - k0 : kDftKB;
-        for (int jj = 0; jj < cnt; ++jj) {
-            const double b = (double)tile[jj * 64 + lane];
-            const double2* cr = coef + (size_t)jj * K + k0;
+    for (int kk = 0; kk < kChunk; ++kk) re[kk] = im[kk] = 0.0;
+    const float* sp = samp + x;
+    int j = 0;
+    for (; j + 4 <= cnt; j += 4) {
+        const double b0 = sp[(size_t)j * P], b1 = sp[(size_t)(j + 1) * P];
+        const double b2 = sp[(size_t)(j + 2) * P], b3 = sp[(size_t)(j + 3) * P];
 #pragma unroll
-            for (int kk = 0; kk < kDftKB; ++kk) {
-                if (kk < nk) {
-                    const double2 cc = cr[kk];
-                    re[kk] = fma(b, cc.x, re[kk]);
-                    im[kk] = fma(b, cc.y, im[kk]);
-                }
-            }
+        for (int kk = 0; kk < kChunk; ++kk) {
+            const double2 c0 = tab[j * kChunk + kk], c1 = tab[(j + 1) * kChunk + kk];
+            const double2 c2 = tab[(j + 2) * kChunk + kk], c3 = tab[(j + 3) * kChunk + kk];
+            re[kk] = fma(b0, c0.x, fma(b1, c1.x, fma(b2, c2.x, fma(b3, c3.x, re[kk]))));
+            im[kk] = fma(b0, c0.y, fma(b1, c1.y, fma(b2, c2.y, fma(b3, c3.y, im[kk]))));
         }
-        if (x < P) {
+    }
+    for (; j < cnt; ++j) {
+        const double b = sp[(size_t)j * P];
 #pragma unroll
-            for (int kk = 0; kk < kDftKB; ++kk) {
-                if (kk < nk) {
-                    double2* d = acc + (size_t)(k0 + kk) * P + x;
-                    double2 v = *d;
-                    v.x += re[kk];
-                    v.y += im[kk];
-                    *d = v;
-                }
-            }
+        for (int kk = 0; kk < kChunk; ++kk) {
+            const double2 c = tab[j * kChunk + kk];
+            re[kk] = fma(b, c.x, re[kk]);
+            im[kk] = fma(b, c.y, im[kk]);
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < kChunk; ++kk) {
+        if (kk < nk) {
+            double2* d = acc + (size_t)(k0 + kk) * P + x;
+            double2 v = *d;
+            v.x += re[kk];
+            v.y += im[kk];
+            *d = v;
         }
     }
 }
 
-void launch_dft_flush(double2* acc, const float* samp, const double2* coef, size_t P, int K, int cnt,
-                      hipStream_t s)
+void launch_dft_flush(double2* acc, const float* samp, const double2* coef, size_t P, int K, int cnt, hipStream_t s)
 {
-    if (cnt <= 0) return;
-    hipLaunchKernelGGL(k_dft_flush, dim3((unsigned)((P + 63) / 64)), dim3(256), 0, s, acc, samp, coef, P, K, cnt);
+    if (cnt <= 0 || K <= 0) return;
+    const int n_blocks = (int)((P + 255) / 256);
+    const int n_chunks = (K + kChunk - 1) / kChunk;
+    const int padded = (n_blocks + kXCD - 1) / kXCD * kXCD;
+    hipLaunchKernelGGL(k_dft_flush, dim3((unsigned)(padded * n_chunks)), dim3(256), 0, s, acc, samp, coef, P, K, cnt,
+                       n_blocks, n_chunks);
 }
 
-constexpr int kSynJB = 32;  // time slices per pass of the synthesis kernel
+constexpr int kSynK = 32;  // frequencies per LDS stage of the synthesis
 
 __global__ __launch_bounds__(256) void k_synth(float* __restrict__ out, const double2* __restrict__ A,
-                                               const double2* __restrict__ coef, size_t P, int K, int cnt)
+                                               const double2* __restrict__ coef, size_t P, int K, int cnt,
+                                               int n_blocks, int n_chunks)
 {
-    const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= P) return;
-    for (int j0 = 0; j0 < cnt; j0 += kSynJB) {
-        const int nj = cnt - j0 < kSynJB ? cnt - j0 : kSynJB;
-        double s[kSynJB];
+    __shared__ double2 tab[kSynK * kChunk];  // [k][j] for this chunk of time slices
+    int block, chunk;
+    xcd_block(blockIdx.x, n_chunks, block, chunk);
+    if (block >= n_blocks) return;
+    const int j0 = chunk * kChunk;
+    const int nj = cnt - j0 < kChunk ? cnt - j0 : kChunk;
+    const size_t x = (size_t)block * 256 + threadIdx.x;
+    const bool live = x < P;
+    double s[kChunk];
 #pragma unroll
-        for (int jj = 0; jj < kSynJB; ++jj) s[jj] = 0.0;
-        for (int k = 0; k < K; ++k) {
-            const double2 av = A[(size_t)k * P + x];
+    for (int jj = 0; jj < kChunk; ++jj) s[jj] = 0.0;
+    for (int kb = 0; kb < K; kb += kSynK) {
+        const int nkb = K - kb < kSynK ? K - kb : kSynK;
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < kSynK * kChunk; idx += 256) {
+            const int kk = idx / kChunk, jj = idx - kk * kChunk;
+            tab[idx] = (kk < nkb && jj < nj) ? coef[(size_t)(j0 + jj) * K + kb + kk] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        if (live) {
+            for (int kk = 0; kk < nkb; ++kk) {
+                const double2 av = A[(size_t)(kb + kk) * P + x];
 #pragma unroll
-            for (int jj = 0; jj < kSynJB; ++jj) {
-                if (jj < nj) {
-                    const double2 cc = coef[(size_t)(j0 + jj) * K + k];
+                for (int jj = 0; jj < kChunk; ++jj) {
+                    const double2 cc = tab[kk * kChunk + jj];
                     s[jj] = fma(av.x, cc.x, fma(-av.y, cc.y, s[jj]));
                 }
             }
         }
-#pragma unroll
-        for (int jj = 0; jj < kSynJB; ++jj)
-            if (jj < nj) out[(size_t)(j0 + jj) * P + x] = (float)s[jj];
     }
+    if (!live) return;
+#pragma unroll
+    for (int jj = 0; jj < kChunk; ++jj)
+        if (jj < nj) out[(size_t)(j0 + jj) * P + x] = (float)s[jj];
 }
 
 void launch_synth(float* out, const double2* A, const double2* coef, size_t P, int K, int cnt, hipStream_t s)
 {
-    if (cnt <= 0) return;
-    hipLaunchKernelGGL(k_synth, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, out, A, coef, P, K, cnt);
+    if (cnt <= 0 || K <= 0) return;
+    const int n_blocks = (int)((P + 255) / 256);
+    const int n_chunks = (cnt + kChunk - 1) / kChunk;
+    const int padded = (n_blocks + kXCD - 1) / kXCD * kXCD;
+    hipLaunchKernelGGL(k_synth, dim3((unsigned)(padded * n_chunks)), dim3(256), 0, s, out, A, coef, P, K, cnt,
+                       n_blocks, n_chunks);
 }
 
 __global__ void k_source_spectrum(double2* A, const double2* Z, int conj_in, const double* sd, int n_sd,

@@ -193,12 +193,12 @@ int fd2d_query_output_dims(const fd2d_config* cfg, int32_t solver, double mode, 
 
 /*
  * Step-kernel timing (instrumentation; it has no counterpart in the reference). When enabled with
- * every > 0, every every-th time step of each solve brackets its velocity and stress launches with
- * HIP events on the solve's stream; the durations are summed per solver until fd2d_timing_read(),
- * which returns and clears them. every = 0 disables it (the default).
+ * every > 0, every every-th time step of each solve brackets its fused step launch with HIP events
+ * on the solve's stream; the durations are summed per solver until fd2d_timing_read(), which
+ * returns and clears them. every = 0 disables it (the default).
  */
 typedef struct fd2d_timing {
-    int64_t launches[3][2];  /* [solver-1][0: velocity kernel, 1: stress kernel] timed launches */
+    int64_t launches[3][2];  /* [solver-1][0: fused step kernel, 1: reserved (0)] timed launches */
     double ms[3][2];         /* summed kernel durations [ms] */
     int64_t points[3];       /* nx*nz of the last timed solve of each solver */
 } fd2d_timing;
