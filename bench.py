@@ -80,6 +80,10 @@ class Pipeline:
         self.stf_fft = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
         self.seis_a = D.empty_like_dims((nr, c.n_corr), f64)
         self.t = c.t_corr()
+        self.t_d = torch.as_tensor(self.t, device=cu)
+        self.stf = torch.zeros((c.n_corr, nr), dtype=cd, device=cu)
+        self.misfit = torch.zeros(nr, dtype=f64, device=cu)
+        self.adstf = torch.zeros((c.n_corr, nr), dtype=f64, device=cu)
         # make_adjoint_sources.m:85-86 measures in [100, 200] s; runs truncated in time for
         # profiling scale the window with the trace length instead
         tmax = self.t[-1]
@@ -92,23 +96,27 @@ class Pipeline:
         D.run_forward1_green(self.mu_obs, self.rho, self.src, None, 0, None, None, config=c, G_fft=self.G_fft)
         D.run_forward2_correlation(self.mu_obs, self.rho, self.G_fft, self.spec, self.sd, self.rec, 0, None, None,
                                    config=c, seismograms=self.corr)
-        self.obs = D.to_host(self.corr).copy()
+        self.obs = self.corr.clone()
+
+    def adjoint_sources(self):
+        """cc_time_shift misfits and adjoint sources on the device, into the complex stf buffer."""
+        from fd2d_noise_amd.measurement import make_adjoint_sources_device
+        make_adjoint_sources_device(self.corr, self.obs, None, self.t_d, "dis", "cc_time_shift", window=self.window,
+                                    misfit=self.misfit, adstf=self.adstf)
+        self.torch.view_as_real(self.stf)[..., 0].copy_(self.adstf)  # complex(adstf)
 
     def step(self):
-        from fd2d_noise_amd.measurement import make_adjoint_sources
         D, c = self.D, self.c
         D.run_forward1_green(self.mu, self.rho, self.src, None, 1, None, None, config=c, G_fft=self.G_fft,
                              G_out=self.G_out)
         D.run_forward2_correlation(self.mu, self.rho, self.G_fft, self.spec, self.sd, self.rec, 1, None, None,
                                    config=c, seismograms=self.corr, C_out=self.C_out)
-        u = D.to_host(self.corr)
-        misfit, adstf = make_adjoint_sources(u, self.obs, None, self.t, "dis", "cc_time_shift", window=self.window)
-        stf = D.to_device(adstf.astype(np.complex128))
-        D.run_noise_adjoint(self.mu, self.rho, self.C_out, stf, self.rec, self.rec, None, None, None, 1, None, None,
+        self.adjoint_sources()
+        D.run_noise_adjoint(self.mu, self.rho, self.C_out, self.stf, self.rec, self.rec, None, None, None, 1, None, None,
                             config=c, K=self.K1, stf_fft=self.stf_fft, seismograms=self.seis_a)
         D.run_noise_adjoint(self.mu, self.rho, self.G_out, self.stf_fft, self.rec, self.rec, self.spec, self.sd, None,
                             0, None, None, config=c, K=self.K2)
-        return float(misfit.sum())
+        return float(self.misfit.sum())
 
     def gp_updates(self):
         c = self.c
@@ -123,10 +131,7 @@ class Pipeline:
         if which in ("adjoint1", "adjoint0"):
             D.run_forward2_correlation(self.mu, self.rho, self.G_fft, self.spec, self.sd, self.rec, 1, None, None,
                                        config=c, seismograms=self.corr, C_out=self.C_out)
-            from fd2d_noise_amd.measurement import make_adjoint_sources
-            _, adstf = make_adjoint_sources(D.to_host(self.corr), self.obs, None, self.t, "dis", "cc_time_shift",
-                                            window=self.window)
-            self.stf = D.to_device(adstf.astype(np.complex128))
+            self.adjoint_sources()
         if which == "adjoint0":
             D.run_noise_adjoint(self.mu, self.rho, self.C_out, self.stf, self.rec, self.rec, None, None, None, 1, None,
                                 None, config=c, K=self.K1, stf_fft=self.stf_fft, seismograms=self.seis_a)

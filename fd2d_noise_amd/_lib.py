@@ -52,6 +52,9 @@ EXPORTS = (
     "fd2d_run_forward1_green_device", "fd2d_run_forward2_correlation_device",
     "fd2d_run_noise_adjoint_device", "fd2d_query_output_dims", "fd2d_last_error", "fd2d_version",
     "fd2d_timing_enable", "fd2d_timing_read",
+    "fd2d_make_adjoint_sources", "fd2d_make_adjoint_sources_device", "fd2d_dot_device", "fd2d_axpby_device",
+    "fd2d_gaussblur2d_device", "fd2d_map_m_to_parameters_device", "fd2d_map_gradparameters_to_gradm_device",
+    "fd2d_regularization_device", "fd2d_general_source_device",
 )
 
 

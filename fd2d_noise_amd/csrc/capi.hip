@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/fd2d_noise.h"
+#include "caller.h"
 #include "engine.h"
 
 using namespace fd2d;
@@ -204,8 +205,9 @@ std::vector<double> interpol_matrix(int K, int nb)
     return I;
 }
 
-// Separable Gaussian taper of code/propagation/init_absbound.m:16-30: absbound(i,j) = ax(i)*az(j).
-void taper(const fd2d_config* cfg, const Geometry& g, std::vector<float>& ax, std::vector<float>& az)
+// Separable Gaussian taper of code/propagation/init_absbound.m:16-30: absbound(i,j) = ax(i)*az(j)
+// (FP64, as the reference builds it).
+void taper_d(const fd2d_config* cfg, const Geometry& g, std::vector<double>& ax, std::vector<double>& az)
 {
     const double w = cfg->absorb_width, Lx = cfg->Lx, Lz = cfg->Lz;
     auto lo_side = [w](double X) { return (double)(X > w) + std::exp(-(X - w) * (X - w) / ((2 * w) * (2 * w))) * (double)(X <= w); };
@@ -219,15 +221,23 @@ void taper(const fd2d_config* cfg, const Geometry& g, std::vector<float>& ax, st
         double a = 1.0;
         if (cfg->absorb_left == 1) a *= lo_side(X);
         if (cfg->absorb_right == 1) a *= hi_side(X, Lx);
-        ax[i] = (float)a;
+        ax[i] = a;
     }
     for (int j = 0; j < g.nz; ++j) {
         const double Z = j * g.dz;
         double a = 1.0;
         if (cfg->absorb_bottom == 1) a *= lo_side(Z);
         if (cfg->absorb_top == 1) a *= hi_side(Z, Lz);
-        az[j] = (float)a;
+        az[j] = a;
     }
+}
+
+void taper(const fd2d_config* cfg, const Geometry& g, std::vector<float>& ax, std::vector<float>& az)
+{
+    std::vector<double> axd, azd;
+    taper_d(cfg, g, axd, azd);
+    ax.assign(axd.begin(), axd.end());
+    az.assign(azd.begin(), azd.end());
 }
 
 // Copies a host table to a new device buffer; waits for the copy so the host vector may go away.
@@ -931,6 +941,55 @@ void require_device()
     }
 }
 
+// Interior block of the parameter map: [ix, iz] = find(absbound == 1, 1, 'first')
+// (map_m_to_parameters.m:18-19), 0-based, with the block rows ix..nx-1-ix, columns iz..nz-1-iz.
+struct MapFrame {
+    int ix, iz, n1, n2;
+    std::vector<double> x1, x2;  // coordinates of the block (define_computational_domain.m:51-52)
+};
+
+MapFrame map_frame(const fd2d_config* cfg, const Geometry& g)
+{
+    std::vector<double> ax, az;
+    taper_d(cfg, g, ax, az);
+    MapFrame f{-1, -1, 0, 0, {}, {}};
+    // the first node of absbound == 1 in column-major order: lowest column with az == 1, then the
+    // lowest row with ax == 1 (absbound = ax * az, each <= 1, is 1 only where both are)
+    for (int j = 0; j < g.nz && f.iz < 0; ++j)
+        if (az[(size_t)j] == 1.0) f.iz = j;
+    for (int i = 0; i < g.nx && f.ix < 0; ++i)
+        if (ax[(size_t)i] == 1.0) f.ix = i;
+    if (f.ix < 0 || f.iz < 0) raise(FD2D_EINVAL, "the absorbing frame leaves no node with absbound == 1");
+    f.n1 = g.nx - 2 * f.ix;
+    f.n2 = g.nz - 2 * f.iz;
+    if (f.n1 < 1 || f.n2 < 1) raise(FD2D_EINVAL, "the absorbing frame leaves an empty interior block");
+    for (int i = 0; i < f.n1; ++i) f.x1.push_back((f.ix + i) * g.dx);
+    for (int j = 0; j < f.n2; ++j) f.x2.push_back((f.iz + j) * g.dz);
+    return f;
+}
+
+void check_sigma(const double* s, const char* what)
+{
+    if (!s || !(s[0] > 0) || !(s[1] > 0)) raise(FD2D_EINVAL, "%s must hold two positive widths", what);
+}
+
+// Blurs the interior block of each of n_maps nx x nz maps of src into dst (dst may equal src).
+void blur_maps(const Geometry& g, const MapFrame& f, const double* src, double* dst, int n_maps,
+               const double* sig_src, const double* sig_str, hipStream_t st)
+{
+    DBuf<double> x1, x2, tmp((size_t)f.n1 * f.n2);
+    upload(x1, f.x1, st);
+    upload(x2, f.x2, st);
+    const size_t off = (size_t)f.ix + (size_t)g.nx * f.iz;
+    for (int q = 0; q < n_maps; ++q) {
+        const double* sg = q < n_maps - 1 ? sig_src : sig_str;
+        launch_gaussblur(src + g.P * q + off, g.nx, dst + g.P * q + off, g.nx, tmp.p, x1.p, x2.p, f.n1, f.n2, sg[0],
+                         sg[1], st);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+}
+
 }  // namespace
 
 extern "C" {
@@ -1051,6 +1110,210 @@ int fd2d_run_noise_adjoint(const fd2d_config* cfg, const fd2d_carray* mu, const 
         if (im == 1 || im == 12 || im == 13) Staged::back(stf_fft, sf, 16);
         if (im == 11 || im == 13) Staged::back(u_out, uo, 4);
         Staged::back(seismograms, sg, 8);
+    });
+}
+
+// ---- callers: measurement, inner products, parameter map ------------------------------------
+int fd2d_make_adjoint_sources_device(const double* u, const double* u0, const double* du, const double* t, int64_t nr,
+                                     int64_t nt, int32_t veldis, int32_t measurement, int32_t deriv_order,
+                                     double win_left, double win_right, double* misfit, double* adstf, void* stream)
+{
+    return guarded([&] {
+        require_device();
+        if (nr < 1 || nt < 2 || !u || !u0 || !t || !misfit || !adstf) raise(FD2D_EINVAL, "u, u_0, t, misfit, adstf and nt >= 2 are required");
+        if (measurement < 0 || measurement > 3) raise(FD2D_EINVAL, "unknown measurement %d", measurement);
+        if (deriv_order != 1 && deriv_order != 2) raise(FD2D_EINVAL, "deriv_order must be 1 or 2");
+        if (measurement == FD2D_CC_TIME_SHIFT && deriv_order == 2)
+            raise(FD2D_EUNSUPPORTED, "no Hessian vector products for traveltime measurements (make_adjoint_sources.m:121)");
+        if (deriv_order == 2 && !du) raise(FD2D_EINVAL, "deriv_order 2 needs du");
+        hipStream_t st = as_stream(stream);
+        // make_adjoint_sources.m:85-99: window limits against the time axis
+        const std::vector<double> th = to_host<double>(t, (size_t)nt);
+        double left = win_left, right = win_right;
+        if (measurement != FD2D_WAVEFORM_DIFFERENCE) {
+            if (left < 0) {
+                int64_t k = 0;
+                while (k < nt && th[(size_t)k] != 0.0) ++k;
+                if (k + 1 >= nt) raise(FD2D_EINVAL, "time axis has no t == 0 sample");
+                left = th[(size_t)k + 1];
+            }
+            if (right > th[(size_t)nt - 1]) right = th[(size_t)nt - 1];
+            if (left > th[(size_t)nt - 1]) raise(FD2D_EINVAL, "time series too short? (make_adjoint_sources.m:99)");
+        }
+        DBuf<double> scratch((size_t)3 * nr * nt);
+        AdjArgs a{u, u0, du, t, nr, nt, th[1] - th[0] >= 0 ? th[1] - th[0] : th[0] - th[1], veldis ? 1 : 0,
+                  measurement, deriv_order, left, right, misfit, adstf, scratch.p};
+        launch_adjoint_sources(a, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int fd2d_make_adjoint_sources(const double* u, const double* u0, const double* du, const double* t, int64_t nr,
+                              int64_t nt, int32_t veldis, int32_t measurement, int32_t deriv_order, double win_left,
+                              double win_right, double* misfit, double* adstf)
+{
+    int rc = FD2D_OK;
+    const int st = guarded([&] {
+        require_device();
+        const size_t n = (size_t)(nr > 0 ? nr : 0) * (size_t)(nt > 0 ? nt : 0);
+        DBuf<double> du_d, u_d(n), u0_d(n), t_d((size_t)(nt > 0 ? nt : 0)), m_d((size_t)(nr > 0 ? nr : 0)), a_d(n);
+        if (!u || !u0 || !t || !misfit || !adstf || n == 0) raise(FD2D_EINVAL, "u, u_0, t, misfit, adstf are required");
+        HIPCHK(hipMemcpy(u_d.p, u, n * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(u0_d.p, u0, n * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(t_d.p, t, (size_t)nt * 8, hipMemcpyHostToDevice));
+        if (du) {
+            du_d.alloc(n);
+            HIPCHK(hipMemcpy(du_d.p, du, n * 8, hipMemcpyHostToDevice));
+        }
+        rc = fd2d_make_adjoint_sources_device(u_d.p, u0_d.p, du ? du_d.p : nullptr, t_d.p, nr, nt, veldis, measurement,
+                                              deriv_order, win_left, win_right, m_d.p, a_d.p, nullptr);
+        if (rc == FD2D_OK) {
+            HIPCHK(hipMemcpy(misfit, m_d.p, (size_t)nr * 8, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(adstf, a_d.p, n * 8, hipMemcpyDeviceToHost));
+        }
+    });
+    return st != FD2D_OK ? st : rc;
+}
+
+int fd2d_dot_device(const double* x, const double* y, int64_t n, double* result, void* stream)
+{
+    return guarded([&] {
+        require_device();
+        if (!result || n < 0 || (n > 0 && (!x || !y))) raise(FD2D_EINVAL, "dot: null argument");
+        hipStream_t st = as_stream(stream);
+        const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
+        DBuf<double> part((size_t)2 * nb + 1);
+        launch_dot(x, y, n, part.p, nb, part.p + 2 * nb, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(result, part.p + 2 * nb, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int fd2d_axpby_device(double alpha, const double* x, double beta, const double* y, double* out, int64_t n, void* stream)
+{
+    return guarded([&] {
+        require_device();
+        if (n < 0 || (n > 0 && (!x || !y || !out))) raise(FD2D_EINVAL, "axpby: null argument");
+        hipStream_t st = as_stream(stream);
+        if (n) launch_axpby(alpha, x, beta, y, out, n, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int fd2d_gaussblur2d_device(const double* Y, int32_t n1, int32_t n2, int32_t ldy, const double* x1, const double* x2,
+                            double sigma1, double sigma2, double* GY, int32_t ldg, void* stream)
+{
+    return guarded([&] {
+        require_device();
+        if (!Y || !GY || !x1 || !x2 || n1 < 1 || n2 < 1 || ldy < n1 || ldg < n1)
+            raise(FD2D_EINVAL, "gaussblur2d: bad arguments");
+        if (!(sigma1 > 0) || !(sigma2 > 0)) raise(FD2D_EINVAL, "gaussblur2d: sigma must be positive");
+        hipStream_t st = as_stream(stream);
+        DBuf<double> tmp((size_t)n1 * n2);
+        launch_gaussblur(Y, ldy, GY, ldg, tmp.p, x1, x2, n1, n2, sigma1, sigma2, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int fd2d_general_source_device(const double* spectrum, int32_t n_sample, int32_t n_src, const double* dist,
+                               int64_t P, int32_t n_basis_fct, double* dist_basis, void* stream)
+{
+    return guarded([&] {
+        require_device();
+        if (!spectrum || !dist || !dist_basis || n_sample < 1 || n_src < 1 || P < 1 || n_basis_fct < 1)
+            raise(FD2D_EINVAL, "general_source: bad arguments");
+        if (n_basis_fct > n_sample) raise(FD2D_EINVAL, "general_source: n_basis_fct > n_sample leaves empty bands");
+        // integration_limits.m:4-12 (1-based limits, stored 0-based)
+        std::vector<int> lo((size_t)n_basis_fct), hi((size_t)n_basis_fct);
+        const double step = (double)n_sample / (double)n_basis_fct;
+        for (int b = 1; b <= n_basis_fct; ++b) {
+            lo[(size_t)b - 1] = (int)std::floor((b - 1) * step + 1) - 1;
+            hi[(size_t)b - 1] = (int)std::floor(b * step) - 1;
+        }
+        hipStream_t st = as_stream(stream);
+        DBuf<int> lo_d, hi_d;
+        upload(lo_d, lo, st);
+        upload(hi_d, hi, st);
+        launch_general_source(spectrum, n_sample, n_src, dist, P, lo_d.p, hi_d.p, n_basis_fct, dist_basis, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int fd2d_map_m_to_parameters_device(const fd2d_config* cfg, const double* m, int32_t n_maps,
+                                    const double sigma_source[2], const double sigma_structure[2],
+                                    double ref_source, double ref_structure, double mu_0, double* m_parameters,
+                                    void* stream)
+{
+    return guarded([&] {
+        require_device();
+        const Geometry g = geometry(cfg);
+        if (!m || !m_parameters || n_maps < 1) raise(FD2D_EINVAL, "m, m_parameters and n_maps >= 1 are required");
+        check_sigma(sigma_source, "sigma_source");
+        check_sigma(sigma_structure, "sigma_structure");
+        const MapFrame f = map_frame(cfg, g);
+        hipStream_t st = as_stream(stream);
+        const size_t n = g.P * (size_t)n_maps;
+        // outside the frame the maps pass through unchanged (map_m_to_parameters.m:16)
+        if (m_parameters != m) HIPCHK(hipMemcpyAsync(m_parameters, m, n * 8, hipMemcpyDeviceToDevice, st));
+        blur_maps(g, f, m, m_parameters, n_maps, sigma_source, sigma_structure, st);
+        // :48-49
+        if (n_maps > 1) launch_affine(m_parameters, (int64_t)(g.P * (size_t)(n_maps - 1)), 1.0, ref_source, st);
+        launch_affine(m_parameters + g.P * (size_t)(n_maps - 1), (int64_t)g.P, mu_0, ref_structure, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int fd2d_map_gradparameters_to_gradm_device(const fd2d_config* cfg, const double* grad_parameters, int32_t n_maps,
+                                            const double sigma_source[2], const double sigma_structure[2],
+                                            double mu_0, double* grad_m, void* stream)
+{
+    return guarded([&] {
+        require_device();
+        const Geometry g = geometry(cfg);
+        if (!grad_parameters || !grad_m || n_maps < 1)
+            raise(FD2D_EINVAL, "grad_parameters, grad_m and n_maps >= 1 are required");
+        if (grad_parameters == grad_m) raise(FD2D_EINVAL, "grad_m must not alias grad_parameters");
+        check_sigma(sigma_source, "sigma_source");
+        check_sigma(sigma_structure, "sigma_structure");
+        const MapFrame f = map_frame(cfg, g);
+        hipStream_t st = as_stream(stream);
+        const int64_t n = (int64_t)(g.P * (size_t)n_maps);
+        // temp = 0.0 * grad_parameters (map_gradparameters_to_gradm.m:70), then the blurred frame
+        launch_axpby(0.0, grad_parameters, 0.0, grad_parameters, grad_m, n, st);
+        blur_maps(g, f, grad_parameters, grad_m, n_maps, sigma_source, sigma_structure, st);
+        launch_affine(grad_m + g.P * (size_t)(n_maps - 1), (int64_t)g.P, mu_0, 0.0, st);  // :101
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+int fd2d_regularization_device(double* j, double* g, const double* m, const double* m0, int64_t n, int64_t n_struct,
+                               const double* w, int64_t n_w, double alpha, double beta, void* stream)
+{
+    return guarded([&] {
+        require_device();
+        if (!j || !m || !m0 || !w || n < 1 || n_struct < 1 || n_struct > n || n_w < 1)
+            raise(FD2D_EINVAL, "regularization: bad arguments");
+        const int64_t n_src = n - n_struct;
+        if (n_src % n_w != 0 || (n_w != 1 && n_w != n_struct))
+            raise(FD2D_EINVAL, "weighting must hold 1 or nx*nz values (regularization.m:115)");
+        hipStream_t st = as_stream(stream);
+        const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
+        DBuf<double> part((size_t)4 * nb + 2);
+        launch_regularization(m, m0, w, n_w, n, n_src, alpha, beta, g, part.p, nb, part.p + 4 * nb, st);
+        HIPCHK(hipGetLastError());
+        double sums[2];
+        HIPCHK(hipMemcpyAsync(sums, part.p + 4 * nb, sizeof sums, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        // regularization.m:114-120, in the reference's order
+        *j = *j + alpha / 2 * sums[0];
+        *j = *j + beta / 2 * sums[1];
     });
 }
 

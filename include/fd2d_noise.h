@@ -191,6 +191,87 @@ int fd2d_run_noise_adjoint_device(const fd2d_config* cfg,
 int fd2d_query_output_dims(const fd2d_config* cfg, int32_t solver, double mode, int64_t n_rec,
                            int64_t dims[4][3]);
 
+/* ---------------------------------------------------------------------------------------------
+ * Callers of the solvers, FP64 on the device
+ * ------------------------------------------------------------------------------------------- */
+
+/* Measurements of tools/measurement/make_adjoint_sources.m:31 (the 'measurement' argument). */
+enum fd2d_measurement {
+    FD2D_WAVEFORM_DIFFERENCE = 0,   /* waveform_difference.m */
+    FD2D_CC_TIME_SHIFT = 1,         /* cc_time_shift.m */
+    FD2D_AMPLITUDE_DIFFERENCE = 2,  /* amp_diff.m */
+    FD2D_LOG_AMPLITUDE_RATIO = 3    /* log_amp_ratio.m */
+};
+
+/*
+ * [misfit_n, adstf] = make_adjoint_sources(u, u_0, du, t, veldis, measurement, src, rec, deriv_order)
+ * tools/measurement/make_adjoint_sources.m:31-175, FP64 with compensated sums.
+ *   u, u0, du    nr x nt double (du may be NULL for deriv_order 1); t: nt times
+ *   veldis       0 = 'dis', 1 = 'vel';  deriv_order 1 = '1st', 2 = '2nd'
+ *   win_left/right  the causal measurement window; the reference uses [100, 200] s
+ *                (make_adjoint_sources.m:85-86), clipped to t(end) as it does, FD2D_EINVAL if
+ *                win_left > t(end) ('time series too short')
+ *   misfit       out nr (misfit_n);  adstf  out nr x nt (time-reversed adjoint sources)
+ * cc_time_shift with deriv_order 2 returns FD2D_EUNSUPPORTED, as the reference errors there.
+ */
+int fd2d_make_adjoint_sources(const double* u, const double* u0, const double* du, const double* t, int64_t nr,
+                              int64_t nt, int32_t veldis, int32_t measurement, int32_t deriv_order,
+                              double win_left, double win_right, double* misfit, double* adstf);
+int fd2d_make_adjoint_sources_device(const double* u, const double* u0, const double* du, const double* t,
+                                     int64_t nr, int64_t nt, int32_t veldis, int32_t measurement,
+                                     int32_t deriv_order, double win_left, double win_right, double* misfit,
+                                     double* adstf, void* stream);
+
+/* x'*y of two device FP64 vectors (n = 0: 0, pointers may be NULL) with compensated summation (the inner products of
+ * inversion/optlib/optlib_lbfgs.m:51,92,102,110,175 and optlib_wolfe.m:59,85,105); result to host. */
+int fd2d_dot_device(const double* x, const double* y, int64_t n, double* result, void* stream);
+/* out = alpha*x + beta*y on the device (the model and direction updates of the same files). */
+int fd2d_axpby_device(double alpha, const double* x, double beta, const double* y, double* out, int64_t n,
+                      void* stream);
+/* GY = gaussblur2d(Y, x1, x2, [sigma1 sigma2]) (tools/addons/gaussblur2d.m:2-38) on an n1 x n2
+ * block of device arrays with leading dimensions ldy / ldg; x1, x2 device coordinate vectors. */
+int fd2d_gaussblur2d_device(const double* Y, int32_t n1, int32_t n2, int32_t ldy, const double* x1,
+                            const double* x2, double sigma1, double sigma2, double* GY, int32_t ldg,
+                            void* stream);
+
+/*
+ * dist_basis_fct = general_source(noise_spectrum, noise_source_distribution, n_basis_fct)
+ * (code/propagation/general_source.m:2-27, called by input/make_noise_source.m:224-227): band
+ * averages of spectrum x distribution over the bands of integration_limits(n_sample, n_basis_fct).
+ *   spectrum  n_sample x n_src FP64;  dist  nx x nz x n_src (P = nx*nz nodes);  dist_basis out
+ *   nx x nz x n_basis_fct. All device arrays.
+ */
+int fd2d_general_source_device(const double* spectrum, int32_t n_sample, int32_t n_src, const double* dist,
+                               int64_t P, int32_t n_basis_fct, double* dist_basis, void* stream);
+
+/*
+ * Parameter map of inversion/optlib_callback_fct/map_m_to_parameters.m:2-52 ('ring' switch 'no'):
+ * m (nx x nz x n_maps, FP64 device) -> m_parameters. Inside the absorbing frame (rows ix..nx-ix+1,
+ * columns iz..nz-iz+1, the first node where init_absbound == 1) every map is blurred with
+ * gaussblur2d (sigma_source for maps 1..n_maps-1, sigma_structure for the last); then
+ * maps 1..n_maps-1 get ref_source added and the last becomes mu_0 * (ref_structure + map).
+ * Geometry and absorbing frame come from cfg (Lx, Lz, nx, nz, absorb_*). In place allowed.
+ */
+int fd2d_map_m_to_parameters_device(const fd2d_config* cfg, const double* m, int32_t n_maps,
+                                    const double sigma_source[2], const double sigma_structure[2],
+                                    double ref_source, double ref_structure, double mu_0, double* m_parameters,
+                                    void* stream);
+/* map_gradparameters_to_gradm.m:53-106: the blur of the same frame applied to grad_parameters,
+ * zero outside the frame, the last map times mu_0. */
+int fd2d_map_gradparameters_to_gradm_device(const fd2d_config* cfg, const double* grad_parameters, int32_t n_maps,
+                                            const double sigma_source[2], const double sigma_structure[2],
+                                            double mu_0, double* grad_m, void* stream);
+/*
+ * [j, g] = regularization(j, g, m, usr_par) (inversion/addons/regularization.m:1-29):
+ *   j += alpha/2 sum w (m - m0)^2 over the source maps (the first n - n_struct entries),
+ *   j += beta/2 sum w (m - m0)^2 over the structure map (the last n_struct = nx*nz entries),
+ *   g += alpha w (m - m0) / beta w (m - m0) on the same parts (skipped when g is NULL, the
+ *   reference's scalar g). w has n_w = 1 or nx*nz entries (repeated over the source maps).
+ * *j is a host in/out scalar; m, m0, w, g are device arrays.
+ */
+int fd2d_regularization_device(double* j, double* g, const double* m, const double* m0, int64_t n, int64_t n_struct,
+                               const double* w, int64_t n_w, double alpha, double beta, void* stream);
+
 /*
  * Step-kernel timing (instrumentation; it has no counterpart in the reference). When enabled with
  * every > 0, every every-th time step of each solve brackets its fused step launch with HIP events
