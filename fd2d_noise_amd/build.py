@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfd2d_noise_amd.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kernels.hip", "caller.hip", "capi.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("kernels.hip", "spectral.hip", "caller.hip", "capi.hip")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -416,14 +416,25 @@ struct Solver {
     void kmu(const float* fwd, double* kmu_acc, double scale) { launch_kmu(g.R, a, u[cur], fwd, kmu_acc, scale, st); }
 };
 
-// Blocked frequency-domain helpers: time samples are staged in FP32 slabs of up to kSlab slices.
+// Blocked frequency-domain helpers: time samples are staged in FP32 slabs of up to kSpecSlab slices
+// (fewer when device memory is short) and contracted on the matrix cores (csrc/spectral.hip).
+int spec_cols(int K) { return (2 * K + 255) / 256 * 256; }
+
+int slab_len(size_t P, size_t n_samples)
+{
+    size_t T = std::min<size_t>(kSpecSlab, std::max<size_t>(n_samples, 1));
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+        while (T > 64 && T * P * sizeof(float) > free_b / 8) T /= 2;
+    return (int)T;
+}
 
 struct DftAccumulator {
     double2* acc = nullptr;
     size_t P = 0;
-    int K = 0;
+    int K = 0, C_pad = 0;
     DBuf<float> slab;
-    DBuf<double2> coef;
+    DBuf<float> etab;  // [n_samples][C_pad]: (Re, Im) of the phase factor of each sample, FP32
     int cnt = 0, done = 0, T = 0;
     hipStream_t st = nullptr;
     void init(double2* acc_, size_t P_, int K_, const std::vector<double2>& table, size_t n_samples, hipStream_t s)
@@ -432,9 +443,16 @@ struct DftAccumulator {
         P = P_;
         K = K_;
         st = s;
-        T = (int)std::min<size_t>(kSlab, std::max<size_t>(n_samples, 1));
+        C_pad = spec_cols(K);
+        T = slab_len(P, n_samples);
         slab.alloc((size_t)T * P);
-        upload(coef, table, st);
+        std::vector<float> e(n_samples * (size_t)C_pad, 0.f);
+        for (size_t j = 0; j < n_samples; ++j)
+            for (int k = 0; k < K; ++k) {
+                e[j * C_pad + 2 * k] = (float)table[j * K + k].x;
+                e[j * C_pad + 2 * k + 1] = (float)table[j * K + k].y;
+            }
+        upload(etab, e, st);
         HIPCHK(hipMemsetAsync(acc, 0, P * (size_t)K * sizeof(double2), st));
     }
     float* next_slot() { return slab.p + (size_t)cnt * P; }
@@ -445,35 +463,49 @@ struct DftAccumulator {
     void flush()
     {
         if (cnt == 0) return;
-        launch_dft_flush(acc, slab.p, coef.p + (size_t)done * K, P, K, cnt, st);
+        launch_dft_mfma(acc, slab.p, etab.p + (size_t)done * C_pad, P, K, C_pad, cnt, st);
         done += cnt;
         cnt = 0;
     }
 };
 
 struct Synthesiser {
-    const double2* A = nullptr;
     size_t P = 0;
-    int K = 0, T = 0, n_total = 0;
+    int K = 0, C_pad = 0, T = 0, J_pad = 0, n_total = 0;
     DBuf<float> slab;
-    DBuf<double2> coef;
+    DBuf<float> a32;  // [C_pad][P]: (Re, Im) rows of the source spectrum, FP32
+    DBuf<float> et;   // per slab [C_pad][J_pad]: (Re, -Im) of the phase factors, FP32
     hipStream_t st = nullptr;
-    void init(const double2* A_, size_t P_, int K_, const std::vector<double2>& table, int n_samples, hipStream_t s)
+    void init(const double2* A, size_t P_, int K_, const std::vector<double2>& table, int n_samples, hipStream_t s)
     {
-        A = A_;
         P = P_;
         K = K_;
         st = s;
         n_total = n_samples;
-        T = std::min(kSlab, std::max(n_samples, 1));
+        C_pad = spec_cols(K);
+        T = slab_len(P, (size_t)std::max(n_samples, 1));
+        J_pad = (T + 255) / 256 * 256;
         slab.alloc((size_t)T * P);
-        upload(coef, table, st);
+        a32.alloc((size_t)C_pad * P);
+        launch_spectrum_f32(a32.p, A, P, K, C_pad, st);
+        const int n_slabs = (std::max(n_samples, 1) + T - 1) / T;
+        std::vector<float> e((size_t)n_slabs * C_pad * J_pad, 0.f);
+        for (int idx = 0; idx < n_samples; ++idx) {
+            const size_t base = (size_t)(idx / T) * C_pad * J_pad + (size_t)(idx % T);
+            for (int k = 0; k < K; ++k) {
+                e[base + (size_t)(2 * k) * J_pad] = (float)table[(size_t)idx * K + k].x;
+                e[base + (size_t)(2 * k + 1) * J_pad] = -(float)table[(size_t)idx * K + k].y;
+            }
+        }
+        upload(et, e, st);
     }
     // Source slice of the idx-th sampled step (0-based); synthesises a new block when needed.
     const float* slice(int idx)
     {
         const int slot = idx % T;
-        if (slot == 0) launch_synth(slab.p, A, coef.p + (size_t)idx * K, P, K, std::min(T, n_total - idx), st);
+        if (slot == 0)
+            launch_synth_mfma(slab.p, a32.p, et.p + (size_t)(idx / T) * C_pad * J_pad, P, C_pad, J_pad,
+                              std::min(T, n_total - idx), st);
         return slab.p + (size_t)slot * P;
     }
 };

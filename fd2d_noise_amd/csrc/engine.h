@@ -106,12 +106,16 @@ void launch_material(const double* mu, const double* rho, float* bv, float* dtmu
                      int pitch, double dt, hipStream_t s);
 void launch_scale_to_float(const double* in, float* out, size_t n, double scale, hipStream_t s);
 
-// acc[k][x] += sum_j samp[j][x] * coef[j][k]   (FP64 complex accumulators, FP32 samples)
-void launch_dft_flush(double2* acc, const float* samp, const double2* coef, size_t P, int K, int cnt,
-                      hipStream_t s);
-// out[j][x] = sum_k Re(A[k][x] * coef[j][k])      (FP64, rounded to FP32)
-void launch_synth(float* out, const double2* A, const double2* coef, size_t P, int K, int cnt,
-                  hipStream_t s);
+// acc[k][x] += sum_j samp[j][x] * e[j][k] over cnt samples: FP32 matrix-core products, FP64
+// accumulators; etab[j][2k], etab[j][2k+1] = Re, Im e[j][k] with C_pad (a multiple of 256) columns.
+void launch_dft_mfma(double2* acc, const float* samp, const float* etab, size_t P, int K, int C_pad, int cnt,
+                     hipStream_t s);
+// out[j][x] = sum_c et[c][j] * a[c][x] for j < cnt (FP32 matrix cores): with a = (Re A, Im A) rows
+// and et = (Re, -Im) of the phase factors this is Re(A[k][x] * coef[j][k]) summed over k.
+void launch_synth_mfma(float* out, const float* a, const float* et, size_t P, int C_pad, int J_pad, int cnt,
+                       hipStream_t s);
+// a[2k][x], a[2k+1][x] = Re, Im A[k][x] (FP32), zero rows 2K..C_pad-1
+void launch_spectrum_f32(float* a, const double2* A, size_t P, int K, int C_pad, hipStream_t s);
 // A[k][x] = spec_sum[k] * dist(x,k) * (conj_in ? conj(Z) : Z)[k][x], dist(x,k) = sum_b I[k,b] sd[b][x]
 void launch_source_spectrum(double2* A, const double2* Z, int conj_in, const double* sd, int n_sd,
                             const double* I, const double* spec_sum, size_t P, int K, hipStream_t s);
@@ -125,6 +129,6 @@ void launch_source_kernel(double* Ks, const double2* G, const double2* D, const 
 // point-source amplitudes: amp[n][q] = (float)Re(stf(q, n)) from complex nr x N interleaved
 void launch_stf_real(float* amp, const double2* stf, int64_t d0, int nq, int nsteps, hipStream_t s);
 
-constexpr int kSlab = 128;  // time samples per DFT flush / synthesis block
+constexpr int kSpecSlab = 512;  // time samples per DFT flush / synthesis block (at most)
 
 }  // namespace fd2d

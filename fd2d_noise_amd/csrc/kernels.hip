@@ -1,14 +1,16 @@
 // kernels.hip -- gfx950 kernels of the 2-D SH staggered-grid engine.
 //
-// One time step is two launches over the nx x nz grid, 64 x 4 nodes per workgroup (one wavefront
-// per z row, so every load and store of a row is 256 contiguous bytes):
-//   velocity phase: DS = div(sigma) (zero frames of code/propagation/div_s.m), point sources,
-//                   synthesised distributed source, v = (v + dt/rho*DS) * taper, u += v*dt,
-//                   snapshot of the pre-update field, DFT sample store, receiver gather;
-//   stress phase:   sigma += dt*mu * D(v) (zero frames of dx_v.m / dz_v.m), Born terms, and in
-//                   the adjoint run the structure-kernel product of D(u) with D(u_fwd).
-// The frequency-domain work is blocked over time: DFT samples and synthesised source slices are
-// staged as FP32 slabs and contracted with FP64 phase tables in separate launches.
+// One time step is one launch of k_tile<R,MODE> over the nx x nz grid (64 x 32 nodes per
+// workgroup, one wavefront per row slice, so every load and store of a row is 256 contiguous
+// bytes). Each workgroup recomputes the velocity on a halo of its tile, so the step needs no second
+// launch:
+//   velocity: DS = div(sigma) (zero frames of code/propagation/div_s.m), point sources, synthesised
+//             distributed source, v = (v + dt/rho*DS) * taper, u += v*dt, snapshot of the
+//             pre-update field, DFT sample store, receiver gather;
+//   stress:   sigma += dt*mu * D(v) (zero frames of dx_v.m / dz_v.m), Born terms.
+// The structure-kernel product K_mu -= fw_nth * (D u . D u_fwd) runs as k_kmu after the steps that
+// take it. The frequency-domain work is blocked over time: DFT samples and synthesised source slices
+// are staged as FP32 slabs and contracted with FP64 phase tables in separate launches.
 #include "engine.h"
 
 namespace fd2d {
@@ -445,139 +447,8 @@ void launch_stf_real(float* amp, const double2* stf, int64_t d0, int nq, int nst
 // ---------------------------------------------------------------------------------------------
 // Frequency domain
 // ---------------------------------------------------------------------------------------------
-// Both contractions are tall-skinny FP64 products over the P grid nodes:
-//   flush:  acc[k][x] += sum_j samp[j][x] * coef[j][k]       (P x cnt) * (cnt x K)
-//   synth:  out[j][x]  = sum_k Re(A[k][x] * coef[j][k])      (P x K) * (K x cnt)
-// One thread owns one node and a chunk of 16 (flush: frequencies; synth: time slices) FP64
-// accumulators; the phase-table chunk is staged in LDS and read as wave-uniform broadcasts. The
-// workgroups that share a 256-node block (one per chunk) are placed on the same XCD one after the
-// other, so the block's samples (flush) or spectrum (synth) come from that XCD's L2 after the first.
-constexpr int kChunk = 16;
-
-// (node block, chunk) of workgroup g when n_blocks is padded to a multiple of kXCD.
-__device__ __forceinline__ void xcd_block(int g, int n_chunks, int& block, int& chunk)
-{
-    const int xcd = g % kXCD, q = g / kXCD;
-    chunk = q % n_chunks;
-    block = (q / n_chunks) * kXCD + xcd;
-}
-
-__global__ __launch_bounds__(256) void k_dft_flush(double2* __restrict__ acc, const float* __restrict__ samp,
-                                                   const double2* __restrict__ coef, size_t P, int K, int cnt,
-                                                   int n_blocks, int n_chunks)
-{
-    __shared__ double2 tab[kSlab * kChunk];
-    int block, chunk;
-    xcd_block(blockIdx.x, n_chunks, block, chunk);
-    if (block >= n_blocks) return;
-    const int k0 = chunk * kChunk;
-    const int nk = K - k0 < kChunk ? K - k0 : kChunk;
-    for (int idx = threadIdx.x; idx < cnt * kChunk; idx += 256) {
-        const int j = idx / kChunk, kk = idx - j * kChunk;
-        tab[idx] = kk < nk ? coef[(size_t)j * K + k0 + kk] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    const size_t x = (size_t)block * 256 + threadIdx.x;
-    if (x >= P) return;
-    double re[kChunk], im[kChunk];
-#pragma unroll
-    for (int kk = 0; kk < kChunk; ++kk) re[kk] = im[kk] = 0.0;
-    const float* sp = samp + x;
-    int j = 0;
-    for (; j + 4 <= cnt; j += 4) {
-        const double b0 = sp[(size_t)j * P], b1 = sp[(size_t)(j + 1) * P];
-        const double b2 = sp[(size_t)(j + 2) * P], b3 = sp[(size_t)(j + 3) * P];
-#pragma unroll
-        for (int kk = 0; kk < kChunk; ++kk) {
-            const double2 c0 = tab[j * kChunk + kk], c1 = tab[(j + 1) * kChunk + kk];
-            const double2 c2 = tab[(j + 2) * kChunk + kk], c3 = tab[(j + 3) * kChunk + kk];
-            re[kk] = fma(b0, c0.x, fma(b1, c1.x, fma(b2, c2.x, fma(b3, c3.x, re[kk]))));
-            im[kk] = fma(b0, c0.y, fma(b1, c1.y, fma(b2, c2.y, fma(b3, c3.y, im[kk]))));
-        }
-    }
-    for (; j < cnt; ++j) {
-        const double b = sp[(size_t)j * P];
-#pragma unroll
-        for (int kk = 0; kk < kChunk; ++kk) {
-            const double2 c = tab[j * kChunk + kk];
-            re[kk] = fma(b, c.x, re[kk]);
-            im[kk] = fma(b, c.y, im[kk]);
-        }
-    }
-#pragma unroll
-    for (int kk = 0; kk < kChunk; ++kk) {
-        if (kk < nk) {
-            double2* d = acc + (size_t)(k0 + kk) * P + x;
-            double2 v = *d;
-            v.x += re[kk];
-            v.y += im[kk];
-            *d = v;
-        }
-    }
-}
-
-void launch_dft_flush(double2* acc, const float* samp, const double2* coef, size_t P, int K, int cnt, hipStream_t s)
-{
-    if (cnt <= 0 || K <= 0) return;
-    const int n_blocks = (int)((P + 255) / 256);
-    const int n_chunks = (K + kChunk - 1) / kChunk;
-    const int padded = (n_blocks + kXCD - 1) / kXCD * kXCD;
-    hipLaunchKernelGGL(k_dft_flush, dim3((unsigned)(padded * n_chunks)), dim3(256), 0, s, acc, samp, coef, P, K, cnt,
-                       n_blocks, n_chunks);
-}
-
-constexpr int kSynK = 32;  // frequencies per LDS stage of the synthesis
-
-__global__ __launch_bounds__(256) void k_synth(float* __restrict__ out, const double2* __restrict__ A,
-                                               const double2* __restrict__ coef, size_t P, int K, int cnt,
-                                               int n_blocks, int n_chunks)
-{
-    __shared__ double2 tab[kSynK * kChunk];  // [k][j] for this chunk of time slices
-    int block, chunk;
-    xcd_block(blockIdx.x, n_chunks, block, chunk);
-    if (block >= n_blocks) return;
-    const int j0 = chunk * kChunk;
-    const int nj = cnt - j0 < kChunk ? cnt - j0 : kChunk;
-    const size_t x = (size_t)block * 256 + threadIdx.x;
-    const bool live = x < P;
-    double s[kChunk];
-#pragma unroll
-    for (int jj = 0; jj < kChunk; ++jj) s[jj] = 0.0;
-    for (int kb = 0; kb < K; kb += kSynK) {
-        const int nkb = K - kb < kSynK ? K - kb : kSynK;
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < kSynK * kChunk; idx += 256) {
-            const int kk = idx / kChunk, jj = idx - kk * kChunk;
-            tab[idx] = (kk < nkb && jj < nj) ? coef[(size_t)(j0 + jj) * K + kb + kk] : make_double2(0.0, 0.0);
-        }
-        __syncthreads();
-        if (live) {
-            for (int kk = 0; kk < nkb; ++kk) {
-                const double2 av = A[(size_t)(kb + kk) * P + x];
-#pragma unroll
-                for (int jj = 0; jj < kChunk; ++jj) {
-                    const double2 cc = tab[kk * kChunk + jj];
-                    s[jj] = fma(av.x, cc.x, fma(-av.y, cc.y, s[jj]));
-                }
-            }
-        }
-    }
-    if (!live) return;
-#pragma unroll
-    for (int jj = 0; jj < kChunk; ++jj)
-        if (jj < nj) out[(size_t)(j0 + jj) * P + x] = (float)s[jj];
-}
-
-void launch_synth(float* out, const double2* A, const double2* coef, size_t P, int K, int cnt, hipStream_t s)
-{
-    if (cnt <= 0 || K <= 0) return;
-    const int n_blocks = (int)((P + 255) / 256);
-    const int n_chunks = (cnt + kChunk - 1) / kChunk;
-    const int padded = (n_blocks + kXCD - 1) / kXCD * kXCD;
-    hipLaunchKernelGGL(k_synth, dim3((unsigned)(padded * n_chunks)), dim3(256), 0, s, out, A, coef, P, K, cnt,
-                       n_blocks, n_chunks);
-}
-
+// The DFT flush and the synthesis of distributed sources run on the matrix cores (spectral.hip);
+// these kernels prepare their spectra and finish their results.
 __global__ void k_source_spectrum(double2* A, const double2* Z, int conj_in, const double* sd, int n_sd,
                                   const double* I, const double* spec_sum, size_t P, int K)
 {
