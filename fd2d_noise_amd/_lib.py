@@ -51,7 +51,7 @@ EXPORTS = (
     "fd2d_run_forward1_green", "fd2d_run_forward2_correlation", "fd2d_run_noise_adjoint",
     "fd2d_run_forward1_green_device", "fd2d_run_forward2_correlation_device",
     "fd2d_run_noise_adjoint_device", "fd2d_query_output_dims", "fd2d_last_error", "fd2d_version",
-    "fd2d_timing_enable", "fd2d_timing_read",
+    "fd2d_timing_enable", "fd2d_timing_read", "fd2d_set_steps_per_launch",
     "fd2d_make_adjoint_sources", "fd2d_make_adjoint_sources_device", "fd2d_dot_device", "fd2d_axpby_device",
     "fd2d_gaussblur2d_device", "fd2d_map_m_to_parameters_device", "fd2d_map_gradparameters_to_gradm_device",
     "fd2d_regularization_device", "fd2d_general_source_device",
@@ -68,11 +68,17 @@ def timing_enable(every: int) -> None:
     load().fd2d_timing_enable(int(every))
 
 
+def set_steps_per_launch(steps: int) -> None:
+    """1 (default): one time step per launch; 2: two consecutive steps per launch where possible."""
+    if load().fd2d_set_steps_per_launch(int(steps)) != abi.FD2D_OK:
+        raise ValueError(last_error())
+
+
 def timing_read() -> dict:
     t = Timing()
     load().fd2d_timing_read(ctypes.byref(t))
     out = {}
     for q, name in enumerate(("green", "correlation", "adjoint")):
-        out[name] = {"launches": int(t.launches[q][0]), "velocity_ms": float(t.ms[q][0]),
-                     "stress_ms": float(t.ms[q][1]), "points": int(t.points[q])}
+        out[name] = {"launches": int(t.launches[q][0]), "steps": int(t.launches[q][1]),
+                     "ms": float(t.ms[q][0]), "points": int(t.points[q])}
     return out

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -281,11 +282,15 @@ struct Solver {
     DBuf<int> ps_off_d, rec_off_d;
     DBuf<int2> ps_ent_d, rec_ent_d;
     StepArgs a{};
+    PairArgs pr{};  // the second step of a two-step launch
 
     Solver(const fd2d_config* c, const Geometry& geo, hipStream_t s, bool with_u_) : cfg(c), g(geo), st(s), with_u(with_u_)
     {
         lay = PaddedLayout::make(g.nx, g.nz);
         const size_t E = lay.elems(), o = lay.origin();
+        // buffer-load offsets are 32-bit signed byte offsets into one padded array
+        if (E * sizeof(float) >= ((size_t)1 << 31))
+            raise(FD2D_EUNSUPPORTED, "nx x nz = %d x %d exceeds 2 GiB per padded field", g.nx, g.nz);
         const int nf = with_u ? 4 : 3;
         fields.alloc(E * nf * 2);
         HIPCHK(hipMemsetAsync(fields.p, 0, E * nf * 2 * sizeof(float), st));
@@ -328,19 +333,20 @@ struct Solver {
         launch_material(mu, rho, coefs.p + lay.origin(), coefs.p + lay.elems() + lay.origin(), g.nx, g.nz,
                         lay.pitch, g.dt, st);
     }
-    // Point sources as per-tile lists over every node whose velocity the tile computes (the tile
-    // plus its (R-1, R) halo), in ascending source order so every copy of a halo node sums
-    // identically.
+    // Point sources as per-tile lists over every node whose velocity the tile computes in a one- or
+    // two-step launch (the tile plus its (3R-2, 3R-1) halo), in ascending source order so every
+    // copy of a halo node sums identically.
     void set_sources(const std::vector<int2>& nodes)
     {
         if (nodes.empty()) return;
+        const int lo = pair_halo_lo(g.R), hi = pair_halo_hi(g.R);
         std::vector<std::vector<int2>> per(n_tiles);
         for (size_t q = 0; q < nodes.size(); ++q) {
             const int x = nodes[q].x, z = nodes[q].y;
             for (int t = 0; t < n_tiles; ++t) {
                 const int i0 = (t % tiles_x) * kTX, j0 = (t / tiles_x) * kTileZ;
-                if (x < i0 - (g.R - 1) || x > i0 + kTX - 1 + g.R) continue;
-                if (z < j0 - (g.R - 1) || z > j0 + kTileZ - 1 + g.R) continue;
+                if (x < i0 - lo || x > i0 + kTX - 1 + hi) continue;
+                if (z < j0 - lo || z > j0 + kTileZ - 1 + hi) continue;
                 per[t].push_back(make_int2(node_key_host(x, z), (int)q));
             }
         }
@@ -389,19 +395,40 @@ struct Solver {
         a.rec_ent = nullptr;
         a.seis_col = nullptr;
         a.born_dmu = a.born_in0 = a.born_in1 = nullptr;
+        pr.ps_amp2 = nullptr;
+        pr.ps_const2 = 0.f;
+        pr.syn2 = nullptr;
+        pr.snap2 = nullptr;
+        pr.dsamp2 = nullptr;
+        pr.seis_col2 = nullptr;
     }
-    void point_sources(const float* amp, float cst)
+    // The per-step fields of step `sub` of a launch (0: a single step or the first of a pair).
+    struct Slots {
+        const float** ps_amp;
+        float* ps_const;
+        const float** syn;
+        float** snap;
+        float** dsamp;
+        double** seis;
+    };
+    Slots slots(int sub)
+    {
+        if (sub == 0) return {&a.ps_amp, &a.ps_const, &a.syn, &a.snap, &a.dsamp, &a.seis_col};
+        return {&pr.ps_amp2, &pr.ps_const2, &pr.syn2, &pr.snap2, &pr.dsamp2, &pr.seis_col2};
+    }
+    void point_sources(int sub, const float* amp, float cst)
     {
         a.ps_off = ps_off_d.p;
         a.ps_ent = ps_ent_d.p;
-        a.ps_amp = amp;
-        a.ps_const = cst;
+        const Slots sl = slots(sub);
+        *sl.ps_amp = amp;
+        *sl.ps_const = cst;
     }
-    void receivers(double* col)
+    void receivers(int sub, double* col)
     {
         a.rec_off = rec_off_d.p;
         a.rec_ent = rec_ent_d.p;
-        a.seis_col = col;
+        *slots(sub).seis = col;
     }
     void born_green(const float* dtdmu, const float* gin)
     {
@@ -410,6 +437,13 @@ struct Solver {
     void step()
     {
         launch_step(g.R, with_u, a, st);
+        cur ^= 1;
+    }
+    // steps n and n+1 in one launch (the state buffers swap once, as after one step)
+    void step_pair()
+    {
+        pr.a = a;
+        launch_pair(g.R, with_u, pr, st);
         cur ^= 1;
     }
     // structure-kernel product with the displacement the last step produced
@@ -517,46 +551,59 @@ hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 // ---------------------------------------------------------------------------------------------
 std::mutex g_timing_mu;
 int g_timing_every = 0;
+std::atomic<int> g_steps_per_launch{1};  // fd2d_set_steps_per_launch
 fd2d_timing g_timing{};
 
 struct StepTimer {
     int solver, every;
     size_t P;
     hipStream_t st;
-    std::vector<hipEvent_t> ev;  // pairs: before and after the step launch
+    std::vector<hipEvent_t> ev;  // pairs: before and after a step launch
+    std::vector<int> steps;      // time steps each timed launch advanced
     StepTimer(int solver_, size_t P_, hipStream_t s) : solver(solver_), P(P_), st(s)
     {
         std::lock_guard<std::mutex> lk(g_timing_mu);
         every = g_timing_every;
     }
-    bool on(int n) const { return every > 0 && n % every == 0; }
-    void mark(int n)
+    // a launch of `k` steps from step n is timed when one of its steps is a multiple of `every`
+    bool on(int n, int k) const { return every > 0 && (n + k - 1) / every > (n - 1) / every; }
+    void record()
     {
-        if (!on(n)) return;
         hipEvent_t e;
         HIPCHK(hipEventCreate(&e));
         HIPCHK(hipEventRecord(e, st));
         ev.push_back(e);
     }
+    void begin(int n, int k)
+    {
+        if (!on(n, k)) return;
+        record();
+        steps.push_back(k);
+    }
+    void end(int n, int k)
+    {
+        if (on(n, k)) record();
+    }
     // Call after the stream has been synchronised.
     void collect()
     {
-        double ms_v = 0, ms_s = 0;
-        int64_t cnt = 0;
+        double ms = 0;
+        int64_t cnt = 0, nsteps = 0;
         for (size_t q = 0; q + 1 < ev.size(); q += 2) {
-            float a = 0;
-            HIPCHK(hipEventElapsedTime(&a, ev[q], ev[q + 1]));
-            ms_v += a;
+            float e = 0;
+            HIPCHK(hipEventElapsedTime(&e, ev[q], ev[q + 1]));
+            ms += e;
             ++cnt;
+            nsteps += steps[q / 2];
         }
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         ev.clear();
+        steps.clear();
         if (cnt) {
             std::lock_guard<std::mutex> lk(g_timing_mu);
             g_timing.launches[solver - 1][0] += cnt;
-            g_timing.launches[solver - 1][1] += 0;
-            g_timing.ms[solver - 1][0] += ms_v;
-            g_timing.ms[solver - 1][1] += ms_s;
+            g_timing.launches[solver - 1][1] += nsteps;
+            g_timing.ms[solver - 1][0] += ms;
             g_timing.points[solver - 1] = (int64_t)P;
         }
     }
@@ -627,21 +674,31 @@ void green_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carr
 
     StepTimer timer(FD2D_GREEN, g.P, st);
     int64_t i_fw_out = 0;
-    for (int n = 1; n <= g.nt; ++n) {
-        S.clear_step();
-        const bool snap_step = mode != 0 && (n + g.nt - 1) % g.f == 0;
-        if (snap_step) {
-            if (Gout) S.a.snap = Gout + (size_t)i_fw_out * g.P;
+    auto sampled = [&](int n) { return Gf != nullptr && (n + g.nt - 1) % g.s == 0; };
+    // per-step fields of step n as step `sub` of the launch
+    auto plan = [&](int n, int sub) {
+        const Solver::Slots sl = S.slots(sub);
+        if (mode != 0 && (n + g.nt - 1) % g.f == 0) {
+            if (Gout) *sl.snap = Gout + (size_t)i_fw_out * g.P;
             ++i_fw_out;
         }
         if (born) S.born_green(dtdmu.p, (const float*)G_in->data + (size_t)(n - 1) * g.P);
-        else if (!sid.empty()) S.point_sources(nullptr, 1.0e9f);  // stf = 1e9 every step (:64)
-        if (Gf && (n + g.nt - 1) % g.s == 0) S.a.dsamp = dft.next_slot();
-        if (sg && nr) S.receivers(sg + (size_t)nr * (n - 1));
-        timer.mark(n);
-        S.step();
-        timer.mark(n);
-        if (S.a.dsamp) dft.commit();
+        else if (!sid.empty()) S.point_sources(sub, nullptr, 1.0e9f);  // stf = 1e9 every step (:64)
+        if (sampled(n)) *sl.dsamp = dft.next_slot();
+        if (sg && nr) S.receivers(sub, sg + (size_t)nr * (n - 1));
+    };
+    const bool pairs = g_steps_per_launch.load() == 2 && !born;
+    for (int n = 1; n <= g.nt;) {
+        const int k = (pairs && n + 1 <= g.nt && !(sampled(n) && sampled(n + 1))) ? 2 : 1;
+        S.clear_step();
+        for (int q = 0; q < k; ++q) plan(n + q, q);
+        timer.begin(n, k);
+        if (k == 2) S.step_pair();
+        else S.step();
+        timer.end(n, k);
+        for (int q = 0; q < k; ++q)
+            if (sampled(n + q)) dft.commit();
+        n += k;
     }
     if (Gf) {
         dft.flush();
@@ -714,22 +771,30 @@ void correlation_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2
     }
     StepTimer timer(FD2D_CORRELATION, g.P, st);
     int64_t i_fw_out = 0;
-    for (int n = 1; n <= N; ++n) {
-        S.clear_step();
+    auto plan = [&](int n, int sub) {
+        const Solver::Slots sl = S.slots(sub);
         if (mode != 0 && n % g.f == 0) {  // :182-186
-            if (Cout) S.a.snap = Cout + (size_t)i_fw_out * g.P;
+            if (Cout) *sl.snap = Cout + (size_t)i_fw_out * g.P;
             ++i_fw_out;
         }
-        if (n % g.s == 0) S.a.syn = syn.slice(n / g.s - 1);  // :195-216
+        if (n % g.s == 0) *sl.syn = syn.slice(n / g.s - 1);  // :195-216
         if (born && n != N) {                                 // :232-238
             S.a.born_dmu = dmu_f.p;
             S.a.born_in0 = (const float*)C_in->data + (size_t)(n - 1) * g.P;
             S.a.born_in1 = (const float*)C_in->data + (size_t)n * g.P;
         }
-        if (sg && nr) S.receivers(sg + (size_t)nr * (n - 1));
-        timer.mark(n);
-        S.step();
-        timer.mark(n);
+        if (sg && nr) S.receivers(sub, sg + (size_t)nr * (n - 1));
+    };
+    const bool pairs = g_steps_per_launch.load() == 2 && !born;
+    for (int n = 1; n <= N;) {
+        const int k = (pairs && n + 1 <= N && !(n % g.s == 0 && (n + 1) % g.s == 0)) ? 2 : 1;
+        S.clear_step();
+        for (int q = 0; q < k; ++q) plan(n + q, q);
+        timer.begin(n, k);
+        if (k == 2) S.step_pair();
+        else S.step();
+        timer.end(n, k);
+        n += k;
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
@@ -857,34 +922,52 @@ void adjoint_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_ca
 
     StepTimer timer(FD2D_ADJOINT, g.P, st);
     int64_t i_fw_in = 0, i_fw_out = 0;
-    for (int n = 1; n <= nsteps; ++n) {
-        S.clear_step();
+    auto kernel_step = [&](int n) { return Kd && n_ufwd && n_ufwd >= i_fw_in + 1 && n % g.f == 0; };  // :324
+    auto sampled = [&](int n) { return need_dft && n % g.s == 0; };
+    auto synth = [&](int n) { return !(stf_point && !sid.empty()) && dist_src && n % g.s == 0; };
+    const float* kfwd[2] = {nullptr, nullptr};
+    auto plan = [&](int n, int sub) {
+        const Solver::Slots sl = S.slots(sub);
         if (save_mode && n % g.f == 0) {  // :237-241
-            if (uo) S.a.snap = uo + (size_t)i_fw_out * g.P;
+            if (uo) *sl.snap = uo + (size_t)i_fw_out * g.P;
             ++i_fw_out;
         }
         if (stf_point && !sid.empty()) {
-            S.point_sources(amp.p + (size_t)(n - 1) * sid.size(), 0.f);
-        } else if (dist_src && n % g.s == 0) {
-            S.a.syn = syn.slice(n / g.s - 1);
+            S.point_sources(sub, amp.p + (size_t)(n - 1) * sid.size(), 0.f);
+        } else if (synth(n)) {
+            *sl.syn = syn.slice(n / g.s - 1);
         }
         if (born && n != N) {  // :285-291
             S.a.born_dmu = dmu_f.p;
             S.a.born_in0 = (const float*)u_in->data + (size_t)(n - 1) * g.P;
             S.a.born_in1 = (const float*)u_in->data + (size_t)n * g.P;
         }
-        const float* kfwd = nullptr;
-        if (Kd && n_ufwd && n_ufwd >= i_fw_in + 1 && n % g.f == 0) {  // :324-331
-            kfwd = (const float*)u_fwd->data + (size_t)(n_ufwd - 1 - i_fw_in) * g.P;
+        kfwd[sub] = nullptr;
+        if (kernel_step(n)) {  // :324-331
+            kfwd[sub] = (const float*)u_fwd->data + (size_t)(n_ufwd - 1 - i_fw_in) * g.P;
             ++i_fw_in;
         }
-        if (need_dft && n % g.s == 0) S.a.dsamp = dft.next_slot();
-        if (sg && nr) S.receivers(sg + (size_t)nr * (n - 1));
-        timer.mark(n);
-        S.step();
-        timer.mark(n);
-        if (kfwd) S.kmu(kfwd, Kd + (size_t)nb_cols * g.P, (double)g.f);
-        if (S.a.dsamp) dft.commit();
+        if (sampled(n)) *sl.dsamp = dft.next_slot();
+        if (sg && nr) S.receivers(sub, sg + (size_t)nr * (n - 1));
+    };
+    // A pair needs the structure-kernel product, if any, after its second step (the displacement
+    // of its first step is not kept), at most one DFT sample and at most one synthesised source.
+    const bool pairs = g_steps_per_launch.load() == 2 && !born;
+    for (int n = 1; n <= nsteps;) {
+        const int k = (pairs && n + 1 <= nsteps && !kernel_step(n) && !(sampled(n) && sampled(n + 1)) &&
+                       !(synth(n) && synth(n + 1)))
+                          ? 2
+                          : 1;
+        S.clear_step();
+        for (int q = 0; q < k; ++q) plan(n + q, q);
+        timer.begin(n, k);
+        if (k == 2) S.step_pair();
+        else S.step();
+        timer.end(n, k);
+        if (kfwd[k - 1]) S.kmu(kfwd[k - 1], Kd + (size_t)nb_cols * g.P, (double)g.f);
+        for (int q = 0; q < k; ++q)
+            if (sampled(n + q)) dft.commit();
+        n += k;
     }
     if (need_dft) {
         dft.flush();
@@ -1027,6 +1110,16 @@ void blur_maps(const Geometry& g, const MapFrame& f, const double* src, double* 
 extern "C" {
 
 const char* fd2d_last_error(void) { return g_err.c_str(); }
+
+int fd2d_set_steps_per_launch(int32_t steps)
+{
+    if (steps != 1 && steps != 2) {
+        g_err = "steps per launch must be 1 or 2";
+        return FD2D_EINVAL;
+    }
+    g_steps_per_launch.store(steps);
+    return FD2D_OK;
+}
 
 int fd2d_timing_enable(int32_t every)
 {

@@ -3,8 +3,9 @@
 // Wavefield layout in HBM: every time-stepped field (v, sigma_xy, sigma_zy, u) and every per-node
 // coefficient (dt/rho, dt*mu) is an FP32 nx x nz array, x fastest (the MATLAB column-major order,
 // so a wavefront reads 64 consecutive x nodes of one z row: 256 contiguous bytes). Each array is
-// padded with kGhost zero nodes on every side and a row pitch rounded up to 64 elements, so every
-// load of a tile's halo that lies inside the padding is in bounds. sigma_xy and sigma_zy share the
+// padded with kGhost zero rows above and below, kGhostX zero columns on the left and a row pitch
+// rounded up to 64 elements plus 64, so every load of a tile's halo has a non-negative byte offset
+// and reads zeros outside the grid (loads past the end are range-checked buffer loads). sigma_xy and sigma_zy share the
 // node (i,j) of the MATLAB arrays sxy(i,j) ((nx-1) x nz) and szy(i,j) (nx x (nz-1)).
 //
 // The time-stepped fields are double-buffered: step n reads the "in" set and writes the "out" set,
@@ -21,7 +22,7 @@
 
 namespace fd2d {
 
-constexpr int kGhost = 8;   // ghost rows (>= 2R-1 for R <= 4: the sigma halo of a fused step)
+constexpr int kGhost = 16;  // ghost rows (>= 4R-2 for R <= 4: the sigma halo of a two-step launch)
 constexpr int kGhostX = 32; // ghost columns: node x = 0 starts a 128-byte line
 constexpr int kTX = 64;     // tile width: one wavefront per row
 constexpr int kXCD = 8;     // MI355X accelerator complex dies (workgroups are dealt round-robin)
@@ -92,7 +93,26 @@ struct StepArgs {
 
 inline int node_key_host(int x, int z) { return ((z + 8) << 16) | (x + 8); }
 
+// Two consecutive time steps n, n+1 in one launch. `a` describes the state before step n, the
+// outputs after step n+1 and the per-step inputs of step n (ps_amp / ps_const, syn, snap, dsamp,
+// seis_col); the fields below are those of step n+1. At most one of the two steps may carry a
+// synthesised source (a.syn or syn2); Born terms are not supported.
+struct PairArgs {
+    StepArgs a;
+    const float* ps_amp2;
+    float ps_const2;
+    const float* syn2;
+    float* snap2;
+    float* dsamp2;
+    double* seis_col2;
+};
+
+// halo of the first step's velocity region of a pair launch (the point-source lists cover it)
+inline int pair_halo_lo(int R) { return 3 * R - 2; }
+inline int pair_halo_hi(int R) { return 3 * R - 1; }
+
 void launch_step(int R, bool with_u, const StepArgs& a, hipStream_t s);
+void launch_pair(int R, bool with_u, const PairArgs& p, hipStream_t s);
 // K_mu -= scale * (D u . D fwd) over the grid (u padded, fwd compact), after a step.
 void launch_kmu(int R, const StepArgs& a, const float* u, const float* fwd, double* kmu, double scale,
                 hipStream_t s);

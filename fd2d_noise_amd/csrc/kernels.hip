@@ -297,6 +297,362 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Two time steps per launch (temporal blocking)
+// ---------------------------------------------------------------------------------------------
+// A workgroup advances its kTX x kTileZ tile by steps n and n+1. It stages sigma of the state before
+// step n on the tile plus a (4R-2)-node halo, computes the step-n velocity on the tile plus a
+// (3R-2, 3R-1) halo (region V1), updates sigma in place where step n+1 needs it, computes the
+// step-(n+1) velocity on the tile plus the (R-1, R) halo (V2) and finally sigma on the tile. The
+// halo values are recomputed by the neighbouring tiles with the same operations, so a pair equals
+// two single steps bit for bit; the compulsory HBM traffic per node is that of one step.
+// Region coordinates relative to the tile origin (i0, j0), L1 = 3R-2, R1 = 3R-1, LS = 4R-2:
+//   VV (V1): columns -L1 .. 63+R1, rows -L1 .. TZ-1+R1
+//   SX:      columns -LS .. 63+LS, rows as VV
+//   SZ:      columns as VV,        rows -LS .. TZ-1+LS
+// Per-thread layout as in k_tile: lane l owns column i0 + l of every region's main block, wave w
+// the rows w, w+4, ...; halo columns are packed lists over the 256 threads.
+template <int R, int MODE>
+__global__ __launch_bounds__(256) void k_pair(PairArgs pa)
+{
+    constexpr bool HAS_U = MODE >= 1;
+    constexpr int TZ = kTileZ;
+    constexpr int L1 = 3 * R - 2, R1 = 3 * R - 1, LS = 4 * R - 2;
+    constexpr int W1 = kTX + L1 + R1, H1 = TZ + L1 + R1;
+    constexpr int WSX = kTX + 2 * LS, HSZ = TZ + 2 * LS;
+    constexpr int HC1 = L1 + R1;      // halo columns of VV and SZ
+    constexpr int HCS = 2 * LS;       // halo columns of SX
+    constexpr int V2R0 = 2 * R - 1, V2R1 = TZ + 4 * R - 3;    // VV rows of V2 (inclusive)
+    constexpr int V2C0 = 2 * R - 1, V2C1 = kTX + 4 * R - 3;   // VV columns of V2 (inclusive)
+    constexpr int HS1X = TZ + 2 * R - 1;     // rows of the step-n sigma_xy update (the V2 rows)
+    constexpr int HS1Z = TZ + 4 * R - 2;     // rows of the step-n sigma_zy update: j0-(2R-1) ..
+    constexpr int HCX1 = 2 * (2 * R - 1);    // halo columns of the sigma_xy update
+    constexpr int HCZ1 = 2 * R - 1;          // halo columns of the sigma_zy update (V2 halo)
+    __shared__ float SX[H1 * WSX];
+    __shared__ float SZ[HSZ * W1];
+    __shared__ float VV[H1 * W1];
+
+    const StepArgs& a = pa.a;
+    const int g = blockIdx.x;
+    const int t = (g % kXCD) * a.tiles_per_xcd + g / kXCD;
+    if (t >= a.n_tiles) return;
+    const int i0 = (t % a.tiles_x) * kTX, j0 = (t / a.tiles_x) * TZ;
+    const int nx = a.nx, nz = a.nz, pitch = a.pitch;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int x = i0 + lane;
+    const bool xg = x < nx;
+
+    const unsigned pb = a.padded_bytes;
+    const auto r_sxy = rsrc(a.sxy_in - a.origin, pb), r_szy = rsrc(a.szy_in - a.origin, pb);
+    const auto r_v = rsrc(a.v_in - a.origin, pb), r_bv = rsrc(a.bv - a.origin, pb);
+    const auto r_u = rsrc(HAS_U ? a.u_in - a.origin : a.v_in - a.origin, pb);
+    const auto r_dm = rsrc(a.dtmu - a.origin, pb);
+    // padded byte offset of node (xx, zz); zz >= -kGhost and xx >= -kGhostX on every use
+    auto po = [&](int xx, int zz) { return ((zz + kGhost) * pitch + xx + kGhostX) * 4; };
+    auto ingrid = [&](int xx, int zz) { return xx >= 0 && xx < nx && zz >= 0 && zz < nz; };
+    // the synthesised source of the pair belongs to step n (a.syn) or to step n+1 (syn2)
+    const float* syn = a.syn ? a.syn : pa.syn2;
+    const bool syn1 = a.syn != nullptr;
+
+    // ---------------- phase A: stage sigma, load the step-n velocity operands of V1
+    constexpr int NM1 = (H1 + 3) / 4, NH1 = (H1 * HC1 + 255) / 256;
+    constexpr int NMZ = (HSZ + 3) / 4;
+    constexpr int NHX = (H1 * HCS + 255) / 256, NHZ = (HSZ * HC1 + 255) / 256;
+    float mv[NM1], mb[NM1], msy[NM1], mu[NM1];
+    float hv[NH1], hb[NH1], ht[NH1], hsy[NH1];
+    {
+        float sxm[NM1], sxh[NHX], szm[NMZ], szh[NHZ];
+#pragma unroll
+        for (int k = 0; k < NM1; ++k) {
+            const int r = w + 4 * k;
+            sxm[k] = r < H1 ? bload(r_sxy, po(x, j0 - L1 + r)) : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < NHX; ++k) {
+            const int idx = tid + 256 * k;
+            const int r = idx / HCS, h = idx - r * HCS;
+            const int c = h < LS ? h : h + kTX;
+            sxh[k] = idx < H1 * HCS ? bload(r_sxy, po(i0 - LS + c, j0 - L1 + r)) : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < NMZ; ++k) {
+            const int r = w + 4 * k;
+            szm[k] = r < HSZ ? bload(r_szy, po(x, j0 - LS + r)) : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < NHZ; ++k) {
+            const int idx = tid + 256 * k;
+            const int r = idx / HC1, h = idx - r * HC1;
+            const int c = h < L1 ? h : h + kTX;
+            szh[k] = idx < HSZ * HC1 ? bload(r_szy, po(i0 - L1 + c, j0 - LS + r)) : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < NM1; ++k) {
+            const int r = w + 4 * k, z = j0 - L1 + r;
+            const bool in = r < H1 && xg && z >= 0 && z < nz;
+            const int o = po(x, z);
+            mv[k] = in ? bload(r_v, o) : 0.f;
+            mb[k] = in ? bload(r_bv, o) : 0.f;
+            mu[k] = (HAS_U && in && r >= L1 && r < L1 + TZ) ? bload(r_u, o) : 0.f;
+            msy[k] = (in && syn) ? syn[z * nx + x] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < NH1; ++k) {
+            const int idx = tid + 256 * k;
+            const int r = idx / HC1, h = idx - r * HC1;
+            const int c = h < L1 ? h : h + kTX;
+            const int xx = i0 - L1 + c, zz = j0 - L1 + r;
+            const bool in = idx < H1 * HC1 && ingrid(xx, zz);
+            const int o = po(xx, zz);
+            hv[k] = in ? bload(r_v, o) : 0.f;
+            hb[k] = in ? bload(r_bv, o) : 0.f;
+            hsy[k] = (in && syn) ? syn[zz * nx + xx] : 0.f;
+            ht[k] = in ? a.ax[xx] * a.az[zz] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < NM1; ++k) {
+            const int r = w + 4 * k;
+            if (r < H1) SX[r * WSX + lane + LS] = sxm[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NHX; ++k) {
+            const int idx = tid + 256 * k;
+            const int r = idx / HCS, h = idx - r * HCS;
+            if (idx < H1 * HCS) SX[r * WSX + (h < LS ? h : h + kTX)] = sxh[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NMZ; ++k) {
+            const int r = w + 4 * k;
+            if (r < HSZ) SZ[r * W1 + lane + L1] = szm[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NHZ; ++k) {
+            const int idx = tid + 256 * k;
+            const int r = idx / HC1, h = idx - r * HC1;
+            if (idx < HSZ * HC1) SZ[r * W1 + (h < L1 ? h : h + kTX)] = szh[k];
+        }
+    }
+    __syncthreads();
+
+    // ---------------- phase B: step-n velocity on V1
+    const int e0 = a.ps_off ? a.ps_off[t] : 0, e1 = a.ps_off ? a.ps_off[t + 1] : 0;
+    const int f0 = a.rec_off ? a.rec_off[t] : 0, f1 = a.rec_off ? a.rec_off[t + 1] : 0;
+    auto divs = [&](int xx, int zz, int r, int c, const float* amp, float cst) {
+        float ds = 0.f;
+        if (xx >= R && xx <= nx - R - 1) {  // div_s x-part frame
+            const float* row = SX + r * WSX + c + R;
+            float s = 0.f;
+#pragma unroll
+            for (int m = 1; m <= R; ++m) s += a.cx[m - 1] * (row[m - 1] - row[-m]);
+            ds = s;
+        }
+        if (zz >= R && zz <= nz - R - 1) {  // div_s z-part frame
+            const float* col = SZ + (r + R) * W1 + c;
+            float s = 0.f;
+#pragma unroll
+            for (int m = 1; m <= R; ++m) s += a.cz[m - 1] * (col[(m - 1) * W1] - col[-m * W1]);
+            ds += s;
+        }
+        if (e1 > e0) {
+            const int key = node_key(xx, zz);
+            for (int e = e0; e < e1; ++e) {
+                const int2 en = a.ps_ent[e];
+                if (en.x == key) ds += amp ? amp[en.y] : cst;
+            }
+        }
+        return ds;
+    };
+    auto gather = [&](int xx, int zz, float val, double* col) {
+        if (f1 > f0 && col) {
+            const int key = node_key(xx, zz);
+            for (int e = f0; e < f1; ++e) {
+                const int2 en = a.rec_ent[e];
+                if (en.x == key) col[en.y] = (double)val;
+            }
+        }
+    };
+    // dt*mu of the sigma updates, prefetched: main columns over the sigma_zy rows (which contain
+    // the sigma_xy rows and the tile), and the two packed halo lists
+    constexpr int NMD = (HS1Z + 3) / 4;
+    constexpr int NDX = (HS1X * HCX1 + 255) / 256, NDZ = (HS1Z * HCZ1 + 255) / 256;
+    float dmm[NMD], dmx[NDX], dmz[NDZ];
+#pragma unroll
+    for (int k = 0; k < NMD; ++k) {
+        const int q = w + 4 * k, z = j0 - (2 * R - 1) + q;
+        dmm[k] = (q < HS1Z && xg && z >= 0 && z < nz) ? bload(r_dm, po(x, z)) : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NDX; ++k) {
+        const int idx = tid + 256 * k;
+        const int r = idx / HCX1, h = idx - r * HCX1;
+        const int xx = h < 2 * R - 1 ? i0 - (2 * R - 1) + h : i0 + kTX + (h - (2 * R - 1));
+        const int zz = j0 - (R - 1) + r;
+        dmx[k] = (idx < HS1X * HCX1 && ingrid(xx, zz)) ? bload(r_dm, po(xx, zz)) : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < NDZ; ++k) {
+        const int idx = tid + 256 * k;
+        const int r = idx / HCZ1, h = idx - r * HCZ1;
+        const int xx = h < R - 1 ? i0 - (R - 1) + h : i0 + kTX + (h - (R - 1));
+        const int zz = j0 - (2 * R - 1) + r;
+        dmz[k] = (idx < HS1Z * HCZ1 && ingrid(xx, zz)) ? bload(r_dm, po(xx, zz)) : 0.f;
+    }
+
+    const float axl = xg ? a.ax[x] : 0.f;
+    float un1[NM1];  // displacement after step n at the tile's own rows
+#pragma unroll
+    for (int k = 0; k < NM1; ++k) {
+        const int r = w + 4 * k;
+        un1[k] = 0.f;
+        if (r >= H1) break;
+        const int z = j0 - L1 + r, c = lane + L1;
+        float vn = 0.f;
+        if (xg && z >= 0 && z < nz) {
+            const float ds = divs(x, z, r, c, a.ps_amp, a.ps_const) + (syn1 ? msy[k] : 0.f);
+            vn = (mv[k] + mb[k] * ds) * (axl * a.az[z]);
+            if (r >= L1 && r < L1 + TZ) {  // own node: outputs of step n
+                const float un = HAS_U ? mu[k] + vn * a.dt : 0.f;
+                un1[k] = un;
+                const long cc = (long)z * nx + x;
+                float out = vn;
+                if (HAS_U) {
+                    if (a.snap) a.snap[cc] = mu[k];
+                    out = un;
+                } else if (a.snap) {
+                    a.snap[cc] = mv[k];
+                }
+                if (a.dsamp) a.dsamp[cc] = out;
+                gather(x, z, out, a.seis_col);
+            }
+        }
+        VV[r * W1 + c] = vn;
+    }
+#pragma unroll
+    for (int k = 0; k < NH1; ++k) {
+        const int idx = tid + 256 * k;
+        if (idx >= H1 * HC1) break;
+        const int r = idx / HC1, h = idx - r * HC1;
+        const int c = h < L1 ? h : h + kTX;
+        const int xx = i0 - L1 + c, zz = j0 - L1 + r;
+        float vn = 0.f;
+        if (ingrid(xx, zz)) {
+            const float ds = divs(xx, zz, r, c, a.ps_amp, a.ps_const) + (syn1 ? hsy[k] : 0.f);
+            vn = (hv[k] + hb[k] * ds) * ht[k];
+        }
+        VV[r * W1 + c] = vn;
+    }
+    __syncthreads();
+
+    // ---------------- phase C: step-n sigma, in place, where step n+1 needs it
+    const bool xin = x >= R - 1 && x <= nx - R - 1;  // dx_v frame
+    auto dxv = [&](int rv, int c) {
+        const float* vrow = VV + rv * W1 + c;
+        float e = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) e += a.cx[m - 1] * (vrow[m] - vrow[1 - m]);
+        return e;
+    };
+    auto dzv = [&](int rv, int c) {
+        const float* vcol = VV + rv * W1 + c;
+        float e = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) e += a.cz[m - 1] * (vcol[m * W1] - vcol[(1 - m) * W1]);
+        return e;
+    };
+#pragma unroll
+    for (int k = 0; k < NMD; ++k) {
+        const int q = w + 4 * k;
+        if (q >= HS1Z) break;
+        const int z = j0 - (2 * R - 1) + q;
+        if (!xg || z < 0 || z >= nz) continue;
+        const int rv = z - (j0 - L1), c = lane + L1;
+        if (z >= R - 1 && z <= nz - R - 1) SZ[(rv + R) * W1 + c] += dmm[k] * dzv(rv, c);  // dz_v frame
+        if (rv >= V2R0 && rv <= V2R1 && xin) SX[rv * WSX + c + R] += dmm[k] * dxv(rv, c);
+    }
+#pragma unroll
+    for (int k = 0; k < NDX; ++k) {
+        const int idx = tid + 256 * k;
+        if (idx >= HS1X * HCX1) break;
+        const int r = idx / HCX1, h = idx - r * HCX1;
+        const int xx = h < 2 * R - 1 ? i0 - (2 * R - 1) + h : i0 + kTX + (h - (2 * R - 1));
+        const int zz = j0 - (R - 1) + r;
+        if (ingrid(xx, zz) && xx >= R - 1 && xx <= nx - R - 1) {
+            const int rv = zz - (j0 - L1), cv = xx - (i0 - L1);
+            SX[rv * WSX + cv + R] += dmx[k] * dxv(rv, cv);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NDZ; ++k) {
+        const int idx = tid + 256 * k;
+        if (idx >= HS1Z * HCZ1) break;
+        const int r = idx / HCZ1, h = idx - r * HCZ1;
+        const int xx = h < R - 1 ? i0 - (R - 1) + h : i0 + kTX + (h - (R - 1));
+        const int zz = j0 - (2 * R - 1) + r;
+        if (ingrid(xx, zz) && zz >= R - 1 && zz <= nz - R - 1) {
+            const int rv = zz - (j0 - L1), cv = xx - (i0 - L1);
+            SZ[(rv + R) * W1 + cv] += dmz[k] * dzv(rv, cv);
+        }
+    }
+    __syncthreads();
+
+    // ---------------- phase D: step-(n+1) velocity on V2, in place
+#pragma unroll
+    for (int k = 0; k < NM1; ++k) {
+        const int r = w + 4 * k;
+        if (r >= H1) break;
+        if (r < V2R0 || r > V2R1) continue;
+        const int z = j0 - L1 + r, c = lane + L1;
+        if (!xg || z < 0 || z >= nz) continue;  // stays 0
+        const float v1 = VV[r * W1 + c];
+        const float ds = divs(x, z, r, c, pa.ps_amp2, pa.ps_const2) + (syn1 ? 0.f : msy[k]);
+        const float vn = (v1 + mb[k] * ds) * (axl * a.az[z]);
+        if (r >= L1 && r < L1 + TZ) {  // own node: outputs of step n+1 and the new state
+            const long p = (long)z * pitch + x;
+            const long cc = (long)z * nx + x;
+            a.v_out[p] = vn;
+            float out = vn;
+            if (HAS_U) {
+                const float un = un1[k] + vn * a.dt;
+                a.u_out[p] = un;
+                if (pa.snap2) pa.snap2[cc] = un1[k];
+                out = un;
+            } else if (pa.snap2) {
+                pa.snap2[cc] = v1;
+            }
+            if (pa.dsamp2) pa.dsamp2[cc] = out;
+            gather(x, z, out, pa.seis_col2);
+        }
+        VV[r * W1 + c] = vn;
+    }
+#pragma unroll
+    for (int k = 0; k < NH1; ++k) {
+        const int idx = tid + 256 * k;
+        if (idx >= H1 * HC1) break;
+        const int r = idx / HC1, h = idx - r * HC1;
+        const int c = h < L1 ? h : h + kTX;
+        if (r < V2R0 || r > V2R1 || c < V2C0 || c > V2C1) continue;
+        const int xx = i0 - L1 + c, zz = j0 - L1 + r;
+        if (!ingrid(xx, zz)) continue;
+        const float ds = divs(xx, zz, r, c, pa.ps_amp2, pa.ps_const2) + (syn1 ? 0.f : hsy[k]);
+        VV[r * W1 + c] = (VV[r * W1 + c] + hb[k] * ds) * ht[k];
+    }
+    __syncthreads();
+
+    // ---------------- phase E: step-(n+1) sigma on the tile
+#pragma unroll
+    for (int k = 0; k < NMD; ++k) {
+        const int q = w + 4 * k;
+        if (q >= HS1Z) break;
+        const int z = j0 - (2 * R - 1) + q;
+        if (z < j0 || z >= j0 + TZ || !xg || z >= nz) continue;
+        const int rv = z - (j0 - L1), c = lane + L1;
+        const long p = (long)z * pitch + x;
+        if (xin) a.sxy_out[p] = SX[rv * WSX + c + R] + dmm[k] * dxv(rv, c);
+        if (z >= R - 1 && z <= nz - R - 1) a.szy_out[p] = SZ[(rv + R) * W1 + c] + dmm[k] * dzv(rv, c);
+    }
+}
+
 // Structure-kernel product of run_noise_adjoint.m:324-331 after a step that stores it:
 // K_mu -= fw_nth * (dx_v(u) .* dx_v(u_fwd) + dz_v(u) .* dz_v(u_fwd)) with u the new displacement
 // (padded) and u_fwd the forward snapshot (compact); the products are formed in FP64.
@@ -364,6 +720,23 @@ __global__ __launch_bounds__(256) void k_born_green(StepArgs a, float* sxy, floa
         for (int m = 1; m <= R; ++m) s += a.cz[m - 1] * (gin[c + m * nxl] - gin[c - (m - 1) * nxl]);
         szy[p] += d * s;
     }
+}
+
+void launch_pair(int R, bool with_u, const PairArgs& p, hipStream_t s)
+{
+    const dim3 g((unsigned)(kXCD * p.a.tiles_per_xcd)), b(256);
+#define FD2D_PAIR(RR)                                                                            \
+    case RR:                                                                                     \
+        if (with_u) hipLaunchKernelGGL((k_pair<RR, 1>), g, b, 0, s, p);                          \
+        else hipLaunchKernelGGL((k_pair<RR, 0>), g, b, 0, s, p);                                 \
+        break;
+    switch (R) {
+        FD2D_PAIR(1)
+        FD2D_PAIR(2)
+        FD2D_PAIR(3)
+        FD2D_PAIR(4)
+    }
+#undef FD2D_PAIR
 }
 
 void launch_step(int R, bool with_u, const StepArgs& a, hipStream_t s)

@@ -273,14 +273,24 @@ int fd2d_regularization_device(double* j, double* g, const double* m, const doub
                                const double* w, int64_t n_w, double alpha, double beta, void* stream);
 
 /*
+ * Steps per launch of the time loop (tuning; it has no counterpart in the reference). 1 (the
+ * default) launches every step on its own; 2 advances two consecutive time steps per launch
+ * (temporal blocking) wherever the two steps need at most one DFT sample, at most one synthesised
+ * source slice, no Born term and no structure-kernel product after the first of them. Results are
+ * the same either way.
+ * Returns FD2D_EINVAL for other values. Applies to solves started afterwards.
+ */
+int fd2d_set_steps_per_launch(int32_t steps);
+
+/*
  * Step-kernel timing (instrumentation; it has no counterpart in the reference). When enabled with
- * every > 0, every every-th time step of each solve brackets its fused step launch with HIP events
- * on the solve's stream; the durations are summed per solver until fd2d_timing_read(), which
- * returns and clears them. every = 0 disables it (the default).
+ * every > 0, each launch of the time loop that advances a step number divisible by `every` is
+ * bracketed with HIP events on the solve's stream; durations and step counts are summed per solver
+ * until fd2d_timing_read(), which returns and clears them. every = 0 disables it (the default).
  */
 typedef struct fd2d_timing {
-    int64_t launches[3][2];  /* [solver-1][0: fused step kernel, 1: reserved (0)] timed launches */
-    double ms[3][2];         /* summed kernel durations [ms] */
+    int64_t launches[3][2];  /* [solver-1][0]: timed launches, [1]: time steps those launches advanced */
+    double ms[3][2];         /* [solver-1][0]: summed launch durations [ms]; [1] reserved (0) */
     int64_t points[3];       /* nx*nz of the last timed solve of each solver */
 } fd2d_timing;
 
