@@ -275,7 +275,7 @@ struct Solver {
     PaddedLayout lay;
     DBuf<float> fields;  // two sets of v, sxy, szy (, u)
     DBuf<float> coefs;   // bv, dtmu
-    DBuf<float> ax_d, az_d;
+    DBuf<float> ax_d, az_d, axp_d, azp_d;
     float *v[2], *sxy[2], *szy[2], *u[2];
     int cur = 0;
     int tiles_x = 0, tiles_z = 0, n_tiles = 0;
@@ -288,9 +288,9 @@ struct Solver {
     {
         lay = PaddedLayout::make(g.nx, g.nz);
         const size_t E = lay.elems(), o = lay.origin();
-        // buffer-load offsets are 32-bit signed byte offsets into one padded array
-        if (E * sizeof(float) >= ((size_t)1 << 31))
-            raise(FD2D_EUNSUPPORTED, "nx x nz = %d x %d exceeds 2 GiB per padded field", g.nx, g.nz);
+        // buffer offsets are 32-bit byte offsets into the field pool (two sets of nf arrays)
+        if (E * sizeof(float) * 8 >= ((size_t)1 << 32))
+            raise(FD2D_EUNSUPPORTED, "nx x nz = %d x %d: the padded fields exceed 4 GiB", g.nx, g.nz);
         const int nf = with_u ? 4 : 3;
         fields.alloc(E * nf * 2);
         HIPCHK(hipMemsetAsync(fields.p, 0, E * nf * 2 * sizeof(float), st));
@@ -300,6 +300,11 @@ struct Solver {
         taper(cfg, g, ax, az);
         upload(ax_d, ax, st);
         upload(az_d, az, st);
+        std::vector<float> axp((size_t)lay.pitch, 0.f), azp((size_t)lay.rows, 0.f);
+        for (int i = 0; i < g.nx; ++i) axp[(size_t)i + kGhostX] = ax[(size_t)i];
+        for (int j = 0; j < g.nz; ++j) azp[(size_t)j + kGhost] = az[(size_t)j];
+        upload(axp_d, axp, st);
+        upload(azp_d, azp, st);
         for (int b = 0; b < 2; ++b) {
             float* base = fields.p + (size_t)b * nf * E;
             v[b] = base + o;
@@ -311,6 +316,15 @@ struct Solver {
         a.dtmu = coefs.p + E + o;
         a.ax = ax_d.p;
         a.az = az_d.p;
+        a.axp = axp_d.p;
+        a.azp = azp_d.p;
+        a.fbase = fields.p;
+        a.fbytes = (unsigned)(E * nf * 2 * sizeof(float));
+        a.cbase = coefs.p;
+        a.cbytes = (unsigned)(E * 2 * sizeof(float));
+        a.o_bv = 0;
+        a.o_dtmu = (int)E;
+        a.compact_bytes = (unsigned)(g.P * sizeof(float));
         a.nx = g.nx;
         a.nz = g.nz;
         a.pitch = lay.pitch;
@@ -384,6 +398,15 @@ struct Solver {
         a.sxy_out = sxy[1 - cur];
         a.szy_out = szy[1 - cur];
         a.u_out = u[1 - cur];
+        auto off = [&](const float* p) { return p ? (int)(p - (const float*)fields.p - (ptrdiff_t)lay.origin()) : 0; };
+        a.o_v_in = off(a.v_in);
+        a.o_sxy_in = off(a.sxy_in);
+        a.o_szy_in = off(a.szy_in);
+        a.o_u_in = off(a.u_in);
+        a.o_v_out = off(a.v_out);
+        a.o_sxy_out = off(a.sxy_out);
+        a.o_szy_out = off(a.szy_out);
+        a.o_u_out = off(a.u_out);
         a.ps_off = nullptr;
         a.ps_ent = nullptr;
         a.ps_amp = nullptr;

@@ -38,10 +38,12 @@ struct PaddedLayout {
         PaddedLayout l;
         l.nx = nx;
         l.nz = nz;
-        // every tile column (the last tile runs to the next multiple of 64) plus its 2R-1 halo
-        // stays inside its own padded row
-        l.pitch = ((kGhostX + nx + 63) / 64) * 64 + 64;
-        l.rows = nz + 2 * kGhost;
+        // whole tiles plus every halo of a one- or two-step launch (<= 4R-2 <= kGhost rows and
+        // columns) stay inside the array: kGhostX columns on the left, >= 32 on the right of the
+        // last tile, kGhost rows above and below the last tile row
+        const int tiles_x = (nx + kTX - 1) / kTX, tiles_z = (nz + kTileZ - 1) / kTileZ;
+        l.pitch = kTX * (tiles_x + 1);
+        l.rows = kGhost + kTileZ * tiles_z + kGhost;
         return l;
     }
 };
@@ -61,6 +63,16 @@ struct StepArgs {
     const float* dtmu;   // dt * mu
     const float* ax;     // separable taper, nx values
     const float* az;     // nz values
+    const float* axp;    // the taper padded like the fields: axp[x + kGhostX], azp[z + kGhost], zero
+    const float* azp;    // outside the grid
+    // the padded arrays as buffer pools: fields (both ping-pong sets) and coefficients (dt/rho,
+    // dt*mu); o_* are element offsets of each padded array (its row -kGhost, column -kGhostX)
+    const float* fbase;
+    unsigned fbytes;
+    const float* cbase;
+    unsigned cbytes;
+    int o_v_in, o_sxy_in, o_szy_in, o_u_in, o_v_out, o_sxy_out, o_szy_out, o_u_out, o_bv, o_dtmu;
+    unsigned compact_bytes;  // nx*nz*4: the range of the compact arrays
     int nx, nz, pitch;
     int tiles_x, n_tiles;           // tiles across x, tiles in all
     int tiles_per_xcd;              // tile t = (g % 8) * tiles_per_xcd + g / 8 for workgroup g

@@ -20,17 +20,22 @@ namespace fd2d {
 // ---------------------------------------------------------------------------------------------
 // One workgroup (4 wavefronts) advances one kTX x kTileZ tile by one step:
 //   phase 1 issues, in one burst, the loads of sigma over the tile plus the halo that the velocity
-//           stencil of the halo needs, the velocity operands and (structure-kernel steps) the
-//           forward snapshot, then stores them to LDS;
+//           stencil of the halo needs and the velocity operands, then stores sigma to LDS;
 //   phase 2 computes the velocity (and displacement) on the tile plus an (R-1, R) halo in both
 //           directions -- the halo is recomputed, bit-identically, by the neighbouring tiles --
 //           and writes the outputs of the tile's own nodes;
-//   phase 3 updates sigma (and K_mu) on the tile.
+//   phase 3 updates sigma on the tile.
 // Lane l of every wavefront owns column i0 + l; wavefront w owns rows w, w+4, ... of each region,
 // so row offsets are wave-uniform and LDS row offsets fold into instruction immediates. The few
 // halo columns (2R-1 of the velocity region, 2(2R-1) of the sigma_xy region) are handled as one
-// packed list. Global loads are raw buffer loads with the whole byte offset in the VGPR, so the
-// hardware range check covers every load (out-of-range reads return 0).
+// packed list.
+//
+// Branch-free by layout: every padded array covers the whole tiles plus their halos, and outside
+// the grid the fields, the coefficients dt/rho and dt*mu and the padded taper are zero, so the
+// stencils run unguarded over every cell (an outside cell computes 0 and stores 0) and the frames
+// of dx_v / dz_v / div_s are 0/1 factors. Only the compact arrays (source slices, snapshots, DFT
+// samples, Born inputs) need an in-grid test, folded into a buffer offset past their end for cells
+// outside the grid (range-checked buffer loads return 0 there and stores are dropped).
 //
 // Tiles are dealt to the XCDs in contiguous bands (tile = (g % 8) * tiles_per_xcd + g / 8), so a
 // tile's x-neighbour, and a tile row later its z-neighbour, read the shared halo lines from the
@@ -43,10 +48,33 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, unsigne
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
 }
-__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int off)
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned off)
 {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, 0);
+}
+constexpr unsigned kOutside = 0x80000000u;  // a buffer offset past every compact array
+
+// Addressing shared by the step kernels.
+struct Grid {
+    int nx, nz, pitch;
+    __amdgpu_buffer_rsrc_t rf, rc;
+    __device__ unsigned pofs(int arr, int xx, int zz) const
+    {
+        return (unsigned)(arr + (zz + kGhost) * pitch + xx + kGhostX) * 4u;
+    }
+    __device__ float field(int arr, int xx, int zz) const { return bload(rf, pofs(arr, xx, zz)); }
+    __device__ float coef(int arr, int xx, int zz) const { return bload(rc, pofs(arr, xx, zz)); }
+    __device__ void put(int arr, int xx, int zz, float v) const { bstore(rf, pofs(arr, xx, zz), v); }
+    // byte offset of (xx, zz) in a compact nx x nz array, or past its end outside the grid
+    __device__ unsigned cofs(int xx, int zz) const
+    {
+        return (xx >= 0 && xx < nx && zz >= 0 && zz < nz) ? (unsigned)(zz * nx + xx) * 4u : kOutside;
+    }
+};
 
 template <int R, int MODE>
 __global__ __launch_bounds__(256) void k_tile(StepArgs a)
@@ -59,30 +87,22 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
     constexpr int HSZ = HV + 2 * R - 1;     // sigma_zy region: rows    j0-(2R-1) .. j0+TZ-1+2R-1
     constexpr int HCV = 2 * R - 1;          // halo columns of the velocity / sigma_zy regions
     constexpr int HCS = 2 * (2 * R - 1);    // halo columns of the sigma_xy region
-    __shared__ float sx[HV * WSX];
-    __shared__ float sz[HSZ * WV];
-    __shared__ float vv[HV * WV];
+    __shared__ float sx[HV * WSX + 1];      // + 1: the sink of the packed lists' idle lanes
+    __shared__ float sz[HSZ * WV + 1];
+    __shared__ float vv[HV * WV + 1];
 
     const int g = blockIdx.x;
     const int t = (g % kXCD) * a.tiles_per_xcd + g / kXCD;
     if (t >= a.n_tiles) return;
     const int i0 = (t % a.tiles_x) * kTX, j0 = (t / a.tiles_x) * TZ;
-    const int nx = a.nx, nz = a.nz, pitch = a.pitch;
+    const int nx = a.nx, nz = a.nz;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x = i0 + lane;                 // this lane's own column
-    const bool xg = x < nx;
-
-    const unsigned pb = a.padded_bytes;
-    const auto r_sxy = rsrc(a.sxy_in - a.origin, pb), r_szy = rsrc(a.szy_in - a.origin, pb);
-    const auto r_v = rsrc(a.v_in - a.origin, pb), r_bv = rsrc(a.bv - a.origin, pb);
-    const auto r_u = rsrc(HAS_U ? a.u_in - a.origin : a.v_in - a.origin, pb);
-    const auto r_dm = rsrc(a.dtmu - a.origin, pb);
-    // padded byte offset of node (xx, zz); zz >= -kGhost and xx >= -kGhostX on every use
-    auto po = [&](int xx, int zz) { return ((zz + kGhost) * pitch + xx + kGhostX) * 4; };
-    auto co = [&](int xx, int zz) { return (zz * nx + xx) * 4; };  // compact byte offset (in grid)
-    auto ingrid = [&](int xx, int zz) { return xx >= 0 && xx < nx && zz >= 0 && zz < nz; };
-    const float* syn = a.syn;
+    const Grid G{nx, nz, a.pitch, rsrc(a.fbase, a.fbytes), rsrc(a.cbase, a.cbytes)};
+    const unsigned cb = a.compact_bytes;
+    const bool has_syn = a.syn != nullptr;
+    const auto r_syn = rsrc(has_syn ? (const void*)a.syn : (const void*)a.fbase, has_syn ? cb : 0u);
 
     // ---------------- phase 1
     constexpr int NRV = (HV + 3) / 4, NRZ = (HSZ + 3) / 4;
@@ -90,54 +110,46 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
     float msx[NRV], msz[NRZ];
     float mv[NRV], mb[NRV], mu[NRV], msy[NRV];
     float hsx[NHS], hsz[NHZ];
-    float hv[NHV], hb[NHV], hu[NHV], hsy[NHV], ht[NHV];
-    const float axl = xg ? a.ax[x] : 0.f;
+    float hv[NHV], hb[NHV], hsy[NHV], ht[NHV];
 #pragma unroll
     for (int k = 0; k < NRV; ++k) {
         const int r = w + 4 * k, z = j0 - (R - 1) + r;
-        const bool live = r < HV;
-        const int o = po(x, z);
-        msx[k] = live ? bload(r_sxy, o) : 0.f;
-        const bool in = live && xg && z >= 0 && z < nz;
-        mv[k] = in ? bload(r_v, o) : 0.f;
-        mb[k] = in ? bload(r_bv, o) : 0.f;
-        mu[k] = (HAS_U && in) ? bload(r_u, o) : 0.f;
-        msy[k] = (in && syn) ? syn[co(x, z) / 4] : 0.f;
+        msx[k] = G.field(a.o_sxy_in, x, z);
+        mv[k] = G.field(a.o_v_in, x, z);
+        mb[k] = G.coef(a.o_bv, x, z);
+        mu[k] = (HAS_U && r >= R - 1 && r < R - 1 + TZ) ? G.field(a.o_u_in, x, z) : 0.f;
+        msy[k] = has_syn ? bload(r_syn, G.cofs(x, z)) : 0.f;
     }
 #pragma unroll
     for (int k = 0; k < NRZ; ++k) {
         const int r = w + 4 * k, z = j0 - (2 * R - 1) + r;
-        msz[k] = r < HSZ ? bload(r_szy, po(x, z)) : 0.f;
+        msz[k] = G.field(a.o_szy_in, x, z);
     }
+    // packed halo lists: idle lanes of the last round repeat the last cell and store to the sink
 #pragma unroll
     for (int k = 0; k < NHS; ++k) {
-        const int idx = tid + 256 * k;
+        const int idx = min(tid + 256 * k, HV * HCS - 1);
         const int r = idx / HCS, h = idx - r * HCS;
         const int c = h < 2 * R - 1 ? h : h + kTX;            // sigma_xy region column
-        const int xx = i0 - (2 * R - 1) + c, zz = j0 - (R - 1) + r;
-        hsx[k] = idx < HV * HCS ? bload(r_sxy, po(xx, zz)) : 0.f;
+        hsx[k] = G.field(a.o_sxy_in, i0 - (2 * R - 1) + c, j0 - (R - 1) + r);
     }
 #pragma unroll
     for (int k = 0; k < NHZ; ++k) {
-        const int idx = tid + 256 * k;
+        const int idx = min(tid + 256 * k, HSZ * HCV - 1);
         const int r = idx / HCV, h = idx - r * HCV;
         const int c = h < R - 1 ? h : h + kTX;                // velocity-region column
-        const int xx = i0 - (R - 1) + c, zz = j0 - (2 * R - 1) + r;
-        hsz[k] = idx < HSZ * HCV ? bload(r_szy, po(xx, zz)) : 0.f;
+        hsz[k] = G.field(a.o_szy_in, i0 - (R - 1) + c, j0 - (2 * R - 1) + r);
     }
 #pragma unroll
     for (int k = 0; k < NHV; ++k) {
-        const int idx = tid + 256 * k;
+        const int idx = min(tid + 256 * k, HV * HCV - 1);
         const int r = idx / HCV, h = idx - r * HCV;
         const int c = h < R - 1 ? h : h + kTX;
         const int xx = i0 - (R - 1) + c, zz = j0 - (R - 1) + r;
-        const bool in = idx < HV * HCV && ingrid(xx, zz);
-        const int o = po(xx, zz);
-        hv[k] = in ? bload(r_v, o) : 0.f;
-        hb[k] = in ? bload(r_bv, o) : 0.f;
-        hu[k] = (HAS_U && in) ? bload(r_u, o) : 0.f;
-        hsy[k] = (in && syn) ? syn[co(xx, zz) / 4] : 0.f;
-        ht[k] = in ? a.ax[xx] * a.az[zz] : 0.f;
+        hv[k] = G.field(a.o_v_in, xx, zz);
+        hb[k] = G.coef(a.o_bv, xx, zz);
+        hsy[k] = has_syn ? bload(r_syn, G.cofs(xx, zz)) : 0.f;
+        ht[k] = a.axp[xx + kGhostX] * a.azp[zz + kGhost];
     }
     // LDS stores
 #pragma unroll
@@ -154,36 +166,31 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
     for (int k = 0; k < NHS; ++k) {
         const int idx = tid + 256 * k;
         const int r = idx / HCS, h = idx - r * HCS;
-        if (idx < HV * HCS) sx[r * WSX + (h < 2 * R - 1 ? h : h + kTX)] = hsx[k];
+        sx[idx < HV * HCS ? r * WSX + (h < 2 * R - 1 ? h : h + kTX) : HV * WSX] = hsx[k];
     }
 #pragma unroll
     for (int k = 0; k < NHZ; ++k) {
         const int idx = tid + 256 * k;
         const int r = idx / HCV, h = idx - r * HCV;
-        if (idx < HSZ * HCV) sz[r * WV + (h < R - 1 ? h : h + kTX)] = hsz[k];
+        sz[idx < HSZ * HCV ? r * WV + (h < R - 1 ? h : h + kTX) : HSZ * WV] = hsz[k];
     }
     __syncthreads();
 
     // ---------------- phase 2
     const int e0 = a.ps_off ? a.ps_off[t] : 0, e1 = a.ps_off ? a.ps_off[t + 1] : 0;
     const int f0 = a.rec_off ? a.rec_off[t] : 0, f1 = a.rec_off ? a.rec_off[t + 1] : 0;
-    // velocity at (xx, zz) = region cell (r, c) from the staged sigma; the point sources of the node
-    auto divs = [&](int xx, int zz, int r, int c) {
-        float ds = 0.f;
-        if (xx >= R && xx <= nx - R - 1) {  // div_s x-part frame
-            const float* row = sx + r * WSX + c + R;
-            float s = 0.f;
+    // div_s at region cell (r, c) = node (xx, zz) from the staged sigma, with the frame factors
+    // fx (x-part, R <= x <= nx-R-1) and fz (z-part, R <= z <= nz-R-1), plus the point sources
+    auto divs = [&](int xx, int zz, int r, int c, float fx, float fz) {
+        const float* row = sx + r * WSX + c + R;
+        float s = 0.f;
 #pragma unroll
-            for (int m = 1; m <= R; ++m) s += a.cx[m - 1] * (row[m - 1] - row[-m]);
-            ds = s;
-        }
-        if (zz >= R && zz <= nz - R - 1) {  // div_s z-part frame
-            const float* col = sz + (r + R) * WV + c;
-            float s = 0.f;
+        for (int m = 1; m <= R; ++m) s += a.cx[m - 1] * (row[m - 1] - row[-m]);
+        const float* col = sz + (r + R) * WV + c;
+        float q = 0.f;
 #pragma unroll
-            for (int m = 1; m <= R; ++m) s += a.cz[m - 1] * (col[(m - 1) * WV] - col[-m * WV]);
-            ds += s;
-        }
+        for (int m = 1; m <= R; ++m) q += a.cz[m - 1] * (col[(m - 1) * WV] - col[-m * WV]);
+        float ds = s * fx + q * fz;
         if (e1 > e0) {
             const int key = node_key(xx, zz);
             for (int e = e0; e < e1; ++e) {
@@ -193,35 +200,34 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
         }
         return ds;
     };
+    auto frame = [](int v, int lo, int hi) { return (v >= lo && v <= hi) ? 1.f : 0.f; };
+    const float fxl = frame(x, R, nx - R - 1);
+    const float axl = a.axp[x + kGhostX];
 #pragma unroll
     for (int k = 0; k < NRV; ++k) {
         const int r = w + 4 * k;
         if (r >= HV) break;
         const int z = j0 - (R - 1) + r, c = lane + R - 1;
-        float vn = 0.f;
-        if (xg && z >= 0 && z < nz) {
-            const float ds = divs(x, z, r, c) + msy[k];
-            vn = (mv[k] + mb[k] * ds) * (axl * a.az[z]);
-            const float un = HAS_U ? mu[k] + vn * a.dt : 0.f;
-            if (r >= R - 1 && r < R - 1 + TZ) {  // own node: outputs
-                const long p = (long)z * pitch + x;
-                const long cc = (long)z * nx + x;
-                a.v_out[p] = vn;
-                float out = vn;
-                if (HAS_U) {
-                    a.u_out[p] = un;
-                    if (a.snap) a.snap[cc] = mu[k];
-                    out = un;
-                } else if (a.snap) {
-                    a.snap[cc] = mv[k];
-                }
-                if (a.dsamp) a.dsamp[cc] = out;
-                if (f1 > f0) {
-                    const int key = node_key(x, z);
-                    for (int e = f0; e < f1; ++e) {
-                        const int2 en = a.rec_ent[e];
-                        if (en.x == key) a.seis_col[en.y] = (double)out;
-                    }
+        const float ds = divs(x, z, r, c, fxl, frame(z, R, nz - R - 1)) + msy[k];
+        const float vn = (mv[k] + mb[k] * ds) * (axl * a.azp[z + kGhost]);
+        if (r >= R - 1 && r < R - 1 + TZ) {  // own node: outputs (zeros outside the grid)
+            G.put(a.o_v_out, x, z, vn);
+            float out = vn;
+            const unsigned co = G.cofs(x, z);
+            if (HAS_U) {
+                const float un = mu[k] + vn * a.dt;
+                G.put(a.o_u_out, x, z, un);
+                if (a.snap) bstore(rsrc(a.snap, cb), co, mu[k]);
+                out = un;
+            } else if (a.snap) {
+                bstore(rsrc(a.snap, cb), co, mv[k]);
+            }
+            if (a.dsamp) bstore(rsrc(a.dsamp, cb), co, out);
+            if (f1 > f0) {
+                const int key = node_key(x, z);
+                for (int e = f0; e < f1; ++e) {
+                    const int2 en = a.rec_ent[e];
+                    if (en.x == key) a.seis_col[en.y] = (double)out;
                 }
             }
         }
@@ -229,17 +235,13 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
     }
 #pragma unroll
     for (int k = 0; k < NHV; ++k) {
-        const int idx = tid + 256 * k;
-        if (idx >= HV * HCV) break;
+        const int idx = min(tid + 256 * k, HV * HCV - 1);
         const int r = idx / HCV, h = idx - r * HCV;
         const int c = h < R - 1 ? h : h + kTX;
         const int xx = i0 - (R - 1) + c, zz = j0 - (R - 1) + r;
-        float vn = 0.f;
-        if (ingrid(xx, zz)) {
-            const float ds = divs(xx, zz, r, c) + hsy[k];
-            vn = (hv[k] + hb[k] * ds) * ht[k];
-        }
-        vv[r * WV + c] = vn;
+        const float ds = divs(xx, zz, r, c, frame(xx, R, nx - R - 1), frame(zz, R, nz - R - 1)) + hsy[k];
+        const float vn = (hv[k] + hb[k] * ds) * ht[k];
+        vv[tid + 256 * k < HV * HCV ? r * WV + c : HV * WV] = vn;
     }
     __syncthreads();
 
@@ -247,53 +249,45 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
     constexpr int NT = TZ / 4;
     float rm[NT];
 #pragma unroll
-    for (int k = 0; k < NT; ++k) {
-        const int z = j0 + w + 4 * k;
-        rm[k] = (xg && z < nz) ? bload(r_dm, po(x, z)) : 0.f;
-    }
-    const bool xin = x >= R - 1 && x <= nx - R - 1;  // dx_v frame
+    for (int k = 0; k < NT; ++k) rm[k] = G.coef(a.o_dtmu, x, j0 + w + 4 * k);
+    const float fxs = frame(x, R - 1, nx - R - 1);  // dx_v frame
     const int c = lane + R - 1;
-    const long nxl = nx;
+    const bool born = a.born_dmu != nullptr;
+    const auto r_b0 = rsrc(born ? (const void*)a.born_in0 : (const void*)a.fbase, born ? cb : 0u);
+    const auto r_b1 = rsrc(born ? (const void*)a.born_in1 : (const void*)a.fbase, born ? cb : 0u);
+    const auto r_bd = rsrc(born ? (const void*)a.born_dmu : (const void*)a.fbase, born ? cb : 0u);
 #pragma unroll
     for (int k = 0; k < NT; ++k) {
         const int lz = w + 4 * k, z = j0 + lz;
-        if (!xg || z >= nz) continue;
         const int r = lz + R - 1;
-        const long p = (long)z * pitch + x;
-        const long cc = (long)z * nx + x;
-        const bool zin = z >= R - 1 && z <= nz - R - 1;  // dz_v frame
+        const float fzs = frame(z, R - 1, nz - R - 1);  // dz_v frame
         const float dm = rm[k];
         const float* vrow = vv + r * WV + c;
-        if (xin) {
-            float e = 0.f;
+        float ex = 0.f, ez = 0.f;
 #pragma unroll
-            for (int m = 1; m <= R; ++m) e += a.cx[m - 1] * (vrow[m] - vrow[1 - m]);
-            float sn = sx[r * WSX + c + R] + dm * e;
-            if (a.born_dmu) {
-                float s = 0.f;
+        for (int m = 1; m <= R; ++m) ex += a.cx[m - 1] * (vrow[m] - vrow[1 - m]);
 #pragma unroll
-                for (int m = 1; m <= R; ++m)
-                    s += a.cx[m - 1] * ((a.born_in0[cc + m] - a.born_in1[cc + m]) -
-                                        (a.born_in0[cc - m + 1] - a.born_in1[cc - m + 1]));
-                sn -= a.born_dmu[cc] * s;
+        for (int m = 1; m <= R; ++m) ez += a.cz[m - 1] * (vrow[m * WV] - vrow[(1 - m) * WV]);
+        float snx = sx[r * WSX + c + R] + dm * (ex * fxs);
+        float snz = sz[(r + R) * WV + c] + dm * (ez * fzs);
+        if (born) {
+            // Born terms: - dmu * D(C_in(n) - C_in(n+1)) (FP32 snapshots, as the reference)
+            const unsigned co = G.cofs(x, z);
+            auto bd = [&](unsigned o) { return bload(r_b0, o) - bload(r_b1, o); };
+            float bx = 0.f, bz = 0.f;
+            if (co != kOutside) {
+                const unsigned nx4 = (unsigned)nx * 4u;
+#pragma unroll
+                for (int m = 1; m <= R; ++m) bx += a.cx[m - 1] * (bd(co + 4u * m) - bd(co - 4u * (m - 1)));
+#pragma unroll
+                for (int m = 1; m <= R; ++m) bz += a.cz[m - 1] * (bd(co + nx4 * m) - bd(co - nx4 * (m - 1)));
             }
-            a.sxy_out[p] = sn;
+            const float dmu = bload(r_bd, co);
+            snx -= dmu * (bx * fxs);
+            snz -= dmu * (bz * fzs);
         }
-        if (zin) {
-            float e = 0.f;
-#pragma unroll
-            for (int m = 1; m <= R; ++m) e += a.cz[m - 1] * (vrow[m * WV] - vrow[(1 - m) * WV]);
-            float sn = sz[(r + R) * WV + c] + dm * e;
-            if (a.born_dmu) {
-                float s = 0.f;
-#pragma unroll
-                for (int m = 1; m <= R; ++m)
-                    s += a.cz[m - 1] * ((a.born_in0[cc + m * nxl] - a.born_in1[cc + m * nxl]) -
-                                        (a.born_in0[cc - (m - 1) * nxl] - a.born_in1[cc - (m - 1) * nxl]));
-                sn -= a.born_dmu[cc] * s;
-            }
-            a.szy_out[p] = sn;
-        }
+        G.put(a.o_sxy_out, x, z, snx);
+        G.put(a.o_szy_out, x, z, snz);
     }
 }
 
@@ -328,32 +322,26 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     constexpr int HS1Z = TZ + 4 * R - 2;     // rows of the step-n sigma_zy update: j0-(2R-1) ..
     constexpr int HCX1 = 2 * (2 * R - 1);    // halo columns of the sigma_xy update
     constexpr int HCZ1 = 2 * R - 1;          // halo columns of the sigma_zy update (V2 halo)
-    __shared__ float SX[H1 * WSX];
-    __shared__ float SZ[HSZ * W1];
-    __shared__ float VV[H1 * W1];
+    __shared__ float SX[H1 * WSX + 1];       // + 1: the sink of the packed lists' idle lanes
+    __shared__ float SZ[HSZ * W1 + 1];
+    __shared__ float VV[H1 * W1 + 1];
 
     const StepArgs& a = pa.a;
     const int g = blockIdx.x;
     const int t = (g % kXCD) * a.tiles_per_xcd + g / kXCD;
     if (t >= a.n_tiles) return;
     const int i0 = (t % a.tiles_x) * kTX, j0 = (t / a.tiles_x) * TZ;
-    const int nx = a.nx, nz = a.nz, pitch = a.pitch;
+    const int nx = a.nx, nz = a.nz;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x = i0 + lane;
-    const bool xg = x < nx;
-
-    const unsigned pb = a.padded_bytes;
-    const auto r_sxy = rsrc(a.sxy_in - a.origin, pb), r_szy = rsrc(a.szy_in - a.origin, pb);
-    const auto r_v = rsrc(a.v_in - a.origin, pb), r_bv = rsrc(a.bv - a.origin, pb);
-    const auto r_u = rsrc(HAS_U ? a.u_in - a.origin : a.v_in - a.origin, pb);
-    const auto r_dm = rsrc(a.dtmu - a.origin, pb);
-    // padded byte offset of node (xx, zz); zz >= -kGhost and xx >= -kGhostX on every use
-    auto po = [&](int xx, int zz) { return ((zz + kGhost) * pitch + xx + kGhostX) * 4; };
-    auto ingrid = [&](int xx, int zz) { return xx >= 0 && xx < nx && zz >= 0 && zz < nz; };
+    const Grid G{nx, nz, a.pitch, rsrc(a.fbase, a.fbytes), rsrc(a.cbase, a.cbytes)};
+    const unsigned cb = a.compact_bytes;
     // the synthesised source of the pair belongs to step n (a.syn) or to step n+1 (syn2)
     const float* syn = a.syn ? a.syn : pa.syn2;
-    const bool syn1 = a.syn != nullptr;
+    const bool syn1 = a.syn != nullptr, has_syn = syn != nullptr;
+    const auto r_syn = rsrc(has_syn ? (const void*)syn : (const void*)a.fbase, has_syn ? cb : 0u);
+    auto frame = [](int v, int lo, int hi) { return (v >= lo && v <= hi) ? 1.f : 0.f; };
 
     // ---------------- phase A: stage sigma, load the step-n velocity operands of V1
     constexpr int NM1 = (H1 + 3) / 4, NH1 = (H1 * HC1 + 255) / 256;
@@ -364,51 +352,41 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     {
         float sxm[NM1], sxh[NHX], szm[NMZ], szh[NHZ];
 #pragma unroll
-        for (int k = 0; k < NM1; ++k) {
-            const int r = w + 4 * k;
-            sxm[k] = r < H1 ? bload(r_sxy, po(x, j0 - L1 + r)) : 0.f;
-        }
+        for (int k = 0; k < NM1; ++k) sxm[k] = G.field(a.o_sxy_in, x, j0 - L1 + w + 4 * k);
 #pragma unroll
         for (int k = 0; k < NHX; ++k) {
-            const int idx = tid + 256 * k;
+            const int idx = min(tid + 256 * k, H1 * HCS - 1);
             const int r = idx / HCS, h = idx - r * HCS;
             const int c = h < LS ? h : h + kTX;
-            sxh[k] = idx < H1 * HCS ? bload(r_sxy, po(i0 - LS + c, j0 - L1 + r)) : 0.f;
+            sxh[k] = G.field(a.o_sxy_in, i0 - LS + c, j0 - L1 + r);
         }
 #pragma unroll
-        for (int k = 0; k < NMZ; ++k) {
-            const int r = w + 4 * k;
-            szm[k] = r < HSZ ? bload(r_szy, po(x, j0 - LS + r)) : 0.f;
-        }
+        for (int k = 0; k < NMZ; ++k) szm[k] = G.field(a.o_szy_in, x, j0 - LS + w + 4 * k);
 #pragma unroll
         for (int k = 0; k < NHZ; ++k) {
-            const int idx = tid + 256 * k;
+            const int idx = min(tid + 256 * k, HSZ * HC1 - 1);
             const int r = idx / HC1, h = idx - r * HC1;
             const int c = h < L1 ? h : h + kTX;
-            szh[k] = idx < HSZ * HC1 ? bload(r_szy, po(i0 - L1 + c, j0 - LS + r)) : 0.f;
+            szh[k] = G.field(a.o_szy_in, i0 - L1 + c, j0 - LS + r);
         }
 #pragma unroll
         for (int k = 0; k < NM1; ++k) {
             const int r = w + 4 * k, z = j0 - L1 + r;
-            const bool in = r < H1 && xg && z >= 0 && z < nz;
-            const int o = po(x, z);
-            mv[k] = in ? bload(r_v, o) : 0.f;
-            mb[k] = in ? bload(r_bv, o) : 0.f;
-            mu[k] = (HAS_U && in && r >= L1 && r < L1 + TZ) ? bload(r_u, o) : 0.f;
-            msy[k] = (in && syn) ? syn[z * nx + x] : 0.f;
+            mv[k] = G.field(a.o_v_in, x, z);
+            mb[k] = G.coef(a.o_bv, x, z);
+            mu[k] = (HAS_U && r >= L1 && r < L1 + TZ) ? G.field(a.o_u_in, x, z) : 0.f;
+            msy[k] = has_syn ? bload(r_syn, G.cofs(x, z)) : 0.f;
         }
 #pragma unroll
         for (int k = 0; k < NH1; ++k) {
-            const int idx = tid + 256 * k;
+            const int idx = min(tid + 256 * k, H1 * HC1 - 1);
             const int r = idx / HC1, h = idx - r * HC1;
             const int c = h < L1 ? h : h + kTX;
             const int xx = i0 - L1 + c, zz = j0 - L1 + r;
-            const bool in = idx < H1 * HC1 && ingrid(xx, zz);
-            const int o = po(xx, zz);
-            hv[k] = in ? bload(r_v, o) : 0.f;
-            hb[k] = in ? bload(r_bv, o) : 0.f;
-            hsy[k] = (in && syn) ? syn[zz * nx + xx] : 0.f;
-            ht[k] = in ? a.ax[xx] * a.az[zz] : 0.f;
+            hv[k] = G.field(a.o_v_in, xx, zz);
+            hb[k] = G.coef(a.o_bv, xx, zz);
+            hsy[k] = has_syn ? bload(r_syn, G.cofs(xx, zz)) : 0.f;
+            ht[k] = a.axp[xx + kGhostX] * a.azp[zz + kGhost];
         }
 #pragma unroll
         for (int k = 0; k < NM1; ++k) {
@@ -419,7 +397,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
         for (int k = 0; k < NHX; ++k) {
             const int idx = tid + 256 * k;
             const int r = idx / HCS, h = idx - r * HCS;
-            if (idx < H1 * HCS) SX[r * WSX + (h < LS ? h : h + kTX)] = sxh[k];
+            SX[idx < H1 * HCS ? r * WSX + (h < LS ? h : h + kTX) : H1 * WSX] = sxh[k];
         }
 #pragma unroll
         for (int k = 0; k < NMZ; ++k) {
@@ -430,7 +408,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
         for (int k = 0; k < NHZ; ++k) {
             const int idx = tid + 256 * k;
             const int r = idx / HC1, h = idx - r * HC1;
-            if (idx < HSZ * HC1) SZ[r * W1 + (h < L1 ? h : h + kTX)] = szh[k];
+            SZ[idx < HSZ * HC1 ? r * W1 + (h < L1 ? h : h + kTX) : HSZ * W1] = szh[k];
         }
     }
     __syncthreads();
@@ -438,22 +416,16 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     // ---------------- phase B: step-n velocity on V1
     const int e0 = a.ps_off ? a.ps_off[t] : 0, e1 = a.ps_off ? a.ps_off[t + 1] : 0;
     const int f0 = a.rec_off ? a.rec_off[t] : 0, f1 = a.rec_off ? a.rec_off[t + 1] : 0;
-    auto divs = [&](int xx, int zz, int r, int c, const float* amp, float cst) {
-        float ds = 0.f;
-        if (xx >= R && xx <= nx - R - 1) {  // div_s x-part frame
-            const float* row = SX + r * WSX + c + R;
-            float s = 0.f;
+    auto divs = [&](int xx, int zz, int r, int c, float fx, float fz, const float* amp, float cst) {
+        const float* row = SX + r * WSX + c + R;
+        float s = 0.f;
 #pragma unroll
-            for (int m = 1; m <= R; ++m) s += a.cx[m - 1] * (row[m - 1] - row[-m]);
-            ds = s;
-        }
-        if (zz >= R && zz <= nz - R - 1) {  // div_s z-part frame
-            const float* col = SZ + (r + R) * W1 + c;
-            float s = 0.f;
+        for (int m = 1; m <= R; ++m) s += a.cx[m - 1] * (row[m - 1] - row[-m]);
+        const float* col = SZ + (r + R) * W1 + c;
+        float q = 0.f;
 #pragma unroll
-            for (int m = 1; m <= R; ++m) s += a.cz[m - 1] * (col[(m - 1) * W1] - col[-m * W1]);
-            ds += s;
-        }
+        for (int m = 1; m <= R; ++m) q += a.cz[m - 1] * (col[(m - 1) * W1] - col[-m * W1]);
+        float ds = s * fx + q * fz;
         if (e1 > e0) {
             const int key = node_key(xx, zz);
             for (int e = e0; e < e1; ++e) {
@@ -478,28 +450,24 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     constexpr int NDX = (HS1X * HCX1 + 255) / 256, NDZ = (HS1Z * HCZ1 + 255) / 256;
     float dmm[NMD], dmx[NDX], dmz[NDZ];
 #pragma unroll
-    for (int k = 0; k < NMD; ++k) {
-        const int q = w + 4 * k, z = j0 - (2 * R - 1) + q;
-        dmm[k] = (q < HS1Z && xg && z >= 0 && z < nz) ? bload(r_dm, po(x, z)) : 0.f;
-    }
+    for (int k = 0; k < NMD; ++k) dmm[k] = G.coef(a.o_dtmu, x, j0 - (2 * R - 1) + w + 4 * k);
 #pragma unroll
     for (int k = 0; k < NDX; ++k) {
-        const int idx = tid + 256 * k;
+        const int idx = min(tid + 256 * k, HS1X * HCX1 - 1);
         const int r = idx / HCX1, h = idx - r * HCX1;
         const int xx = h < 2 * R - 1 ? i0 - (2 * R - 1) + h : i0 + kTX + (h - (2 * R - 1));
-        const int zz = j0 - (R - 1) + r;
-        dmx[k] = (idx < HS1X * HCX1 && ingrid(xx, zz)) ? bload(r_dm, po(xx, zz)) : 0.f;
+        dmx[k] = G.coef(a.o_dtmu, xx, j0 - (R - 1) + r);
     }
 #pragma unroll
     for (int k = 0; k < NDZ; ++k) {
-        const int idx = tid + 256 * k;
+        const int idx = min(tid + 256 * k, HS1Z * HCZ1 - 1);
         const int r = idx / HCZ1, h = idx - r * HCZ1;
         const int xx = h < R - 1 ? i0 - (R - 1) + h : i0 + kTX + (h - (R - 1));
-        const int zz = j0 - (2 * R - 1) + r;
-        dmz[k] = (idx < HS1Z * HCZ1 && ingrid(xx, zz)) ? bload(r_dm, po(xx, zz)) : 0.f;
+        dmz[k] = G.coef(a.o_dtmu, xx, j0 - (2 * R - 1) + r);
     }
 
-    const float axl = xg ? a.ax[x] : 0.f;
+    const float fxl = frame(x, R, nx - R - 1);
+    const float axl = a.axp[x + kGhostX];
     float un1[NM1];  // displacement after step n at the tile's own rows
 #pragma unroll
     for (int k = 0; k < NM1; ++k) {
@@ -507,45 +475,38 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
         un1[k] = 0.f;
         if (r >= H1) break;
         const int z = j0 - L1 + r, c = lane + L1;
-        float vn = 0.f;
-        if (xg && z >= 0 && z < nz) {
-            const float ds = divs(x, z, r, c, a.ps_amp, a.ps_const) + (syn1 ? msy[k] : 0.f);
-            vn = (mv[k] + mb[k] * ds) * (axl * a.az[z]);
-            if (r >= L1 && r < L1 + TZ) {  // own node: outputs of step n
-                const float un = HAS_U ? mu[k] + vn * a.dt : 0.f;
-                un1[k] = un;
-                const long cc = (long)z * nx + x;
-                float out = vn;
-                if (HAS_U) {
-                    if (a.snap) a.snap[cc] = mu[k];
-                    out = un;
-                } else if (a.snap) {
-                    a.snap[cc] = mv[k];
-                }
-                if (a.dsamp) a.dsamp[cc] = out;
-                gather(x, z, out, a.seis_col);
+        const float ds = divs(x, z, r, c, fxl, frame(z, R, nz - R - 1), a.ps_amp, a.ps_const) + (syn1 ? msy[k] : 0.f);
+        const float vn = (mv[k] + mb[k] * ds) * (axl * a.azp[z + kGhost]);
+        if (r >= L1 && r < L1 + TZ) {  // own node: outputs of step n
+            const float un = HAS_U ? mu[k] + vn * a.dt : 0.f;
+            un1[k] = un;
+            const unsigned co = G.cofs(x, z);
+            float out = vn;
+            if (HAS_U) {
+                if (a.snap) bstore(rsrc(a.snap, cb), co, mu[k]);
+                out = un;
+            } else if (a.snap) {
+                bstore(rsrc(a.snap, cb), co, mv[k]);
             }
+            if (a.dsamp) bstore(rsrc(a.dsamp, cb), co, out);
+            gather(x, z, out, a.seis_col);
         }
         VV[r * W1 + c] = vn;
     }
 #pragma unroll
     for (int k = 0; k < NH1; ++k) {
-        const int idx = tid + 256 * k;
-        if (idx >= H1 * HC1) break;
+        const int idx = min(tid + 256 * k, H1 * HC1 - 1);
         const int r = idx / HC1, h = idx - r * HC1;
         const int c = h < L1 ? h : h + kTX;
         const int xx = i0 - L1 + c, zz = j0 - L1 + r;
-        float vn = 0.f;
-        if (ingrid(xx, zz)) {
-            const float ds = divs(xx, zz, r, c, a.ps_amp, a.ps_const) + (syn1 ? hsy[k] : 0.f);
-            vn = (hv[k] + hb[k] * ds) * ht[k];
-        }
-        VV[r * W1 + c] = vn;
+        const float ds = divs(xx, zz, r, c, frame(xx, R, nx - R - 1), frame(zz, R, nz - R - 1), a.ps_amp, a.ps_const) +
+                         (syn1 ? hsy[k] : 0.f);
+        VV[tid + 256 * k < H1 * HC1 ? r * W1 + c : H1 * W1] = (hv[k] + hb[k] * ds) * ht[k];
     }
     __syncthreads();
 
     // ---------------- phase C: step-n sigma, in place, where step n+1 needs it
-    const bool xin = x >= R - 1 && x <= nx - R - 1;  // dx_v frame
+    const float fxs = frame(x, R - 1, nx - R - 1);  // dx_v frame
     auto dxv = [&](int rv, int c) {
         const float* vrow = VV + rv * W1 + c;
         float e = 0.f;
@@ -565,34 +526,29 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
         const int q = w + 4 * k;
         if (q >= HS1Z) break;
         const int z = j0 - (2 * R - 1) + q;
-        if (!xg || z < 0 || z >= nz) continue;
         const int rv = z - (j0 - L1), c = lane + L1;
-        if (z >= R - 1 && z <= nz - R - 1) SZ[(rv + R) * W1 + c] += dmm[k] * dzv(rv, c);  // dz_v frame
-        if (rv >= V2R0 && rv <= V2R1 && xin) SX[rv * WSX + c + R] += dmm[k] * dxv(rv, c);
+        SZ[(rv + R) * W1 + c] += dmm[k] * (dzv(rv, c) * frame(z, R - 1, nz - R - 1));
+        if (rv >= V2R0 && rv <= V2R1) SX[rv * WSX + c + R] += dmm[k] * (dxv(rv, c) * fxs);
     }
 #pragma unroll
     for (int k = 0; k < NDX; ++k) {
-        const int idx = tid + 256 * k;
-        if (idx >= HS1X * HCX1) break;
+        const int idx = min(tid + 256 * k, HS1X * HCX1 - 1);
         const int r = idx / HCX1, h = idx - r * HCX1;
         const int xx = h < 2 * R - 1 ? i0 - (2 * R - 1) + h : i0 + kTX + (h - (2 * R - 1));
         const int zz = j0 - (R - 1) + r;
-        if (ingrid(xx, zz) && xx >= R - 1 && xx <= nx - R - 1) {
-            const int rv = zz - (j0 - L1), cv = xx - (i0 - L1);
-            SX[rv * WSX + cv + R] += dmx[k] * dxv(rv, cv);
-        }
+        const int rv = zz - (j0 - L1), cv = xx - (i0 - L1);
+        const float upd = dmx[k] * (dxv(rv, cv) * frame(xx, R - 1, nx - R - 1));
+        if (tid + 256 * k < HS1X * HCX1) SX[rv * WSX + cv + R] += upd;
     }
 #pragma unroll
     for (int k = 0; k < NDZ; ++k) {
-        const int idx = tid + 256 * k;
-        if (idx >= HS1Z * HCZ1) break;
+        const int idx = min(tid + 256 * k, HS1Z * HCZ1 - 1);
         const int r = idx / HCZ1, h = idx - r * HCZ1;
         const int xx = h < R - 1 ? i0 - (R - 1) + h : i0 + kTX + (h - (R - 1));
         const int zz = j0 - (2 * R - 1) + r;
-        if (ingrid(xx, zz) && zz >= R - 1 && zz <= nz - R - 1) {
-            const int rv = zz - (j0 - L1), cv = xx - (i0 - L1);
-            SZ[(rv + R) * W1 + cv] += dmz[k] * dzv(rv, cv);
-        }
+        const int rv = zz - (j0 - L1), cv = xx - (i0 - L1);
+        const float upd = dmz[k] * (dzv(rv, cv) * frame(zz, R - 1, nz - R - 1));
+        if (tid + 256 * k < HS1Z * HCZ1) SZ[(rv + R) * W1 + cv] += upd;
     }
     __syncthreads();
 
@@ -603,39 +559,37 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
         if (r >= H1) break;
         if (r < V2R0 || r > V2R1) continue;
         const int z = j0 - L1 + r, c = lane + L1;
-        if (!xg || z < 0 || z >= nz) continue;  // stays 0
         const float v1 = VV[r * W1 + c];
-        const float ds = divs(x, z, r, c, pa.ps_amp2, pa.ps_const2) + (syn1 ? 0.f : msy[k]);
-        const float vn = (v1 + mb[k] * ds) * (axl * a.az[z]);
+        const float ds = divs(x, z, r, c, fxl, frame(z, R, nz - R - 1), pa.ps_amp2, pa.ps_const2) + (syn1 ? 0.f : msy[k]);
+        const float vn = (v1 + mb[k] * ds) * (axl * a.azp[z + kGhost]);
         if (r >= L1 && r < L1 + TZ) {  // own node: outputs of step n+1 and the new state
-            const long p = (long)z * pitch + x;
-            const long cc = (long)z * nx + x;
-            a.v_out[p] = vn;
+            G.put(a.o_v_out, x, z, vn);
+            const unsigned co = G.cofs(x, z);
             float out = vn;
             if (HAS_U) {
                 const float un = un1[k] + vn * a.dt;
-                a.u_out[p] = un;
-                if (pa.snap2) pa.snap2[cc] = un1[k];
+                G.put(a.o_u_out, x, z, un);
+                if (pa.snap2) bstore(rsrc(pa.snap2, cb), co, un1[k]);
                 out = un;
             } else if (pa.snap2) {
-                pa.snap2[cc] = v1;
+                bstore(rsrc(pa.snap2, cb), co, v1);
             }
-            if (pa.dsamp2) pa.dsamp2[cc] = out;
+            if (pa.dsamp2) bstore(rsrc(pa.dsamp2, cb), co, out);
             gather(x, z, out, pa.seis_col2);
         }
         VV[r * W1 + c] = vn;
     }
 #pragma unroll
     for (int k = 0; k < NH1; ++k) {
-        const int idx = tid + 256 * k;
-        if (idx >= H1 * HC1) break;
+        const int idx = min(tid + 256 * k, H1 * HC1 - 1);
         const int r = idx / HC1, h = idx - r * HC1;
         const int c = h < L1 ? h : h + kTX;
-        if (r < V2R0 || r > V2R1 || c < V2C0 || c > V2C1) continue;
+        const bool in2 = tid + 256 * k < H1 * HC1 && r >= V2R0 && r <= V2R1 && c >= V2C0 && c <= V2C1;
         const int xx = i0 - L1 + c, zz = j0 - L1 + r;
-        if (!ingrid(xx, zz)) continue;
-        const float ds = divs(xx, zz, r, c, pa.ps_amp2, pa.ps_const2) + (syn1 ? 0.f : hsy[k]);
-        VV[r * W1 + c] = (VV[r * W1 + c] + hb[k] * ds) * ht[k];
+        const float ds = divs(xx, zz, r, c, frame(xx, R, nx - R - 1), frame(zz, R, nz - R - 1), pa.ps_amp2,
+                              pa.ps_const2) + (syn1 ? 0.f : hsy[k]);
+        const float vn = (VV[r * W1 + c] + hb[k] * ds) * ht[k];
+        VV[in2 ? r * W1 + c : H1 * W1] = vn;
     }
     __syncthreads();
 
@@ -645,11 +599,10 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
         const int q = w + 4 * k;
         if (q >= HS1Z) break;
         const int z = j0 - (2 * R - 1) + q;
-        if (z < j0 || z >= j0 + TZ || !xg || z >= nz) continue;
+        if (z < j0 || z >= j0 + TZ) continue;
         const int rv = z - (j0 - L1), c = lane + L1;
-        const long p = (long)z * pitch + x;
-        if (xin) a.sxy_out[p] = SX[rv * WSX + c + R] + dmm[k] * dxv(rv, c);
-        if (z >= R - 1 && z <= nz - R - 1) a.szy_out[p] = SZ[(rv + R) * W1 + c] + dmm[k] * dzv(rv, c);
+        G.put(a.o_sxy_out, x, z, SX[rv * WSX + c + R] + dmm[k] * (dxv(rv, c) * fxs));
+        G.put(a.o_szy_out, x, z, SZ[(rv + R) * W1 + c] + dmm[k] * (dzv(rv, c) * frame(z, R - 1, nz - R - 1)));
     }
 }
 
