@@ -157,6 +157,20 @@ class Pipeline:
         return c.nx * c.nz * c.nt
 
 
+def measured_traffic(kernel, c):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    (profiles/step_kernel_traffic.md), when they were taken on this grid and order; else None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "step_kernel_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("grid") != [c.nx, c.nz] or t.get("order") != c.order:
+        return None
+    return t.get(kernel)
+
+
 def cpu_baseline(w, nt_sample, threads):
     """The FP64 oracle on the same chain at the full grid, truncated to nt_sample Green steps."""
     import oracle
@@ -280,8 +294,9 @@ def main():
     bytes_launch = alg_bytes(best, c.fw_nth) * c.nx * c.nz * steps_per_launch
     achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
     kname = "k_pair" if steps_per_launch > 1.5 else "k_tile"
+    kkey = f"{kname}<{c.order // 2},{0 if best == 'green' else 1}>"
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kkey, c),
             "kernel": f"{kname}<{c.order // 2},{0 if best == 'green' else 1}> ({best} run, "
                       f"{steps_per_launch:.2f} time steps per launch)",
             "avg_launch_us": round(launch_ms * 1e3, 2), "steps_per_launch": round(steps_per_launch, 3),

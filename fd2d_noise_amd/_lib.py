@@ -54,7 +54,7 @@ EXPORTS = (
     "fd2d_timing_enable", "fd2d_timing_read", "fd2d_set_steps_per_launch",
     "fd2d_make_adjoint_sources", "fd2d_make_adjoint_sources_device", "fd2d_dot_device", "fd2d_axpby_device",
     "fd2d_gaussblur2d_device", "fd2d_map_m_to_parameters_device", "fd2d_map_gradparameters_to_gradm_device",
-    "fd2d_regularization_device", "fd2d_general_source_device",
+    "fd2d_regularization_device", "fd2d_general_source_device", "fd2d_axpby_f32_device",
 )
 
 

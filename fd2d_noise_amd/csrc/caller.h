@@ -28,6 +28,8 @@ struct AdjArgs {
 void launch_adjoint_sources(const AdjArgs& a, hipStream_t s);
 // *out = sum x.*y (compensated FP64); part holds 2*nb doubles
 void launch_dot(const double* x, const double* y, int64_t n, double* part, int nb, double* out, hipStream_t s);
+// out = alpha*x + beta*y (FP32)
+void launch_axpby_f32(float alpha, const float* x, float beta, const float* y, float* out, int64_t n, hipStream_t s);
 // out = alpha*x + beta*y
 void launch_axpby(double alpha, const double* x, double beta, const double* y, double* out, int64_t n, hipStream_t s);
 // GY = gaussblur2d(Y, x1, x2, [s1 s2]) on an n1 x n2 block (leading dimensions ldy, ldg); tmp n1*n2

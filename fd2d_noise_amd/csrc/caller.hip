@@ -418,6 +418,20 @@ void launch_axpby(double alpha, const double* x, double beta, const double* y, d
     hipLaunchKernelGGL(k_axpby, dim3((unsigned)(nb < 65536 ? nb : 65536)), dim3(256), 0, s, alpha, x, beta, y, out, n);
 }
 
+__global__ void k_axpby_f32(float alpha, const float* __restrict__ x, float beta, const float* __restrict__ y,
+                            float* __restrict__ out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = alpha * x[i] + beta * y[i];
+}
+
+void launch_axpby_f32(float alpha, const float* x, float beta, const float* y, float* out, int64_t n, hipStream_t s)
+{
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_axpby_f32, dim3((unsigned)(nb < 65536 ? nb : 65536)), dim3(256), 0, s, alpha, x, beta, y, out,
+                       n);
+}
+
 // ---------------------------------------------------------------------------------------------
 // gaussblur2d.m:2-38: separable Gaussian blur truncated at sqrt(14) sigma, normalised by the
 // outer product of the row sums of the two 1-D weight matrices.

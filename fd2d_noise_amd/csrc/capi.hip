@@ -1351,6 +1351,18 @@ int fd2d_axpby_device(double alpha, const double* x, double beta, const double* 
     });
 }
 
+int fd2d_axpby_f32_device(float alpha, const float* x, float beta, const float* y, float* out, int64_t n, void* stream)
+{
+    return guarded([&] {
+        require_device();
+        if (n < 0 || (n > 0 && (!x || !y || !out))) raise(FD2D_EINVAL, "axpby_f32: null argument");
+        hipStream_t st = as_stream(stream);
+        if (n) launch_axpby_f32(alpha, x, beta, y, out, n, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
 int fd2d_gaussblur2d_device(const double* Y, int32_t n1, int32_t n2, int32_t ldy, const double* x1, const double* x2,
                             double sigma1, double sigma2, double* GY, int32_t ldg, void* stream)
 {

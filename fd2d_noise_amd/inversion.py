@@ -36,6 +36,7 @@ def _bind():
     if not getattr(lib, "_inversion_bound", False):
         lib.fd2d_dot_device.argtypes = [_dp, _dp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double), _dp]
         lib.fd2d_axpby_device.argtypes = [ctypes.c_double, _dp, ctypes.c_double, _dp, _dp, ctypes.c_int64, _dp]
+        lib.fd2d_axpby_f32_device.argtypes = [ctypes.c_float, _dp, ctypes.c_float, _dp, _dp, ctypes.c_int64, _dp]
         lib.fd2d_gaussblur2d_device.argtypes = [_dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _dp, _dp,
                                                 ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int32, _dp]
         d2 = ctypes.c_double * 2
@@ -46,7 +47,7 @@ def _bind():
         lib.fd2d_regularization_device.argtypes = [ctypes.POINTER(ctypes.c_double), _dp, _dp, _dp, ctypes.c_int64,
                                                    ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_double,
                                                    ctypes.c_double, _dp]
-        for f in ("fd2d_dot_device", "fd2d_axpby_device", "fd2d_gaussblur2d_device", "fd2d_map_m_to_parameters_device",
+        for f in ("fd2d_dot_device", "fd2d_axpby_device", "fd2d_axpby_f32_device", "fd2d_gaussblur2d_device", "fd2d_map_m_to_parameters_device",
                   "fd2d_map_gradparameters_to_gradm_device", "fd2d_regularization_device"):
             getattr(lib, f).restype = ctypes.c_int
         lib._inversion_bound = True
@@ -93,6 +94,16 @@ def axpby(alpha: float, x, beta: float, y, out=None):
     out = torch.empty_like(x) if out is None else _vec(out)
     _check(_bind().fd2d_axpby_device(float(alpha), x.data_ptr(), float(beta), y.data_ptr(), out.data_ptr(), x.numel(),
                                      _stream()))
+    return out
+
+
+def axpby_f32(alpha: float, x, beta: float, y, out=None):
+    """out = alpha*x + beta*y on contiguous FP32 CUDA tensors (out may alias x or y)."""
+    import torch
+    assert x.is_cuda and x.is_contiguous() and x.dtype == torch.float32 and y.shape == x.shape
+    out = torch.empty_like(x) if out is None else out
+    _check(_bind().fd2d_axpby_f32_device(float(alpha), x.data_ptr(), float(beta), y.data_ptr(), out.data_ptr(),
+                                         x.numel(), _stream()))
     return out
 
 
@@ -333,6 +344,152 @@ class NoiseProblem:
         g = map_gradparameters_to_gradm(grad, usr)
         j, g = regularization(j, g, m, usr)
         return j, g, c_all
+
+    def eval_hessian_vector_product(self, m, dm_parameters, name: str = ""):
+        """eval_hessian_vector_product_2.m:2-197: H_m for the model m and the parameter
+        perturbation dm_parameters (physical units, nx x nz x n_maps as an (n_maps, nz, nx) device
+        tensor, or only its source / structure maps for usr.type 'source' / 'structure'). Per
+        station: Green and Born Green (mode 5), correlation, Born correlation and the correlation
+        of the perturbed sources, first- and second-order adjoint sources, then the adjoint runs
+        YELLOW (mode 13), GRAY (11), BLACK (12, Born) and RED (10, Born). The Born inputs are the
+        saved wavefields of every step, so the configuration needs fw_nth = 1."""
+        from .measurement import make_adjoint_sources_device
+        torch, D, c, usr = self.torch, self.D, self.c, self.usr
+        if c.fw_nth != 1:
+            raise ValueError("the Hessian-vector product needs fw_nth = 1 (eval_hessian_vector_product_2.m:75,154)")
+        Pm = map_m_to_parameters(m, usr)
+        n_maps = Pm.shape[0]
+        mu, sd_ref = Pm[n_maps - 1], Pm[:n_maps - 1]
+        rho, spec = self.rho, self.spectrum
+        dmp = dm_parameters if torch.is_tensor(dm_parameters) else torch.as_tensor(
+            np.ascontiguousarray(np.asarray(dm_parameters, dtype=np.float64).reshape(-1, order="F")), device="cuda")
+        if dmp.numel() != Pm.numel():  # :19-31
+            dm = torch.zeros_like(Pm)
+            if usr.type == "source":
+                dm[:n_maps - 1].view(-1).copy_(dmp.reshape(-1))
+            elif usr.type == "structure":
+                dm[n_maps - 1].view(-1).copy_(dmp.reshape(-1))
+        else:
+            dm = dmp.reshape(Pm.shape).contiguous()
+        sd_pert = dm[:n_maps - 1]
+        dmu = dm[n_maps - 1]
+        has_dmu = bool(torch.count_nonzero(dmu).item())
+        has_dsd = bool(torch.count_nonzero(sd_pert).item())
+        f64, f32, cd = torch.float64, torch.float32, torch.complex128
+        nr, N = self.n_rec, c.n_corr
+        w = float(usr.kernel_weighting)
+        nbk = max(c.n_basis_fct, 1) + 1
+        H = torch.zeros((nbk, c.nz, c.nx), dtype=f64, device="cuda")
+        K = torch.zeros_like(H)
+
+        def green(mode, dmu_, G_in):
+            G_fft = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
+            G_out = D.empty_like_dims((c.nx, c.nz, c.n_fw_green()), f32)
+            D.run_forward1_green(mu, rho, src, None, mode, dmu_, G_in, config=c, G_fft=G_fft, G_out=G_out)
+            return G_fft, G_out
+
+        def corr(G_fft, sd, dmu_, C_in):
+            s_ = D.empty_like_dims((nr, N), f64)
+            C_ = D.empty_like_dims((c.nx, c.nz, c.n_fw_corr()), f32)
+            D.run_forward2_correlation(mu, rho, G_fft, spec, sd, rec, 1, dmu_, C_in, config=c, seismograms=s_,
+                                       C_out=C_)
+            return s_, C_
+
+        def adjoint(u_fwd, stf, sd, G_fft, mode, dmu_, u_in):
+            K.zero_()
+            sf = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd) if mode in (12, 13) else None
+            uo = D.empty_like_dims((c.nx, c.nz, c.n_fw_corr()), f32) if mode in (11, 13) else None
+            D.run_noise_adjoint(mu, rho, u_fwd, stf, rec, None, spec, sd, G_fft, mode, dmu_, u_in, config=c, K=K,
+                                stf_fft=sf, u_out=uo)
+            return sf, uo
+
+        def add(Kpart=None):
+            # H += K, with the kernel maps of the other parameter class zeroed (:120, :157-159)
+            if Kpart == "structure":
+                K[:nbk - 1].zero_()
+            elif Kpart == "source":
+                K[nbk - 1].zero_()
+            axpby(1.0, H.view(-1), 1.0, K.view(-1), out=H.view(-1))
+
+        for i in range(self.ref.shape[0]):
+            src = D.to_device(self.ref[i:i + 1])
+            rec = D.to_device(self.recs[i])
+            G_fft, G = green(1, None, None)  # :57
+            dG_fft = dG = None
+            if has_dmu:
+                dG_fft, dG = green(5, dmu, G)  # :61
+            c_ref, C = corr(G_fft, sd_ref, None, None)  # :71
+            c_pert = torch.zeros_like(c_ref)
+            dC = None
+            if has_dmu:  # :75
+                c1, dC = corr(dG_fft, sd_ref, dmu, C)
+                axpby(1.0, c_pert.view(-1), 1.0, c1.view(-1), out=c_pert.view(-1))
+            if has_dsd:  # :84
+                c2, dC2 = corr(G_fft, sd_pert, None, None)
+                axpby(1.0, c_pert.view(-1), 1.0, c2.view(-1), out=c_pert.view(-1))
+                dC = dC2 if dC is None else axpby_f32(1.0, dC, 1.0, dC2)  # single arrays (:91)
+            stf = {}
+            for order in ("1st", "2nd"):  # :99-105
+                _, a_s = make_adjoint_sources_device(c_ref, self.c_data[i], c_pert, self.t, usr.veldis,
+                                                     usr.measurement_source, order, usr.window)
+                _, a_t = make_adjoint_sources_device(c_ref, self.c_data[i], c_pert, self.t, usr.veldis,
+                                                     usr.measurement_structure, order, usr.window)
+                st = torch.zeros((N, nr), dtype=cd, device="cuda")
+                torch.view_as_real(st)[..., 0].copy_(axpby(1.0 - w, a_s, w, a_t))
+                stf[order] = st
+            u_dag = uT = p = None
+            if usr.type != "source":  # YELLOW :113-121
+                uT, u_dag = adjoint(dC, stf["1st"], None, dG_fft, 13, None, None)
+                add("structure" if usr.type == "structure" else None)
+            if usr.type != "source" and has_dmu:  # GRAY :129-133
+                _, p = adjoint(dG, uT, sd_ref, None, 11, None, None)
+                add()
+            if usr.type == "joint" and has_dsd:  # :141-146
+                _, pa = adjoint(G, uT, sd_pert, None, 11, None, None)
+                add()
+                p = pa if p is None else axpby_f32(1.0, p, 1.0, pa)
+            # BLACK :154-160
+            kT, _ = adjoint(C, stf["2nd"], None, G_fft, 12, dmu if u_dag is not None else None, u_dag)
+            add("source" if usr.type == "source" else ("structure" if usr.type == "structure" else None))
+            if usr.type != "source":  # RED :168-170
+                adjoint(G, kT, sd_ref, None, 10, dmu if p is not None else None, p)
+                add()
+        Hm = map_gradparameters_to_gradm(H, usr)  # map_Hparameters_to_Hm.m:19-38
+        # :191-196: alpha / beta * weighting on the interior (the reference adds them without dm)
+        if usr.reg_alpha or usr.reg_beta:
+            add_v = self._hvp_reg_term(Hm.numel())
+            axpby(1.0, Hm, 1.0, add_v, out=Hm)
+        return Hm
+
+    def _hvp_reg_term(self, n):
+        """alpha * weighting (source maps) and beta * weighting (structure map) on the nodes where
+        init_absbound == 1, as FP64 device vector (eval_hessian_vector_product_2.m:191-196)."""
+        import torch
+        c, usr = self.c, self.usr
+        ax, az = np.ones(c.nx), np.ones(c.nz)
+        w_ = c.absorb_width
+
+        def lo(X):
+            return (X > w_) + np.exp(-(X - w_) ** 2 / (2 * w_) ** 2) * (X <= w_)
+
+        def hi(X, L_):
+            return (X < (L_ - w_)) + np.exp(-(X - (L_ - w_)) ** 2 / (2 * w_) ** 2) * (X >= (L_ - w_))
+        X, Z = c.x(), c.z()
+        if c.absorb_left == 1:
+            ax = ax * lo(X)
+        if c.absorb_right == 1:
+            ax = ax * hi(X, c.Lx)
+        if c.absorb_bottom == 1:
+            az = az * lo(Z)
+        if c.absorb_top == 1:
+            az = az * hi(Z, c.Lz)
+        inner = (np.outer(ax, az) == 1.0).astype(np.float64).ravel(order="F")
+        ns = c.nx * c.nz
+        wv = np.atleast_1d(np.asarray(usr.reg_weighting, dtype=np.float64)).ravel(order="F")
+        v = np.zeros(n)
+        v[:n - ns] = np.tile(inner, (n - ns) // ns) * usr.reg_alpha * np.tile(wv, (n - ns) // wv.size)
+        v[n - ns:] = inner * usr.reg_beta * (wv if wv.size == ns else wv[0])
+        return torch.as_tensor(v, device="cuda")
 
     def eval_objective(self, m, name: str = ""):
         j, _, c = self.eval_objective_and_gradient(m, name, only_objective=True)

@@ -228,6 +228,10 @@ int fd2d_dot_device(const double* x, const double* y, int64_t n, double* result,
 /* out = alpha*x + beta*y on the device (the model and direction updates of the same files). */
 int fd2d_axpby_device(double alpha, const double* x, double beta, const double* y, double* out, int64_t n,
                       void* stream);
+/* out = alpha*x + beta*y on FP32 device arrays (the single-precision wavefield sums of
+ * inversion/optlib_callback_fct/eval_hessian_vector_product_2.m:91,143-144). */
+int fd2d_axpby_f32_device(float alpha, const float* x, float beta, const float* y, float* out, int64_t n,
+                          void* stream);
 /* GY = gaussblur2d(Y, x1, x2, [sigma1 sigma2]) (tools/addons/gaussblur2d.m:2-38) on an n1 x n2
  * block of device arrays with leading dimensions ldy / ldg; x1, x2 device coordinate vectors. */
 int fd2d_gaussblur2d_device(const double* Y, int32_t n1, int32_t n2, int32_t ldy, const double* x1,

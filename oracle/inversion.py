@@ -386,3 +386,95 @@ def general_source(noise_spectrum, noise_source_distribution, n_basis_fct):
             for ns in range(n_src):
                 out[:, :, ib - 1] += spec[k - 1, ns] * dist[:, :, ns] / ni
     return out
+
+
+def eval_hessian_vector_product(m, dm_parameters, usr, solvers):
+    """eval_hessian_vector_product_2.m:2-197 on the oracle solvers: H_m for the model m and the
+    parameter perturbation dm_parameters (nx x nz x n_maps, or only its source / structure maps
+    for usr.type 'source' / 'structure', :19-31). Needs fw_nth = 1 (the Born inputs are the saved
+    wavefields of every step)."""
+    c = usr.config
+    P = map_m_to_parameters(m, c, usr.sigma_source, usr.sigma_structure, usr.ref_source, usr.ref_structure,
+                            usr.mu_0)
+    mu = np.asfortranarray(P[:, :, -1])
+    rho = usr.rho
+    sd_ref = np.asfortranarray(P[:, :, :-1])
+    spectrum = usr.spectrum
+    dmp = np.asarray(dm_parameters, dtype=np.float64)
+    if dmp.size != P.size:  # :19-31
+        dm = np.zeros_like(P)
+        if usr.type == "source":
+            dm[:, :, :-1] = dmp.reshape(P[:, :, :-1].shape, order="F")
+        elif usr.type == "structure":
+            dm[:, :, -1] = dmp.reshape(P.shape[:2], order="F")
+    else:
+        dm = dmp.reshape(P.shape, order="F")
+    sd_pert = np.asfortranarray(dm[:, :, :-1])
+    dmu = np.asfortranarray(dm[:, :, -1])
+    has_dmu = np.any(dmu != 0)
+    has_dsd = np.any(sd_pert != 0)
+    t = usr.t
+    ref = np.atleast_2d(usr.ref_stat)
+    n_rec = usr.array.shape[0] - 1
+    w = usr.kernel_weighting
+    H = np.zeros_like(P)
+    for i in range(ref.shape[0]):
+        src = ref[i:i + 1]
+        rec = receivers_of(usr.array, ref[i])
+        c_data = usr.c_data[i * n_rec:(i + 1) * n_rec, :]
+        G_fft, G, _ = solvers.run_forward1_green(mu, rho, src, None, 1, None, None, config=c)  # :57
+        dG_fft = dG = None
+        if has_dmu:  # :60-65
+            dG_fft, dG, _ = solvers.run_forward1_green(mu, rho, src, None, 5, dmu, G, config=c)
+        c_ref, C = solvers.run_forward2_correlation(mu, rho, G_fft, spectrum, sd_ref, rec, 1, None, None, config=c)
+        c_pert = np.zeros_like(c_ref)
+        dC = None
+        if has_dmu:  # :74-80
+            c1, dC = solvers.run_forward2_correlation(mu, rho, dG_fft, spectrum, sd_ref, rec, 1, dmu, C, config=c)
+            c_pert = c_pert + c1
+        if has_dsd:  # :83-89
+            c2, dC2 = solvers.run_forward2_correlation(mu, rho, G_fft, spectrum, sd_pert, rec, 1, None, None, config=c)
+            c_pert = c_pert + c2
+            dC = dC2 if dC is None else dC + dC2
+        a = {}
+        for order in ("1st", "2nd"):  # :99-105
+            _, a_s = measurement.make_adjoint_sources(c_ref, c_data, c_pert, t, usr.veldis, usr.measurement_source,
+                                                      order, usr.window)
+            _, a_t = measurement.make_adjoint_sources(c_ref, c_data, c_pert, t, usr.veldis,
+                                                      usr.measurement_structure, order, usr.window)
+            a[order] = np.asfortranarray(((1.0 - w) * a_s + w * a_t).astype(np.complex128))
+        K1 = K2 = K4 = 0.0
+        u_dag = uT = p = None
+        if usr.type != "source":  # YELLOW :113-121
+            K1, uT, u_dag, _ = solvers.run_noise_adjoint(mu, rho, dC, a["1st"], rec, None, spectrum, None, dG_fft, 13,
+                                                         None, None, config=c)
+            if usr.type == "structure":
+                K1[:, :, :-1] = 0.0
+        if usr.type != "source" and has_dmu:  # GRAY :129-133
+            K2, _, p, _ = solvers.run_noise_adjoint(mu, rho, dG, uT, rec, None, spectrum, sd_ref, None, 11, None, None,
+                                                    config=c)
+        if usr.type == "joint" and has_dsd:  # :141-146
+            K2a, _, pa, _ = solvers.run_noise_adjoint(mu, rho, G, uT, rec, None, spectrum, sd_pert, None, 11, None, None,
+                                                      config=c)
+            K2 = K2 + K2a
+            p = pa if p is None else p + pa
+        # BLACK :154-160
+        K3, kT, _, _ = solvers.run_noise_adjoint(mu, rho, C, a["2nd"], rec, None, spectrum, None, G_fft, 12,
+                                                 dmu if u_dag is not None else None, u_dag, config=c)
+        if usr.type == "source":
+            K3[:, :, -1] = 0.0
+        elif usr.type == "structure":
+            K3[:, :, :-1] = 0.0
+        if usr.type != "source":  # RED :168-170
+            K4, _, _, _ = solvers.run_noise_adjoint(mu, rho, G, kT, rec, None, spectrum, sd_ref, None, 10,
+                                                    dmu if p is not None else None, p, config=c)
+        H = H + K1 + K2 + K3 + K4
+    Hm = map_gradparameters_to_gradm(H, c, usr.sigma_source, usr.sigma_structure, usr.mu_0)  # :187
+    # :191-196: the regularisation term adds alpha / beta * weighting on the interior (not times dm)
+    inner = (primitives.absbound(c).ravel(order="F") == 1.0).astype(np.float64)
+    ns = c.nx * c.nz
+    wv = np.atleast_1d(np.asarray(usr.reg_weighting, dtype=np.float64))
+    w_src = np.tile(wv, (Hm.size - ns) // wv.size)
+    Hm[:-ns] = Hm[:-ns] + np.tile(inner, (Hm.size - ns) // ns) * usr.reg_alpha * w_src
+    Hm[-ns:] = Hm[-ns:] + inner * usr.reg_beta * (wv if wv.size == ns else wv[0])
+    return Hm

@@ -241,3 +241,63 @@ def test_objective_gradient_finite_differences(orc, typ):
             assert g @ dm == 0.0  # that kernel is not requested by the type
         else:
             assert g @ dm == pytest.approx(fd, rel=1e-4 if part == "source" else 2e-3)
+
+
+@pytest.mark.parametrize("typ,measurement", [("source", "waveform_difference"), ("structure", "waveform_difference"),
+                                             ("joint", "waveform_difference"),
+                                             ("structure", "amplitude_difference")])
+def test_hessian_vector_product_finite_differences(orc, typ, measurement):
+    """eval_hessian_vector_product_2: H dm against central differences of the gradient along dm
+    (the optlib_check_hessian.m test). dm lies in parameter space (d mu in Pa); with the identity
+    blur the model-space direction is (d source, d mu / mu_0). For the source problem the misfit is
+    quadratic in the source maps and the product is exact; the structure terms carry the
+    inexactness of the discrete adjoint."""
+    usr, m_true, blob = _noise_usr(typ)
+    usr.measurement_source = usr.measurement_structure = measurement
+    usr.window = (2.0, 8.0)
+    c = usr.config
+    P = c.nx * c.nz
+    _, _, usr.c_data = I.eval_objective_and_gradient(m_true, usr, orc, only_objective=True)
+    m = np.concatenate([np.ones(P), np.zeros(P)])
+    rng = np.random.default_rng(1)
+    dm = np.zeros(2 * P)
+    if typ != "structure":
+        dm[:P] = (blob * (0.5 + rng.random((c.nx, c.nz)))).ravel(order="F")
+    if typ != "source":
+        dm[P:] = 0.03 * (blob * (0.5 + rng.random((c.nx, c.nz)))).ravel(order="F")
+    dmp = dm.copy()
+    dmp[P:] *= usr.mu_0
+    Hm = I.eval_hessian_vector_product(m, dmp, usr, orc)
+    h = 1e-3
+    _, gp, _ = I.eval_objective_and_gradient(m + h * dm, usr, orc)
+    _, gm, _ = I.eval_objective_and_gradient(m - h * dm, usr, orc)
+    fd = (gp - gm) / (2 * h)
+    tol = {"source": (1e-9, 0), "structure": (0, 2e-3), "joint": (1e-6, 1e-2)}[typ]
+    for sl, t in ((slice(0, P), tol[0]), (slice(P, 2 * P), tol[1])):
+        if t == 0:
+            assert np.all(Hm[sl] == 0) and np.all(fd[sl] == 0)
+        else:
+            assert np.linalg.norm(Hm[sl] - fd[sl]) <= t * np.linalg.norm(fd[sl])
+
+
+def test_hessian_regularisation_term(orc):
+    """eval_hessian_vector_product_2.m:191-196 adds alpha / beta * weighting on the interior nodes,
+    independent of dm (pinned as the reference states it)."""
+    usr, m_true, _ = _noise_usr("source")
+    c = usr.config
+    P = c.nx * c.nz
+    _, _, usr.c_data = I.eval_objective_and_gradient(m_true, usr, orc, only_objective=True)
+    m = np.concatenate([np.ones(P), np.zeros(P)])
+    dm = np.zeros(2 * P)
+    dm[:P] = 1e-3
+    H0 = I.eval_hessian_vector_product(m, dm, usr, orc)
+    usr.reg_alpha, usr.reg_beta = 0.25, 0.5
+    H1 = I.eval_hessian_vector_product(m, dm, usr, orc)
+    inner = (orc_absbound(c).ravel(order="F") == 1.0)
+    np.testing.assert_allclose((H1 - H0)[:P], 0.25 * inner, atol=1e-15)
+    np.testing.assert_allclose((H1 - H0)[P:], 0.5 * inner, atol=1e-15)
+
+
+def orc_absbound(c):
+    from oracle import primitives
+    return primitives.absbound(c)
