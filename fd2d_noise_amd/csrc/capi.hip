@@ -653,317 +653,403 @@ int64_t green_n_fw(const Geometry& g)
 }
 
 // =============================================================================================
-// run_forward1_green (code/run_forward1_green.m)
+// Solves as step sequences. A Job holds one solve's set-up (the prologue of the *_device entry
+// points), fills the per-step fields of step n, does the work after a launch and finishes the
+// solve. run_job() advances one job alone (one or two steps per launch); the batch engine below
+// advances many jobs with grouped launches.
 // =============================================================================================
-void green_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* src,
-                  const fd2d_carray* rec, double mode, const fd2d_carray* dmu, const fd2d_carray* G_in,
-                  fd2d_marray* G_fft, fd2d_marray* G_out, fd2d_marray* seis, hipStream_t st)
-{
-    const Geometry g = geometry(cfg);
-    const int64_t P = (int64_t)g.P;
-    if (numel(mu) != P || numel(rho) != P) raise(FD2D_EINVAL, "mu and rho must have nx*nz = %lld elements", (long long)P);
-    const bool born = numel(dmu) != 0;
-    if (born && numel(dmu) != P) raise(FD2D_EINVAL, "dmu must have nx*nz elements");
-    if (born && (numel(G_in) == 0 || G_in->dims[0] * G_in->dims[1] != P || dim(G_in, 2) < g.nt))
-        raise(FD2D_EINVAL, "Born mode needs G_in with nt slices of nx x nz (run_forward1_green.m:141)");
-    const std::vector<int2> sid = node_ids(src, g, "src");
-    const std::vector<int2> rid = node_ids(rec, g, "rec");
-    const int64_t nr = (int64_t)rid.size();
-    const int64_t n_fw = green_n_fw(g);
-    check_out(G_fft, g.nx, g.nz, g.K, "G_fft");
-    if (mode != 0) check_out(G_out, g.nx, g.nz, n_fw, "G_out");
-    check_out(seis, nr, g.nt, 1, "seismograms");
-    double2* Gf = (G_fft && G_fft->data && g.K > 0) ? (double2*)G_fft->data : nullptr;
-    float* Gout = (mode != 0 && G_out && G_out->data) ? (float*)G_out->data : nullptr;
-    double* sg = (seis && seis->data) ? (double*)seis->data : nullptr;
-
-    Solver S(cfg, g, st, false);
-    S.material((const double*)mu->data, (const double*)rho->data);
-    S.set_sources(sid);
-    S.set_receivers(rid);
-    DBuf<float> dtdmu;
-    if (born) {
-        dtdmu.alloc(g.P);
-        launch_scale_to_float((const double*)dmu->data, dtdmu.p, g.P, g.dt, st);
-    }
-    if (sg && nr) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * g.nt, st));
-
-    // sampled steps of :176 and their times t(n) = (n-1) dt (fft_coeff, :81-90)
-    std::vector<double> ts;
-    for (int n = 1; n <= g.nt; ++n)
-        if ((n + g.nt - 1) % g.s == 0) ts.push_back((n - 1) * g.dt);
-    DftAccumulator dft;
-    if (Gf) dft.init(Gf, g.P, g.K, phase_table(cfg, ts, -1.0, 1.0), ts.size(), st);
-
-    StepTimer timer(FD2D_GREEN, g.P, st);
-    int64_t i_fw_out = 0;
-    auto sampled = [&](int n) { return Gf != nullptr && (n + g.nt - 1) % g.s == 0; };
-    // per-step fields of step n as step `sub` of the launch
-    auto plan = [&](int n, int sub) {
-        const Solver::Slots sl = S.slots(sub);
-        if (mode != 0 && (n + g.nt - 1) % g.f == 0) {
-            if (Gout) *sl.snap = Gout + (size_t)i_fw_out * g.P;
-            ++i_fw_out;
+struct Job {
+    const fd2d_config* cfg = nullptr;
+    Geometry g{};
+    hipStream_t st = nullptr;
+    std::unique_ptr<Solver> S;
+    int solver = 0, nsteps = 0, next = 1;
+    bool born = false;      // Born terms need one launch per step and their own set-up per step
+    bool records = false;   // writes seismograms (receiver lists are used)
+    virtual ~Job() = default;
+    virtual void plan(int n, int sub) = 0;  // fields of step n as step `sub` of the launch
+    virtual bool pairable(int n) = 0;       // may steps n, n+1 share a two-step launch
+    virtual void after(int n, int k) = 0;   // after the launch that advanced steps n .. n+k-1
+    virtual void finish() = 0;              // after the last step
+    // StepArgs with the fields that do not change from step to step, for ping-pong direction `parity`
+    StepArgs static_args(int parity)
+    {
+        const int keep = S->cur;
+        S->cur = parity;
+        S->clear_step();
+        StepArgs s = S->a;
+        S->cur = keep;
+        S->clear_step();
+        if (S->ps_ent_d.n) {
+            s.ps_off = S->ps_off_d.p;
+            s.ps_ent = S->ps_ent_d.p;
         }
-        if (born) S.born_green(dtdmu.p, (const float*)G_in->data + (size_t)(n - 1) * g.P);
-        else if (!sid.empty()) S.point_sources(sub, nullptr, 1.0e9f);  // stf = 1e9 every step (:64)
-        if (sampled(n)) *sl.dsamp = dft.next_slot();
-        if (sg && nr) S.receivers(sub, sg + (size_t)nr * (n - 1));
-    };
-    const bool pairs = g_steps_per_launch.load() == 2 && !born;
-    for (int n = 1; n <= g.nt;) {
-        const int k = (pairs && n + 1 <= g.nt && !(sampled(n) && sampled(n + 1))) ? 2 : 1;
-        S.clear_step();
-        for (int q = 0; q < k; ++q) plan(n + q, q);
+        if (records && S->rec_ent_d.n) {
+            s.rec_off = S->rec_off_d.p;
+            s.rec_ent = S->rec_ent_d.p;
+        }
+        return s;
+    }
+};
+
+void run_job(Job& j)
+{
+    StepTimer timer(j.solver, j.g.P, j.st);
+    const bool pairs = g_steps_per_launch.load() == 2 && !j.born;
+    for (int n = 1; n <= j.nsteps;) {
+        const int k = (pairs && n + 1 <= j.nsteps && j.pairable(n)) ? 2 : 1;
+        j.S->clear_step();
+        for (int q = 0; q < k; ++q) j.plan(n + q, q);
         timer.begin(n, k);
-        if (k == 2) S.step_pair();
-        else S.step();
+        if (k == 2) j.S->step_pair();
+        else j.S->step();
         timer.end(n, k);
-        for (int q = 0; q < k; ++q)
-            if (sampled(n + q)) dft.commit();
+        j.after(n, k);
         n += k;
     }
-    if (Gf) {
-        dft.flush();
-        // G_fft = conj(sum v * (2pi)^-1/2 e^{-iwt} dt)  (:178-180, :196-198)
-        launch_finalize_dft(Gf, Gf, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 1, st);
-    }
+    j.finish();
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipStreamSynchronize(j.st));
     timer.collect();
 }
 
 // =============================================================================================
+// run_forward1_green (code/run_forward1_green.m)
+// =============================================================================================
+struct GreenJob : Job {
+    double mode;
+    const fd2d_carray* G_in;
+    std::vector<int2> sid, rid;
+    int64_t nr = 0, i_fw_out = 0;
+    double2* Gf = nullptr;
+    float* Gout = nullptr;
+    double* sg = nullptr;
+    DBuf<float> dtdmu;
+    DftAccumulator dft;
+
+    GreenJob(const fd2d_config* c, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* src,
+             const fd2d_carray* rec, double mode_, const fd2d_carray* dmu, const fd2d_carray* G_in_,
+             fd2d_marray* G_fft, fd2d_marray* G_out, fd2d_marray* seis, hipStream_t s)
+        : mode(mode_), G_in(G_in_)
+    {
+        cfg = c;
+        st = s;
+        solver = FD2D_GREEN;
+        g = geometry(cfg);
+        const int64_t P = (int64_t)g.P;
+        if (numel(mu) != P || numel(rho) != P)
+            raise(FD2D_EINVAL, "mu and rho must have nx*nz = %lld elements", (long long)P);
+        born = numel(dmu) != 0;
+        if (born && numel(dmu) != P) raise(FD2D_EINVAL, "dmu must have nx*nz elements");
+        if (born && (numel(G_in) == 0 || G_in->dims[0] * G_in->dims[1] != P || dim(G_in, 2) < g.nt))
+            raise(FD2D_EINVAL, "Born mode needs G_in with nt slices of nx x nz (run_forward1_green.m:141)");
+        sid = node_ids(src, g, "src");
+        rid = node_ids(rec, g, "rec");
+        nr = (int64_t)rid.size();
+        const int64_t n_fw = green_n_fw(g);
+        check_out(G_fft, g.nx, g.nz, g.K, "G_fft");
+        if (mode != 0) check_out(G_out, g.nx, g.nz, n_fw, "G_out");
+        check_out(seis, nr, g.nt, 1, "seismograms");
+        Gf = (G_fft && G_fft->data && g.K > 0) ? (double2*)G_fft->data : nullptr;
+        Gout = (mode != 0 && G_out && G_out->data) ? (float*)G_out->data : nullptr;
+        sg = (seis && seis->data) ? (double*)seis->data : nullptr;
+        records = sg && nr;
+        nsteps = g.nt;
+
+        S.reset(new Solver(cfg, g, st, false));
+        S->material((const double*)mu->data, (const double*)rho->data);
+        if (!born) S->set_sources(sid);
+        S->set_receivers(rid);
+        if (born) {
+            dtdmu.alloc(g.P);
+            launch_scale_to_float((const double*)dmu->data, dtdmu.p, g.P, g.dt, st);
+        }
+        if (records) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * g.nt, st));
+        // sampled steps of :176 and their times t(n) = (n-1) dt (fft_coeff, :81-90)
+        std::vector<double> ts;
+        for (int n = 1; n <= g.nt; ++n)
+            if ((n + g.nt - 1) % g.s == 0) ts.push_back((n - 1) * g.dt);
+        if (Gf) dft.init(Gf, g.P, g.K, phase_table(cfg, ts, -1.0, 1.0), ts.size(), st);
+    }
+    bool sampled(int n) const { return Gf != nullptr && (n + g.nt - 1) % g.s == 0; }
+    void plan(int n, int sub) override
+    {
+        const Solver::Slots sl = S->slots(sub);
+        if (mode != 0 && (n + g.nt - 1) % g.f == 0) {
+            if (Gout) *sl.snap = Gout + (size_t)i_fw_out * g.P;
+            ++i_fw_out;
+        }
+        if (born) S->born_green(dtdmu.p, (const float*)G_in->data + (size_t)(n - 1) * g.P);
+        else if (!sid.empty()) S->point_sources(sub, nullptr, 1.0e9f);  // stf = 1e9 every step (:64)
+        if (sampled(n)) *sl.dsamp = dft.next_slot();
+        if (records) S->receivers(sub, sg + (size_t)nr * (n - 1));
+    }
+    bool pairable(int n) override { return !(sampled(n) && sampled(n + 1)); }
+    void after(int n, int k) override
+    {
+        for (int q = 0; q < k; ++q)
+            if (sampled(n + q)) dft.commit();
+    }
+    void finish() override
+    {
+        if (Gf) {
+            dft.flush();
+            // G_fft = conj(sum v * (2pi)^-1/2 e^{-iwt} dt)  (:178-180, :196-198)
+            launch_finalize_dft(Gf, Gf, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 1, st);
+        }
+    }
+};
+
+// =============================================================================================
 // run_forward2_correlation (code/run_forward2_correlation.m)
 // =============================================================================================
-void correlation_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* G,
-                        const fd2d_carray* spectrum, const fd2d_carray* sd, const fd2d_carray* rec, double mode,
-                        const fd2d_carray* dmu, const fd2d_carray* C_in, fd2d_marray* seis, fd2d_marray* C_out,
-                        hipStream_t st)
-{
-    const Geometry g = geometry(cfg);
-    const int64_t P = (int64_t)g.P;
-    const int N = 2 * g.nt - 1;  // :68-69
-    const int nb = cfg->n_basis_fct, nb_cols = nb == 0 ? 1 : nb;
-    if (numel(mu) != P || numel(rho) != P) raise(FD2D_EINVAL, "mu and rho must have nx*nz = %lld elements", (long long)P);
-    if (numel(G) != P * g.K || g.K == 0) raise(FD2D_EINVAL, "G_fft must be nx x nz x n_sample");
-    const int64_t n_src = dim(spectrum, 1);
-    if (dim(spectrum, 0) != g.K || n_src < 1 || dim(spectrum, 2) != 1)
-        raise(FD2D_EINVAL, "spectrum must be n_sample x n_noise_sources");
-    if (nb == 0 && n_src != 1)
-        raise(FD2D_EINVAL, "with n_basis_fct = 0 the reference accepts one noise source (run_forward2_correlation.m:171)");
-    if (numel(sd) != P * (nb == 0 ? n_src : nb)) raise(FD2D_EINVAL, "source_dist must be nx x nz x %d", nb == 0 ? (int)n_src : nb);
-    const bool born = numel(dmu) != 0;
-    if (born && (numel(dmu) != P || numel(C_in) == 0 || C_in->dims[0] * C_in->dims[1] != P || dim(C_in, 2) < N))
-        raise(FD2D_EINVAL, "Born mode needs dmu (nx x nz) and C_in with 2nt-1 slices (run_forward2_correlation.m:236)");
-    const std::vector<int2> rid = node_ids(rec, g, "rec");
-    const int64_t nr = (int64_t)rid.size();
-    const int64_t n_fw = N / g.f;
-    if (mode != 0) check_out(C_out, g.nx, g.nz, n_fw, "C_out");
-    check_out(seis, nr, N, 1, "seismograms");
-    float* Cout = (mode != 0 && C_out && C_out->data) ? (float*)C_out->data : nullptr;
-    double* sg = (seis && seis->data) ? (double*)seis->data : nullptr;
-
-    Solver S(cfg, g, st, true);
-    S.material((const double*)mu->data, (const double*)rho->data);
-    S.set_receivers(rid);
-    if (sg && nr) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * N, st));
-
-    // source spectrum A = (sum_ns spectrum) * distribution .* G_fft (:168-173, :201-205)
-    const std::vector<double> spec = to_host<double>(spectrum->data, (size_t)g.K * n_src);
-    std::vector<double> spec_sum((size_t)g.K, 0.0);
-    for (int k = 0; k < g.K; ++k)
-        for (int64_t q = 0; q < n_src; ++q) spec_sum[k] += spec[(size_t)k + (size_t)g.K * q];
+struct CorrelationJob : Job {
+    double mode;
+    const fd2d_carray* C_in;
+    std::vector<int2> rid;
+    int64_t nr = 0, i_fw_out = 0;
+    int N = 0;
+    float* Cout = nullptr;
+    double* sg = nullptr;
     DBuf<double> I_d, ss_d;
-    upload(I_d, interpol_matrix(g.K, nb), st);
-    upload(ss_d, spec_sum, st);
-    DBuf<double2> A(g.P * (size_t)g.K);
-    launch_source_spectrum(A.p, (const double2*)G->data, 0, (const double*)sd->data, nb_cols, I_d.p, ss_d.p, g.P,
-                           g.K, st);
-    // ifft_coeff of :73-85 at t(n) = (n - nt) dt for n = s, 2s, ...
-    const int n_ftc = N / g.s;
-    std::vector<double> ts;
-    for (int q = 1; q <= n_ftc; ++q) ts.push_back((double)(q * g.s - g.nt) * g.dt);
+    DBuf<double2> A;
     Synthesiser syn;
-    syn.init(A.p, g.P, g.K, phase_table(cfg, ts, 1.0, kInvSqrt2Pi * g.dw), n_ftc, st);
-
     DBuf<float> dmu_f;
-    if (born) {
-        dmu_f.alloc(g.P);
-        launch_scale_to_float((const double*)dmu->data, dmu_f.p, g.P, 1.0, st);
+
+    CorrelationJob(const fd2d_config* c, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* G,
+                   const fd2d_carray* spectrum, const fd2d_carray* sd, const fd2d_carray* rec, double mode_,
+                   const fd2d_carray* dmu, const fd2d_carray* C_in_, fd2d_marray* seis, fd2d_marray* C_out,
+                   hipStream_t s)
+        : mode(mode_), C_in(C_in_)
+    {
+        cfg = c;
+        st = s;
+        solver = FD2D_CORRELATION;
+        g = geometry(cfg);
+        const int64_t P = (int64_t)g.P;
+        N = 2 * g.nt - 1;  // :68-69
+        const int nb = cfg->n_basis_fct, nb_cols = nb == 0 ? 1 : nb;
+        if (numel(mu) != P || numel(rho) != P)
+            raise(FD2D_EINVAL, "mu and rho must have nx*nz = %lld elements", (long long)P);
+        if (numel(G) != P * g.K || g.K == 0) raise(FD2D_EINVAL, "G_fft must be nx x nz x n_sample");
+        const int64_t n_src = dim(spectrum, 1);
+        if (dim(spectrum, 0) != g.K || n_src < 1 || dim(spectrum, 2) != 1)
+            raise(FD2D_EINVAL, "spectrum must be n_sample x n_noise_sources");
+        if (nb == 0 && n_src != 1)
+            raise(FD2D_EINVAL, "with n_basis_fct = 0 the reference accepts one noise source (run_forward2_correlation.m:171)");
+        if (numel(sd) != P * (nb == 0 ? n_src : nb))
+            raise(FD2D_EINVAL, "source_dist must be nx x nz x %d", nb == 0 ? (int)n_src : nb);
+        born = numel(dmu) != 0;
+        if (born && (numel(dmu) != P || numel(C_in) == 0 || C_in->dims[0] * C_in->dims[1] != P || dim(C_in, 2) < N))
+            raise(FD2D_EINVAL, "Born mode needs dmu (nx x nz) and C_in with 2nt-1 slices (run_forward2_correlation.m:236)");
+        rid = node_ids(rec, g, "rec");
+        nr = (int64_t)rid.size();
+        const int64_t n_fw = N / g.f;
+        if (mode != 0) check_out(C_out, g.nx, g.nz, n_fw, "C_out");
+        check_out(seis, nr, N, 1, "seismograms");
+        Cout = (mode != 0 && C_out && C_out->data) ? (float*)C_out->data : nullptr;
+        sg = (seis && seis->data) ? (double*)seis->data : nullptr;
+        records = sg && nr;
+        nsteps = N;
+
+        S.reset(new Solver(cfg, g, st, true));
+        S->material((const double*)mu->data, (const double*)rho->data);
+        S->set_receivers(rid);
+        if (records) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * N, st));
+        // source spectrum A = (sum_ns spectrum) * distribution .* G_fft (:168-173, :201-205)
+        const std::vector<double> spec = to_host<double>(spectrum->data, (size_t)g.K * n_src);
+        std::vector<double> spec_sum((size_t)g.K, 0.0);
+        for (int k = 0; k < g.K; ++k)
+            for (int64_t q = 0; q < n_src; ++q) spec_sum[k] += spec[(size_t)k + (size_t)g.K * q];
+        upload(I_d, interpol_matrix(g.K, nb), st);
+        upload(ss_d, spec_sum, st);
+        A.alloc(g.P * (size_t)g.K);
+        launch_source_spectrum(A.p, (const double2*)G->data, 0, (const double*)sd->data, nb_cols, I_d.p, ss_d.p, g.P,
+                               g.K, st);
+        // ifft_coeff of :73-85 at t(n) = (n - nt) dt for n = s, 2s, ...
+        const int n_ftc = N / g.s;
+        std::vector<double> ts;
+        for (int q = 1; q <= n_ftc; ++q) ts.push_back((double)(q * g.s - g.nt) * g.dt);
+        syn.init(A.p, g.P, g.K, phase_table(cfg, ts, 1.0, kInvSqrt2Pi * g.dw), n_ftc, st);
+        if (born) {
+            dmu_f.alloc(g.P);
+            launch_scale_to_float((const double*)dmu->data, dmu_f.p, g.P, 1.0, st);
+        }
     }
-    StepTimer timer(FD2D_CORRELATION, g.P, st);
-    int64_t i_fw_out = 0;
-    auto plan = [&](int n, int sub) {
-        const Solver::Slots sl = S.slots(sub);
+    void plan(int n, int sub) override
+    {
+        const Solver::Slots sl = S->slots(sub);
         if (mode != 0 && n % g.f == 0) {  // :182-186
             if (Cout) *sl.snap = Cout + (size_t)i_fw_out * g.P;
             ++i_fw_out;
         }
         if (n % g.s == 0) *sl.syn = syn.slice(n / g.s - 1);  // :195-216
         if (born && n != N) {                                 // :232-238
-            S.a.born_dmu = dmu_f.p;
-            S.a.born_in0 = (const float*)C_in->data + (size_t)(n - 1) * g.P;
-            S.a.born_in1 = (const float*)C_in->data + (size_t)n * g.P;
+            S->a.born_dmu = dmu_f.p;
+            S->a.born_in0 = (const float*)C_in->data + (size_t)(n - 1) * g.P;
+            S->a.born_in1 = (const float*)C_in->data + (size_t)n * g.P;
         }
-        if (sg && nr) S.receivers(sub, sg + (size_t)nr * (n - 1));
-    };
-    const bool pairs = g_steps_per_launch.load() == 2 && !born;
-    for (int n = 1; n <= N;) {
-        const int k = (pairs && n + 1 <= N && !(n % g.s == 0 && (n + 1) % g.s == 0)) ? 2 : 1;
-        S.clear_step();
-        for (int q = 0; q < k; ++q) plan(n + q, q);
-        timer.begin(n, k);
-        if (k == 2) S.step_pair();
-        else S.step();
-        timer.end(n, k);
-        n += k;
+        if (records) S->receivers(sub, sg + (size_t)nr * (n - 1));
     }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
-    timer.collect();
-}
+    bool pairable(int n) override { return !(n % g.s == 0 && (n + 1) % g.s == 0); }
+    void after(int, int) override {}
+    void finish() override {}
+};
 
 // =============================================================================================
 // run_noise_adjoint (code/run_noise_adjoint.m)
 // =============================================================================================
-void adjoint_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* u_fwd,
-                    const fd2d_carray* stf, const fd2d_carray* src, const fd2d_carray* rec,
-                    const fd2d_carray* spectrum, const fd2d_carray* sd, const fd2d_carray* G, double mode,
-                    const fd2d_carray* dmu, const fd2d_carray* u_in, fd2d_marray* K_out, fd2d_marray* stf_fft,
-                    fd2d_marray* u_out, fd2d_marray* seis, hipStream_t st)
-{
-    const Geometry g = geometry(cfg);
-    const int64_t P = (int64_t)g.P;
-    const int N = 2 * g.nt - 1;  // :47-49
-    const int nb = cfg->n_basis_fct, nb_cols = nb == 0 ? 1 : nb;
-    if (numel(mu) != P || numel(rho) != P) raise(FD2D_EINVAL, "mu and rho must have nx*nz = %lld elements", (long long)P);
-    const int imode = (int)mode;
-    if ((double)imode != mode || !(imode == 0 || imode == 1 || (imode >= 10 && imode <= 13)))
-        raise(FD2D_EINVAL, "mode must be 0, 1, 10, 11, 12 or 13 (run_noise_adjoint.m:16-22)");
-    const bool fft_mode = imode == 1 || imode == 12 || imode == 13;
-    const bool save_mode = imode == 11 || imode == 13;
-
-    const int64_t n_stf = numel(stf);
-    const bool stf_point = n_stf != 0 && stf->dims[2] == 1;  // :249
-    const bool stf_dist = n_stf != 0 && stf->dims[2] != 1;   // :255
-    if (stf_dist && fft_mode)
-        raise(FD2D_EUNSUPPORTED, "a distributed stf with mode %d advances the reference's DFT counter twice per sample "
-                                 "(run_noise_adjoint.m:265,343)", imode);
-    if (stf_dist && (stf->dims[0] * stf->dims[1] != P || stf->dims[2] != g.K))
-        raise(FD2D_EINVAL, "a distributed stf must be nx x nz x n_sample");
-    const int nsteps = ((imode == 0 || imode == 1) && stf_dist) ? g.nt : N;  // :165-167
-
-    const std::vector<int2> sid = node_ids(src, g, "src");
-    const std::vector<int2> rid = node_ids(rec, g, "rec");
-    const int64_t nr = (int64_t)rid.size();
-    if (stf_point && (stf->dims[0] < (int64_t)sid.size() || stf->dims[1] < nsteps))
-        raise(FD2D_EINVAL, "a point stf must be n_src x (2nt-1)");
-
-    const bool have_sd = numel(sd) != 0;  // :112-123
-    const int64_t n_src = have_sd ? dim(spectrum, 1) : 0;
+struct AdjointJob : Job {
+    int imode = 0, N = 0, nb_cols = 1;
+    const fd2d_carray *u_fwd, *dmu, *u_in, *G;
+    bool save_mode = false, stf_point = false, dist_src = false, src_kernel = false, need_dft = false;
+    std::vector<int2> sid, rid;
     std::vector<double> spec;
-    if (numel(spectrum)) spec = to_host<double>(spectrum->data, (size_t)numel(spectrum));
-    if (have_sd) {
-        if (dim(spectrum, 0) != g.K) raise(FD2D_EINVAL, "spectrum must have n_sample rows");
-        if (nb == 0 && n_src != 1) raise(FD2D_EINVAL, "with n_basis_fct = 0 the reference accepts one noise source");
-        if (numel(sd) != P * (nb == 0 ? n_src : nb)) raise(FD2D_EINVAL, "source_dist size");
-    }
-    const bool src_kernel = numel(G) != 0 && dim(G, 2) == g.K && G->dims[0] * G->dims[1] == P && g.K > 0;  // :302
-    if (src_kernel && (dim(spectrum, 0) != g.K || dim(spectrum, 1) != 1))
-        raise(FD2D_EINVAL, "the source kernel needs an n_sample x 1 spectrum (run_noise_adjoint.m:312)");
-    const int64_t n_ufwd = numel(u_fwd) ? dim(u_fwd, 2) : 0;  // :324
-    if (n_ufwd && u_fwd->dims[0] * u_fwd->dims[1] != P) raise(FD2D_EINVAL, "u_fwd slices must have nx*nz elements");
-    const int64_t n_uin = numel(u_in) ? dim(u_in, 2) : 0;
-    const bool born = numel(dmu) != 0 && n_uin == N;  // :285
-    if (born && (numel(dmu) != P || u_in->dims[0] * u_in->dims[1] != P)) raise(FD2D_EINVAL, "dmu / u_in sizes");
-
-    const int64_t n_fw = N / g.f;
-    check_out(K_out, g.nx, g.nz, nb_cols + 1, "K");
-    if (fft_mode) check_out(stf_fft, g.nx, g.nz, g.K, "stf_fft");
-    if (save_mode) check_out(u_out, g.nx, g.nz, n_fw, "u_out");
-    check_out(seis, nr, N, 1, "seismograms");
-    double* Kd = (K_out && K_out->data) ? (double*)K_out->data : nullptr;
-    double2* sfft = (fft_mode && stf_fft && stf_fft->data && g.K > 0) ? (double2*)stf_fft->data : nullptr;
-    float* uo = (save_mode && u_out && u_out->data) ? (float*)u_out->data : nullptr;
-    double* sg = (seis && seis->data) ? (double*)seis->data : nullptr;
-
-    Solver S(cfg, g, st, true);
-    S.material((const double*)mu->data, (const double*)rho->data);
-    if (stf_point) S.set_sources(sid);
-    S.set_receivers(rid);
-    if (sg && nr) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * N, st));
-    if (Kd) HIPCHK(hipMemsetAsync(Kd, 0, sizeof(double) * g.P * (size_t)(nb_cols + 1), st));
-
-    // point adjoint sources: real part of stf(i, n) (:249-253)
+    int64_t nr = 0, n_ufwd = 0, i_fw_in = 0, i_fw_out = 0;
+    double* Kd = nullptr;
+    double2* sfft = nullptr;
+    float* uo = nullptr;
+    double* sg = nullptr;
     DBuf<float> amp;
-    if (stf_point && !sid.empty()) {
-        amp.alloc(sid.size() * (size_t)nsteps);
-        launch_stf_real(amp.p, (const double2*)stf->data, stf->dims[0], (int)sid.size(), nsteps, st);
-    }
-    // distributed adjoint source: A = (sum_ns spectrum) * distribution .* conj(stf) (:255-268)
-    const std::vector<double> I = interpol_matrix(g.K, nb);
-    DBuf<double> I_d, spec_d;
-    upload(I_d, I, st);
-    DBuf<double2> A;
+    DBuf<double> I_d, ss_d, sp_d;
+    DBuf<double2> A, acc_tmp;
     Synthesiser syn;
-    bool dist_src = stf_dist && have_sd;
-    if (dist_src) {
-        std::vector<double> spec_sum((size_t)g.K, 0.0);
-        for (int k = 0; k < g.K; ++k)
-            for (int64_t q = 0; q < n_src; ++q) spec_sum[k] += spec[(size_t)k + (size_t)g.K * q];
-        DBuf<double> ss_d;
-        upload(ss_d, spec_sum, st);
-        A.alloc(g.P * (size_t)g.K);
-        launch_source_spectrum(A.p, (const double2*)stf->data, 1, (const double*)sd->data, nb_cols, I_d.p, ss_d.p,
-                               g.P, g.K, st);
-        HIPCHK(hipStreamSynchronize(st));  // ss_d goes out of scope
-        const int n_syn = nsteps / g.s;
-        std::vector<double> ts;
-        for (int q = 1; q <= n_syn; ++q) ts.push_back((double)(q * g.s - g.nt) * g.dt);
-        syn.init(A.p, g.P, g.K, phase_table(cfg, ts, 1.0, kInvSqrt2Pi * g.dw), n_syn, st);
-    }
-    // DFT of u at the sampled steps: stf_fft (:335-346) and the source kernel (:302-320) share it
-    const bool need_dft = (sfft != nullptr) || (src_kernel && Kd);
-    DBuf<double2> acc_tmp;
     DftAccumulator dft;
-    if (need_dft) {
-        double2* acc = sfft;
-        if (!acc) {
-            acc_tmp.alloc(g.P * (size_t)g.K);
-            acc = acc_tmp.p;
-        }
-        std::vector<double> ts;
-        for (int n = g.s; n <= nsteps; n += g.s) ts.push_back((double)(n - g.nt) * g.dt);
-        dft.init(acc, g.P, g.K, phase_table(cfg, ts, -1.0, 1.0), ts.size(), st);
-    }
     DBuf<float> dmu_f;
-    if (born) {
-        dmu_f.alloc(g.P);
-        launch_scale_to_float((const double*)dmu->data, dmu_f.p, g.P, 1.0, st);
-    }
-
-    StepTimer timer(FD2D_ADJOINT, g.P, st);
-    int64_t i_fw_in = 0, i_fw_out = 0;
-    auto kernel_step = [&](int n) { return Kd && n_ufwd && n_ufwd >= i_fw_in + 1 && n % g.f == 0; };  // :324
-    auto sampled = [&](int n) { return need_dft && n % g.s == 0; };
-    auto synth = [&](int n) { return !(stf_point && !sid.empty()) && dist_src && n % g.s == 0; };
     const float* kfwd[2] = {nullptr, nullptr};
-    auto plan = [&](int n, int sub) {
-        const Solver::Slots sl = S.slots(sub);
+
+    AdjointJob(const fd2d_config* c, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* u_fwd_,
+               const fd2d_carray* stf, const fd2d_carray* src, const fd2d_carray* rec, const fd2d_carray* spectrum,
+               const fd2d_carray* sd, const fd2d_carray* G_, double mode, const fd2d_carray* dmu_,
+               const fd2d_carray* u_in_, fd2d_marray* K_out, fd2d_marray* stf_fft, fd2d_marray* u_out,
+               fd2d_marray* seis, hipStream_t s)
+        : u_fwd(u_fwd_), dmu(dmu_), u_in(u_in_), G(G_)
+    {
+        cfg = c;
+        st = s;
+        solver = FD2D_ADJOINT;
+        g = geometry(cfg);
+        const int64_t P = (int64_t)g.P;
+        N = 2 * g.nt - 1;  // :47-49
+        const int nb = cfg->n_basis_fct;
+        nb_cols = nb == 0 ? 1 : nb;
+        if (numel(mu) != P || numel(rho) != P)
+            raise(FD2D_EINVAL, "mu and rho must have nx*nz = %lld elements", (long long)P);
+        imode = (int)mode;
+        if ((double)imode != mode || !(imode == 0 || imode == 1 || (imode >= 10 && imode <= 13)))
+            raise(FD2D_EINVAL, "mode must be 0, 1, 10, 11, 12 or 13 (run_noise_adjoint.m:16-22)");
+        const bool fft_mode = imode == 1 || imode == 12 || imode == 13;
+        save_mode = imode == 11 || imode == 13;
+
+        const int64_t n_stf = numel(stf);
+        stf_point = n_stf != 0 && stf->dims[2] == 1;            // :249
+        const bool stf_dist = n_stf != 0 && stf->dims[2] != 1;  // :255
+        if (stf_dist && fft_mode)
+            raise(FD2D_EUNSUPPORTED, "a distributed stf with mode %d advances the reference's DFT counter twice per "
+                                     "sample (run_noise_adjoint.m:265,343)", imode);
+        if (stf_dist && (stf->dims[0] * stf->dims[1] != P || stf->dims[2] != g.K))
+            raise(FD2D_EINVAL, "a distributed stf must be nx x nz x n_sample");
+        nsteps = ((imode == 0 || imode == 1) && stf_dist) ? g.nt : N;  // :165-167
+
+        sid = node_ids(src, g, "src");
+        rid = node_ids(rec, g, "rec");
+        nr = (int64_t)rid.size();
+        if (stf_point && (stf->dims[0] < (int64_t)sid.size() || stf->dims[1] < nsteps))
+            raise(FD2D_EINVAL, "a point stf must be n_src x (2nt-1)");
+
+        const bool have_sd = numel(sd) != 0;  // :112-123
+        const int64_t n_src = have_sd ? dim(spectrum, 1) : 0;
+        if (numel(spectrum)) spec = to_host<double>(spectrum->data, (size_t)numel(spectrum));
+        if (have_sd) {
+            if (dim(spectrum, 0) != g.K) raise(FD2D_EINVAL, "spectrum must have n_sample rows");
+            if (nb == 0 && n_src != 1) raise(FD2D_EINVAL, "with n_basis_fct = 0 the reference accepts one noise source");
+            if (numel(sd) != P * (nb == 0 ? n_src : nb)) raise(FD2D_EINVAL, "source_dist size");
+        }
+        src_kernel = numel(G) != 0 && dim(G, 2) == g.K && G->dims[0] * G->dims[1] == P && g.K > 0;  // :302
+        if (src_kernel && (dim(spectrum, 0) != g.K || dim(spectrum, 1) != 1))
+            raise(FD2D_EINVAL, "the source kernel needs an n_sample x 1 spectrum (run_noise_adjoint.m:312)");
+        n_ufwd = numel(u_fwd) ? dim(u_fwd, 2) : 0;  // :324
+        if (n_ufwd && u_fwd->dims[0] * u_fwd->dims[1] != P) raise(FD2D_EINVAL, "u_fwd slices must have nx*nz elements");
+        const int64_t n_uin = numel(u_in) ? dim(u_in, 2) : 0;
+        born = numel(dmu) != 0 && n_uin == N;  // :285
+        if (born && (numel(dmu) != P || u_in->dims[0] * u_in->dims[1] != P)) raise(FD2D_EINVAL, "dmu / u_in sizes");
+
+        const int64_t n_fw = N / g.f;
+        check_out(K_out, g.nx, g.nz, nb_cols + 1, "K");
+        if (fft_mode) check_out(stf_fft, g.nx, g.nz, g.K, "stf_fft");
+        if (save_mode) check_out(u_out, g.nx, g.nz, n_fw, "u_out");
+        check_out(seis, nr, N, 1, "seismograms");
+        Kd = (K_out && K_out->data) ? (double*)K_out->data : nullptr;
+        sfft = (fft_mode && stf_fft && stf_fft->data && g.K > 0) ? (double2*)stf_fft->data : nullptr;
+        uo = (save_mode && u_out && u_out->data) ? (float*)u_out->data : nullptr;
+        sg = (seis && seis->data) ? (double*)seis->data : nullptr;
+        records = sg && nr;
+
+        S.reset(new Solver(cfg, g, st, true));
+        S->material((const double*)mu->data, (const double*)rho->data);
+        if (stf_point) S->set_sources(sid);
+        S->set_receivers(rid);
+        if (records) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * N, st));
+        if (Kd) HIPCHK(hipMemsetAsync(Kd, 0, sizeof(double) * g.P * (size_t)(nb_cols + 1), st));
+
+        // point adjoint sources: real part of stf(i, n) (:249-253)
+        if (stf_point && !sid.empty()) {
+            amp.alloc(sid.size() * (size_t)nsteps);
+            launch_stf_real(amp.p, (const double2*)stf->data, stf->dims[0], (int)sid.size(), nsteps, st);
+        }
+        // distributed adjoint source: A = (sum_ns spectrum) * distribution .* conj(stf) (:255-268)
+        const std::vector<double> I = interpol_matrix(g.K, nb);
+        upload(I_d, I, st);
+        dist_src = stf_dist && have_sd;
+        if (dist_src) {
+            std::vector<double> spec_sum((size_t)g.K, 0.0);
+            for (int k = 0; k < g.K; ++k)
+                for (int64_t q = 0; q < n_src; ++q) spec_sum[k] += spec[(size_t)k + (size_t)g.K * q];
+            upload(ss_d, spec_sum, st);
+            A.alloc(g.P * (size_t)g.K);
+            launch_source_spectrum(A.p, (const double2*)stf->data, 1, (const double*)sd->data, nb_cols, I_d.p, ss_d.p,
+                                   g.P, g.K, st);
+            const int n_syn = nsteps / g.s;
+            std::vector<double> ts;
+            for (int q = 1; q <= n_syn; ++q) ts.push_back((double)(q * g.s - g.nt) * g.dt);
+            syn.init(A.p, g.P, g.K, phase_table(cfg, ts, 1.0, kInvSqrt2Pi * g.dw), n_syn, st);
+        }
+        // DFT of u at the sampled steps: stf_fft (:335-346) and the source kernel (:302-320) share it
+        need_dft = (sfft != nullptr) || (src_kernel && Kd);
+        if (need_dft) {
+            double2* accp = sfft;
+            if (!accp) {
+                acc_tmp.alloc(g.P * (size_t)g.K);
+                accp = acc_tmp.p;
+            }
+            std::vector<double> ts;
+            for (int n = g.s; n <= nsteps; n += g.s) ts.push_back((double)(n - g.nt) * g.dt);
+            dft.init(accp, g.P, g.K, phase_table(cfg, ts, -1.0, 1.0), ts.size(), st);
+        }
+        if (born) {
+            dmu_f.alloc(g.P);
+            launch_scale_to_float((const double*)dmu->data, dmu_f.p, g.P, 1.0, st);
+        }
+    }
+    bool kernel_step(int n) const { return Kd && n_ufwd && n_ufwd >= i_fw_in + 1 && n % g.f == 0; }  // :324
+    bool sampled(int n) const { return need_dft && n % g.s == 0; }
+    bool synth(int n) const { return !(stf_point && !sid.empty()) && dist_src && n % g.s == 0; }
+    void plan(int n, int sub) override
+    {
+        const Solver::Slots sl = S->slots(sub);
         if (save_mode && n % g.f == 0) {  // :237-241
             if (uo) *sl.snap = uo + (size_t)i_fw_out * g.P;
             ++i_fw_out;
         }
         if (stf_point && !sid.empty()) {
-            S.point_sources(sub, amp.p + (size_t)(n - 1) * sid.size(), 0.f);
+            S->point_sources(sub, amp.p + (size_t)(n - 1) * sid.size(), 0.f);
         } else if (synth(n)) {
             *sl.syn = syn.slice(n / g.s - 1);
         }
         if (born && n != N) {  // :285-291
-            S.a.born_dmu = dmu_f.p;
-            S.a.born_in0 = (const float*)u_in->data + (size_t)(n - 1) * g.P;
-            S.a.born_in1 = (const float*)u_in->data + (size_t)n * g.P;
+            S->a.born_dmu = dmu_f.p;
+            S->a.born_in0 = (const float*)u_in->data + (size_t)(n - 1) * g.P;
+            S->a.born_in1 = (const float*)u_in->data + (size_t)n * g.P;
         }
         kfwd[sub] = nullptr;
         if (kernel_step(n)) {  // :324-331
@@ -971,43 +1057,314 @@ void adjoint_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_ca
             ++i_fw_in;
         }
         if (sampled(n)) *sl.dsamp = dft.next_slot();
-        if (sg && nr) S.receivers(sub, sg + (size_t)nr * (n - 1));
-    };
+        if (records) S->receivers(sub, sg + (size_t)nr * (n - 1));
+    }
     // A pair needs the structure-kernel product, if any, after its second step (the displacement
     // of its first step is not kept), at most one DFT sample and at most one synthesised source.
-    const bool pairs = g_steps_per_launch.load() == 2 && !born;
-    for (int n = 1; n <= nsteps;) {
-        const int k = (pairs && n + 1 <= nsteps && !kernel_step(n) && !(sampled(n) && sampled(n + 1)) &&
-                       !(synth(n) && synth(n + 1)))
-                          ? 2
-                          : 1;
-        S.clear_step();
-        for (int q = 0; q < k; ++q) plan(n + q, q);
-        timer.begin(n, k);
-        if (k == 2) S.step_pair();
-        else S.step();
-        timer.end(n, k);
-        if (kfwd[k - 1]) S.kmu(kfwd[k - 1], Kd + (size_t)nb_cols * g.P, (double)g.f);
+    bool pairable(int n) override
+    {
+        return !kernel_step(n) && !(sampled(n) && sampled(n + 1)) && !(synth(n) && synth(n + 1));
+    }
+    void after(int n, int k) override
+    {
+        if (kfwd[k - 1]) S->kmu(kfwd[k - 1], Kd + (size_t)nb_cols * g.P, (double)g.f);
         for (int q = 0; q < k; ++q)
             if (sampled(n + q)) dft.commit();
-        n += k;
     }
-    if (need_dft) {
+    void finish() override
+    {
+        if (!need_dft) return;
         dft.flush();
         if (src_kernel && Kd) {
             // K_s(:,:,b) = sum_n Re(M_b(n) u(n)) = (2pi)^-1/2 dw sum_k I(k,b) spec_k Re(conj(G_k D_k)),
             // D_k = sum_n u(n) e^{-i w_k t_n}
-            DBuf<double> sp_d;
             upload(sp_d, std::vector<double>(spec.begin(), spec.begin() + g.K), st);
             launch_source_kernel(Kd, (const double2*)G->data, dft.acc, I_d.p, sp_d.p, g.P, g.K, nb_cols,
                                  kInvSqrt2Pi * g.dw, st);
-            HIPCHK(hipStreamSynchronize(st));
         }
         if (sfft) launch_finalize_dft(sfft, sfft, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 0, st);
     }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
-    timer.collect();
+};
+
+void green_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* src,
+                  const fd2d_carray* rec, double mode, const fd2d_carray* dmu, const fd2d_carray* G_in,
+                  fd2d_marray* G_fft, fd2d_marray* G_out, fd2d_marray* seis, hipStream_t st)
+{
+    GreenJob j(cfg, mu, rho, src, rec, mode, dmu, G_in, G_fft, G_out, seis, st);
+    run_job(j);
+}
+
+void correlation_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* G,
+                        const fd2d_carray* spectrum, const fd2d_carray* sd, const fd2d_carray* rec, double mode,
+                        const fd2d_carray* dmu, const fd2d_carray* C_in, fd2d_marray* seis, fd2d_marray* C_out,
+                        hipStream_t st)
+{
+    CorrelationJob j(cfg, mu, rho, G, spectrum, sd, rec, mode, dmu, C_in, seis, C_out, st);
+    run_job(j);
+}
+
+void adjoint_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* u_fwd,
+                    const fd2d_carray* stf, const fd2d_carray* src, const fd2d_carray* rec,
+                    const fd2d_carray* spectrum, const fd2d_carray* sd, const fd2d_carray* G, double mode,
+                    const fd2d_carray* dmu, const fd2d_carray* u_in, fd2d_marray* K_out, fd2d_marray* stf_fft,
+                    fd2d_marray* u_out, fd2d_marray* seis, hipStream_t st)
+{
+    AdjointJob j(cfg, mu, rho, u_fwd, stf, src, rec, spectrum, sd, G, mode, dmu, u_in, K_out, stf_fft, u_out, seis, st);
+    run_job(j);
+}
+
+// =============================================================================================
+// Batch engine: many solves advanced together. Every round advances each active solve by one
+// time step; solves with the same FD order and field set are stepped by one grouped launch
+// (k_tile_group) over the tiles of all of them, the rest one launch each. A solve's results are
+// those of its own entry point bit for bit (the same tile kernel body runs on the same data).
+// =============================================================================================
+struct Entry {
+    fd2d_config cfg{};
+    std::vector<double> w;
+    fd2d_carray in[11]{};
+    bool has_in[11]{};
+    fd2d_marray out[4]{};
+    bool has_out[4]{};
+    std::unique_ptr<Job> job;
+    int64_t ticket = 0;
+    const fd2d_carray* cin(int q) const { return has_in[q] ? &in[q] : nullptr; }
+    fd2d_marray* mout(int q) { return has_out[q] ? &out[q] : nullptr; }
+};
+
+enum { kMu, kRho, kSrc, kRec, kGfft, kSpec, kSd, kStf, kUfwd, kDmu, kBornIn };
+
+void check_solve(const fd2d_solve* s)
+{
+    if (!s || !s->cfg) raise(FD2D_EINVAL, "solve and its cfg must not be NULL");
+    if (s->solver < FD2D_GREEN || s->solver > FD2D_ADJOINT)
+        raise(FD2D_EINVAL, "solver must be FD2D_GREEN, FD2D_CORRELATION or FD2D_ADJOINT");
+    if (s->cfg->n_sample > 0 && !s->cfg->w_sample) raise(FD2D_EINVAL, "cfg->w_sample is NULL");
+}
+
+std::unique_ptr<Entry> make_entry(const fd2d_solve* s, hipStream_t st)
+{
+    check_solve(s);
+    std::unique_ptr<Entry> e(new Entry);
+    e->cfg = *s->cfg;
+    if (e->cfg.n_sample > 0) e->w.assign(s->cfg->w_sample, s->cfg->w_sample + e->cfg.n_sample);
+    e->cfg.w_sample = e->w.data();
+    const fd2d_carray* src_in[11] = {s->mu, s->rho, s->src, s->rec, s->G_fft, s->spectrum, s->source_dist,
+                                     s->stf, s->u_fwd, s->dmu, s->born_in};
+    for (int q = 0; q < 11; ++q)
+        if (src_in[q]) {
+            e->in[q] = *src_in[q];
+            e->has_in[q] = true;
+        }
+    for (int q = 0; q < 4; ++q)
+        if (s->out[q]) {
+            e->out[q] = *s->out[q];
+            e->has_out[q] = true;
+        }
+    const fd2d_config* c = &e->cfg;
+    Entry& E = *e;
+    switch (s->solver) {
+    case FD2D_GREEN:
+        E.job.reset(new GreenJob(c, E.cin(kMu), E.cin(kRho), E.cin(kSrc), E.cin(kRec), s->mode, E.cin(kDmu),
+                                 E.cin(kBornIn), E.mout(0), E.mout(1), E.mout(2), st));
+        break;
+    case FD2D_CORRELATION:
+        E.job.reset(new CorrelationJob(c, E.cin(kMu), E.cin(kRho), E.cin(kGfft), E.cin(kSpec), E.cin(kSd),
+                                       E.cin(kRec), s->mode, E.cin(kDmu), E.cin(kBornIn), E.mout(0), E.mout(1), st));
+        break;
+    default:
+        E.job.reset(new AdjointJob(c, E.cin(kMu), E.cin(kRho), E.cin(kUfwd), E.cin(kStf), E.cin(kSrc), E.cin(kRec),
+                                   E.cin(kSpec), E.cin(kSd), E.cin(kGfft), s->mode, E.cin(kDmu), E.cin(kBornIn),
+                                   E.mout(0), E.mout(1), E.mout(2), E.mout(3), st));
+        break;
+    }
+    return e;
+}
+
+struct Engine {
+    hipStream_t st = nullptr;
+    std::vector<std::unique_ptr<Entry>> act;
+    int64_t next_ticket = 1;
+    bool dirty = true;
+    DBuf<StepArgs> statics;  // per grouped job: its StepArgs for parity 0 and 1
+    struct Group {
+        int R = 0;
+        bool with_u = false;
+        int first = 0;             // index of the first member's pair in statics
+        std::vector<Entry*> members;
+    };
+    std::vector<Group> groups;
+    std::vector<Entry*> singles;
+
+    // Re-partition the active jobs after the set changed (stream already idle).
+    void rebuild()
+    {
+        groups.clear();
+        singles.clear();
+        std::vector<std::pair<std::pair<int, int>, Entry*>> cls;
+        for (auto& e : act) {
+            Job& j = *e->job;
+            if (j.next > j.nsteps) continue;
+            if (j.born) singles.push_back(e.get());
+            else cls.push_back({{j.g.R, j.S->with_u ? 1 : 0}, e.get()});
+        }
+        std::stable_sort(cls.begin(), cls.end(),
+                         [](const auto& x, const auto& y) { return x.first < y.first; });
+        std::vector<StepArgs> h;
+        for (size_t q = 0; q < cls.size();) {
+            size_t r = q;
+            while (r < cls.size() && cls[r].first == cls[q].first && r - q < (size_t)kGroupMax) ++r;
+            if (r - q == 1) {
+                singles.push_back(cls[q].second);
+            } else {
+                Group gr;
+                gr.R = cls[q].first.first;
+                gr.with_u = cls[q].first.second != 0;
+                gr.first = (int)(h.size() / 2);
+                for (size_t k = q; k < r; ++k) {
+                    Job& j = *cls[k].second->job;
+                    h.push_back(j.static_args(0));
+                    h.push_back(j.static_args(1));
+                    gr.members.push_back(cls[k].second);
+                }
+                groups.push_back(std::move(gr));
+            }
+            q = r;
+        }
+        if (h.empty()) statics.release();
+        else upload(statics, h, st);
+        dirty = false;
+    }
+    void step_single(Job& j)
+    {
+        j.S->clear_step();
+        j.plan(j.next, 0);
+        j.S->step();
+        j.after(j.next, 1);
+        ++j.next;
+    }
+    void step_group(const Group& gr)
+    {
+        GroupArgs ga{};
+        ga.jobs = statics.p + 2 * (size_t)gr.first;
+        ga.n_jobs = (int)gr.members.size();
+        int total = 0;
+        for (int q = 0; q < ga.n_jobs; ++q) {
+            Job& j = *gr.members[q]->job;
+            j.S->clear_step();
+            j.plan(j.next, 0);
+            const StepArgs& a = j.S->a;
+            ga.tile_start[q] = total;
+            total += j.S->n_tiles;
+            ga.step[q] = JobStep{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.ps_const, j.S->cur};
+        }
+        ga.tile_start[ga.n_jobs] = total;
+        ga.total_tiles = total;
+        ga.tiles_per_xcd = (total + kXCD - 1) / kXCD;
+        launch_group(gr.R, gr.with_u, ga, st);
+        for (int q = 0; q < ga.n_jobs; ++q) {
+            Job& j = *gr.members[q]->job;
+            j.S->cur ^= 1;
+            j.after(j.next, 1);
+            ++j.next;
+        }
+    }
+    // Finish the solves that have taken their last step and drop them; true if any.
+    bool retire()
+    {
+        bool any = false;
+        for (auto& e : act)
+            if (e->job->next > e->job->nsteps) {
+                e->job->finish();
+                any = true;
+            }
+        if (!any) return false;
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+        act.erase(std::remove_if(act.begin(), act.end(),
+                                 [](const std::unique_ptr<Entry>& e) { return e->job->next > e->job->nsteps; }),
+                  act.end());
+        dirty = true;
+        return true;
+    }
+    int64_t submit(const fd2d_solve* s)
+    {
+        std::unique_ptr<Entry> e = make_entry(s, st);
+        e->ticket = next_ticket++;
+        act.push_back(std::move(e));
+        dirty = true;
+        return act.back()->ticket;
+    }
+    void advance(int rounds)
+    {
+        for (int r = 0; r < rounds && !act.empty(); ++r) {
+            retire();
+            if (act.empty()) break;
+            if (dirty) {
+                HIPCHK(hipStreamSynchronize(st));
+                rebuild();
+            }
+            for (const Group& gr : groups) step_group(gr);
+            for (Entry* e : singles) step_single(*e->job);
+        }
+        retire();
+        HIPCHK(hipGetLastError());
+    }
+    bool active(int64_t ticket) const
+    {
+        for (const auto& e : act)
+            if (e->ticket == ticket) return true;
+        return false;
+    }
+    ~Engine()
+    {
+        if (!act.empty()) (void)hipStreamSynchronize(st);
+    }
+};
+
+// Device bytes a solve allocates inside the library (fields, coefficients, lists, spectra and slabs),
+// an upper bound for the sizes of its arrays; the caller's own arrays are not counted.
+int64_t solve_bytes(const fd2d_solve* s)
+{
+    check_solve(s);
+    const fd2d_config* cfg = s->cfg;
+    const Geometry g = geometry(cfg);
+    const PaddedLayout lay = PaddedLayout::make(g.nx, g.nz);
+    const int64_t P = (int64_t)g.P, E = (int64_t)lay.elems(), K = g.K, C_pad = spec_cols(g.K);
+    const bool with_u = s->solver != FD2D_GREEN;
+    int64_t b = E * 4 * (with_u ? 4 : 3) * 2 + E * 4 * 2 + 4 * (int64_t)(lay.pitch + lay.rows + g.nx + g.nz);
+    const int64_t n_src = numel(s->src) ? dim(s->src, 0) : 0, n_rec = numel(s->rec) ? dim(s->rec, 0) : 0;
+    b += (int64_t)(g.nx / kTX + 2) * (g.nz / kTileZ + 2) * 8 + 8 * 9 * (n_src + n_rec);  // tile lists
+    const int N = 2 * g.nt - 1;
+    auto slab = [&](int64_t n) { return std::min<int64_t>(kSpecSlab, std::max<int64_t>(n, 1)); };
+    auto dft_bytes = [&](int64_t n) { return slab(n) * P * 4 + n * C_pad * 4; };
+    auto syn_bytes = [&](int64_t n) {
+        const int64_t T = slab(n), J = (T + 255) / 256 * 256, slabs = (std::max<int64_t>(n, 1) + T - 1) / T;
+        return T * P * 4 + C_pad * P * 4 + slabs * C_pad * J * 4 + P * K * 16;  // slab, a32, tables, A
+    };
+    const bool born = numel(s->dmu) != 0;
+    if (s->solver == FD2D_GREEN) {
+        if (numel(s->out[0]) && K) b += dft_bytes(g.nt / std::max(g.s, 1) + 1);
+        if (born) b += P * 4;
+    } else if (s->solver == FD2D_CORRELATION) {
+        b += syn_bytes(N / std::max(g.s, 1)) + K * (int64_t)std::max(cfg->n_basis_fct, 1) * 8 + K * 8;
+        if (born) b += P * 4;
+    } else {
+        const bool stf_dist = numel(s->stf) && s->stf->dims[2] != 1;
+        const int64_t nsteps = N;
+        if (numel(s->stf) && !stf_dist) b += n_src * nsteps * 4;
+        if (stf_dist && numel(s->source_dist)) b += syn_bytes(nsteps / std::max(g.s, 1));
+        const int imode = (int)s->mode;
+        const bool fft_mode = imode == 1 || imode == 12 || imode == 13;
+        const bool src_kernel = numel(s->G_fft) != 0;
+        if (fft_mode || src_kernel) {
+            b += dft_bytes(nsteps / std::max(g.s, 1));
+            if (!(fft_mode && numel(s->out[1]))) b += P * K * 16;
+        }
+        b += K * (int64_t)std::max(cfg->n_basis_fct, 1) * 8 + 2 * K * 8;
+        if (born) b += P * 4;
+    }
+    return b;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1130,6 +1487,10 @@ void blur_maps(const Geometry& g, const MapFrame& f, const double* src, double* 
 
 }  // namespace
 
+struct fd2d_engine {
+    Engine e;
+};
+
 extern "C" {
 
 const char* fd2d_last_error(void) { return g_err.c_str(); }
@@ -1142,6 +1503,70 @@ int fd2d_set_steps_per_launch(int32_t steps)
     }
     g_steps_per_launch.store(steps);
     return FD2D_OK;
+}
+
+int fd2d_engine_create(void* stream, fd2d_engine** engine)
+{
+    return guarded([&] {
+        if (!engine) raise(FD2D_EINVAL, "engine is NULL");
+        require_device();
+        std::unique_ptr<fd2d_engine> e(new fd2d_engine);
+        e->e.st = as_stream(stream);
+        *engine = e.release();
+    });
+}
+
+int fd2d_engine_submit(fd2d_engine* engine, const fd2d_solve* solve, int64_t* ticket)
+{
+    return guarded([&] {
+        if (!engine) raise(FD2D_EINVAL, "engine is NULL");
+        const int64_t t = engine->e.submit(solve);
+        if (ticket) *ticket = t;
+    });
+}
+
+int fd2d_engine_advance(fd2d_engine* engine, int32_t rounds, int32_t* n_active)
+{
+    return guarded([&] {
+        if (!engine) raise(FD2D_EINVAL, "engine is NULL");
+        if (rounds < 0) raise(FD2D_EINVAL, "rounds must be >= 0");
+        engine->e.advance(rounds);
+        if (n_active) *n_active = (int32_t)engine->e.act.size();
+    });
+}
+
+int fd2d_engine_done(const fd2d_engine* engine, int64_t ticket, int32_t* done)
+{
+    return guarded([&] {
+        if (!engine || !done) raise(FD2D_EINVAL, "engine and done must not be NULL");
+        if (ticket < 1 || ticket >= engine->e.next_ticket) raise(FD2D_EINVAL, "unknown ticket %lld", (long long)ticket);
+        *done = engine->e.active(ticket) ? 0 : 1;
+    });
+}
+
+int fd2d_engine_destroy(fd2d_engine* engine)
+{
+    return guarded([&] { delete engine; });
+}
+
+int fd2d_run_batch_device(const fd2d_solve* solves, int32_t n, void* stream)
+{
+    return guarded([&] {
+        if (n < 0 || (n > 0 && !solves)) raise(FD2D_EINVAL, "solves is NULL");
+        require_device();
+        fd2d_engine e;
+        e.e.st = as_stream(stream);
+        for (int32_t q = 0; q < n; ++q) e.e.submit(&solves[q]);
+        while (!e.e.act.empty()) e.e.advance(1 << 20);
+    });
+}
+
+int fd2d_solve_device_bytes(const fd2d_solve* solve, int64_t* bytes)
+{
+    return guarded([&] {
+        if (!bytes) raise(FD2D_EINVAL, "bytes is NULL");
+        *bytes = solve_bytes(solve);
+    });
 }
 
 int fd2d_timing_enable(int32_t every)

@@ -119,12 +119,39 @@ struct PairArgs {
     double* seis_col2;
 };
 
+// The per-step fields of one solve in a grouped launch (the rest of its StepArgs is fixed).
+struct JobStep {
+    const float* ps_amp;
+    const float* syn;
+    float* snap;
+    float* dsamp;
+    double* seis_col;
+    float ps_const;
+    int parity;  // which of the solve's two StepArgs (ping-pong direction) this step uses
+};
+
+constexpr int kGroupMax = 48;  // solves per grouped launch (kernel arguments stay under 4 KB)
+
+// One time step of up to kGroupMax solves with the same order and field set in one launch: the
+// tiles of all solves are concatenated (solve j owns tiles tile_start[j] .. tile_start[j+1]-1)
+// and dealt to the XCDs in contiguous bands. jobs[2j + parity] holds solve j's StepArgs.
+struct GroupArgs {
+    const StepArgs* jobs;
+    int n_jobs;
+    int total_tiles;
+    int tiles_per_xcd;
+    int tile_start[kGroupMax + 1];
+    JobStep step[kGroupMax];
+};
+static_assert(sizeof(GroupArgs) <= 4096, "grouped-launch arguments exceed the 4 KB kernel-argument limit");
+
 // halo of the first step's velocity region of a pair launch (the point-source lists cover it)
 inline int pair_halo_lo(int R) { return 3 * R - 2; }
 inline int pair_halo_hi(int R) { return 3 * R - 1; }
 
 void launch_step(int R, bool with_u, const StepArgs& a, hipStream_t s);
 void launch_pair(int R, bool with_u, const PairArgs& p, hipStream_t s);
+void launch_group(int R, bool with_u, const GroupArgs& g, hipStream_t s);
 // K_mu -= scale * (D u . D fwd) over the grid (u padded, fwd compact), after a step.
 void launch_kmu(int R, const StepArgs& a, const float* u, const float* fwd, double* kmu, double scale,
                 hipStream_t s);

@@ -77,7 +77,7 @@ struct Grid {
 };
 
 template <int R, int MODE>
-__global__ __launch_bounds__(256) void k_tile(StepArgs a)
+__device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, int t)
 {
     constexpr bool HAS_U = MODE >= 1;
     constexpr int TZ = kTileZ;
@@ -91,8 +91,6 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
     __shared__ float sz[HSZ * WV + 1];
     __shared__ float vv[HV * WV + 1];
 
-    const int g = blockIdx.x;
-    const int t = (g % kXCD) * a.tiles_per_xcd + g / kXCD;
     if (t >= a.n_tiles) return;
     const int i0 = (t % a.tiles_x) * kTX, j0 = (t / a.tiles_x) * TZ;
     const int nx = a.nx, nz = a.nz;
@@ -101,8 +99,8 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
     const int x = i0 + lane;                 // this lane's own column
     const Grid G{nx, nz, a.pitch, rsrc(a.fbase, a.fbytes), rsrc(a.cbase, a.cbytes)};
     const unsigned cb = a.compact_bytes;
-    const bool has_syn = a.syn != nullptr;
-    const auto r_syn = rsrc(has_syn ? (const void*)a.syn : (const void*)a.fbase, has_syn ? cb : 0u);
+    const bool has_syn = st.syn != nullptr;
+    const auto r_syn = rsrc(has_syn ? (const void*)st.syn : (const void*)a.fbase, has_syn ? cb : 0u);
 
     // ---------------- phase 1
     constexpr int NRV = (HV + 3) / 4, NRZ = (HSZ + 3) / 4;
@@ -195,7 +193,7 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
             const int key = node_key(xx, zz);
             for (int e = e0; e < e1; ++e) {
                 const int2 en = a.ps_ent[e];
-                if (en.x == key) ds += a.ps_amp ? a.ps_amp[en.y] : a.ps_const;
+                if (en.x == key) ds += st.ps_amp ? st.ps_amp[en.y] : st.ps_const;
             }
         }
         return ds;
@@ -217,17 +215,17 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
             if (HAS_U) {
                 const float un = mu[k] + vn * a.dt;
                 G.put(a.o_u_out, x, z, un);
-                if (a.snap) bstore(rsrc(a.snap, cb), co, mu[k]);
+                if (st.snap) bstore(rsrc(st.snap, cb), co, mu[k]);
                 out = un;
-            } else if (a.snap) {
-                bstore(rsrc(a.snap, cb), co, mv[k]);
+            } else if (st.snap) {
+                bstore(rsrc(st.snap, cb), co, mv[k]);
             }
-            if (a.dsamp) bstore(rsrc(a.dsamp, cb), co, out);
-            if (f1 > f0) {
+            if (st.dsamp) bstore(rsrc(st.dsamp, cb), co, out);
+            if (f1 > f0 && st.seis_col) {
                 const int key = node_key(x, z);
                 for (int e = f0; e < f1; ++e) {
                     const int2 en = a.rec_ent[e];
-                    if (en.x == key) a.seis_col[en.y] = (double)out;
+                    if (en.x == key) st.seis_col[en.y] = (double)out;
                 }
             }
         }
@@ -289,6 +287,26 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
         G.put(a.o_sxy_out, x, z, snx);
         G.put(a.o_szy_out, x, z, snz);
     }
+}
+
+template <int R, int MODE>
+__global__ __launch_bounds__(256) void k_tile(StepArgs a)
+{
+    const int g = blockIdx.x;
+    const JobStep st{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.ps_const, 0};
+    tile_step<R, MODE>(a, st, (g % kXCD) * a.tiles_per_xcd + g / kXCD);
+}
+
+template <int R, int MODE>
+__global__ __launch_bounds__(256) void k_tile_group(GroupArgs G)
+{
+    const int g = blockIdx.x;
+    const int T = (g % kXCD) * G.tiles_per_xcd + g / kXCD;
+    if (T >= G.total_tiles) return;
+    int j = 0;
+    while (j + 1 < G.n_jobs && G.tile_start[j + 1] <= T) ++j;  // wave-uniform scan
+    const JobStep& st = G.step[j];
+    tile_step<R, MODE>(G.jobs[2 * j + st.parity], st, T - G.tile_start[j]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -613,36 +631,49 @@ template <int R>
 __global__ __launch_bounds__(256) void k_kmu(StepArgs a, const float* __restrict__ u, const float* __restrict__ fwd,
                                              double* __restrict__ kmu, double scale)
 {
-    const int x = blockIdx.x * kTX + threadIdx.x, z = blockIdx.y * 4 + threadIdx.y;
-    if (x >= a.nx || z >= a.nz) return;
-    const bool xin = x >= R - 1 && x <= a.nx - R - 1, zin = z >= R - 1 && z <= a.nz - R - 1;
-    if (!xin && !zin) return;
-    const int p = z * a.pitch + x, c = z * a.nx + x, pz = a.pitch, cz = a.nx;
-    double acc = 0.0;
-    if (xin) {
-        float su = 0.f, sf = 0.f;
+    // the step kernel's 64 x 32 tiles in the same XCD-banded order, so the z-stencil rows a tile
+    // shares with the tiles above and below come from the same L2
+    const int g = blockIdx.x;
+    const int t = (g % kXCD) * a.tiles_per_xcd + g / kXCD;
+    if (t >= a.n_tiles) return;
+    const int x = (t % a.tiles_x) * kTX + (threadIdx.x & 63);
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    if (x >= a.nx) return;
+    const bool xin = x >= R - 1 && x <= a.nx - R - 1;
+    const int pz = a.pitch, cz = a.nx;
+#pragma unroll 2
+    for (int k = 0; k < kTileZ / 4; ++k) {
+        const int z = (t / a.tiles_x) * kTileZ + w + 4 * k;
+        if (z >= a.nz) break;
+        const bool zin = z >= R - 1 && z <= a.nz - R - 1;
+        if (!xin && !zin) continue;
+        const int p = z * pz + x, c = z * cz + x;
+        double acc = 0.0;
+        if (xin) {
+            float su = 0.f, sf = 0.f;
 #pragma unroll
-        for (int m = 1; m <= R; ++m) {
-            su += a.cx[m - 1] * (u[p + m] - u[p - m + 1]);
-            sf += a.cx[m - 1] * (fwd[c + m] - fwd[c - m + 1]);
+            for (int m = 1; m <= R; ++m) {
+                su += a.cx[m - 1] * (u[p + m] - u[p - m + 1]);
+                sf += a.cx[m - 1] * (fwd[c + m] - fwd[c - m + 1]);
+            }
+            acc += (double)su * (double)sf;
         }
-        acc += (double)su * (double)sf;
-    }
-    if (zin) {
-        float su = 0.f, sf = 0.f;
+        if (zin) {
+            float su = 0.f, sf = 0.f;
 #pragma unroll
-        for (int m = 1; m <= R; ++m) {
-            su += a.cz[m - 1] * (u[p + m * pz] - u[p - (m - 1) * pz]);
-            sf += a.cz[m - 1] * (fwd[c + m * cz] - fwd[c - (m - 1) * cz]);
+            for (int m = 1; m <= R; ++m) {
+                su += a.cz[m - 1] * (u[p + m * pz] - u[p - (m - 1) * pz]);
+                sf += a.cz[m - 1] * (fwd[c + m * cz] - fwd[c - (m - 1) * cz]);
+            }
+            acc += (double)su * (double)sf;
         }
-        acc += (double)su * (double)sf;
+        kmu[c] -= acc * scale;
     }
-    kmu[c] -= acc * scale;
 }
 
 void launch_kmu(int R, const StepArgs& a, const float* u, const float* fwd, double* kmu, double scale, hipStream_t s)
 {
-    const dim3 g((unsigned)((a.nx + kTX - 1) / kTX), (unsigned)((a.nz + 3) / 4)), b(kTX, 4);
+    const dim3 g((unsigned)(kXCD * a.tiles_per_xcd)), b(256);
     switch (R) {
     case 1: hipLaunchKernelGGL(k_kmu<1>, g, b, 0, s, a, u, fwd, kmu, scale); break;
     case 2: hipLaunchKernelGGL(k_kmu<2>, g, b, 0, s, a, u, fwd, kmu, scale); break;
@@ -690,6 +721,23 @@ void launch_pair(int R, bool with_u, const PairArgs& p, hipStream_t s)
         FD2D_PAIR(4)
     }
 #undef FD2D_PAIR
+}
+
+void launch_group(int R, bool with_u, const GroupArgs& ga, hipStream_t s)
+{
+    const dim3 g((unsigned)(kXCD * ga.tiles_per_xcd)), b(256);
+#define FD2D_GROUP(RR)                                                                           \
+    case RR:                                                                                     \
+        if (with_u) hipLaunchKernelGGL((k_tile_group<RR, 1>), g, b, 0, s, ga);                   \
+        else hipLaunchKernelGGL((k_tile_group<RR, 0>), g, b, 0, s, ga);                          \
+        break;
+    switch (R) {
+        FD2D_GROUP(1)
+        FD2D_GROUP(2)
+        FD2D_GROUP(3)
+        FD2D_GROUP(4)
+    }
+#undef FD2D_GROUP
 }
 
 void launch_step(int R, bool with_u, const StepArgs& a, hipStream_t s)

@@ -276,6 +276,63 @@ int fd2d_map_gradparameters_to_gradm_device(const fd2d_config* cfg, const double
 int fd2d_regularization_device(double* j, double* g, const double* m, const double* m0, int64_t n, int64_t n_struct,
                                const double* w, int64_t n_w, double alpha, double beta, void* stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Batched solves (no counterpart in the reference, which runs one MEX call at a time; they serve
+ * the loops that call the three solvers per station and per basis function, e.g.
+ * inversion/optlib_callback_fct/eval_noise_objective_and_gradient.m:54-160)
+ * ------------------------------------------------------------------------------------------- */
+
+/*
+ * One solve: the arguments of one of the three *_device entry points. Inputs a solver does not take
+ * are NULL (see the per-solver lists). born_in is G_in, C_in or u_in. out[] are the outputs in
+ * MATLAB order, as fd2d_query_output_dims() lists them (Green: G_fft, G_out, seismograms;
+ * correlation: seismograms, C_out; adjoint: K, stf_fft, u_out, seismograms). The config, w_sample
+ * and every descriptor are copied at submission; the device arrays they point to must stay valid,
+ * and the inputs unchanged, until the solve is done.
+ */
+typedef struct fd2d_solve {
+    int32_t solver;                    /* FD2D_GREEN, FD2D_CORRELATION or FD2D_ADJOINT */
+    double mode;
+    const fd2d_config* cfg;
+    const fd2d_carray* mu;
+    const fd2d_carray* rho;
+    const fd2d_carray* src;            /* Green, adjoint */
+    const fd2d_carray* rec;
+    const fd2d_carray* G_fft;          /* correlation, adjoint */
+    const fd2d_carray* spectrum;       /* correlation, adjoint */
+    const fd2d_carray* source_dist;    /* correlation, adjoint */
+    const fd2d_carray* stf;            /* adjoint */
+    const fd2d_carray* u_fwd;          /* adjoint */
+    const fd2d_carray* dmu;
+    const fd2d_carray* born_in;
+    fd2d_marray* out[4];
+} fd2d_solve;
+
+/*
+ * An engine advances many solves together on one stream: each round advances every active solve by
+ * one time step, with one launch for all solves of the same FD order and field set (Green runs
+ * carry no displacement field, correlation and adjoint runs do) and one launch per Born solve.
+ * Each solve's outputs equal those of its own entry point bit for bit. Solves may be submitted
+ * between rounds (continuous batching); a solve's outputs are complete once
+ * fd2d_engine_done() reports it. An engine is used by one host thread at a time.
+ */
+typedef struct fd2d_engine fd2d_engine;
+
+int fd2d_engine_create(void* stream, fd2d_engine** engine);
+/* Validates and sets up a solve (its device memory is allocated here); *ticket names it. */
+int fd2d_engine_submit(fd2d_engine* engine, const fd2d_solve* solve, int64_t* ticket);
+/* Runs up to `rounds` rounds; *n_active (may be NULL) = solves still running afterwards. */
+int fd2d_engine_advance(fd2d_engine* engine, int32_t rounds, int32_t* n_active);
+/* *done = 1 when the solve's outputs are complete (it has finished and the stream is synchronised). */
+int fd2d_engine_done(const fd2d_engine* engine, int64_t ticket, int32_t* done);
+/* Waits for the stream and frees the engine; unfinished solves are abandoned. */
+int fd2d_engine_destroy(fd2d_engine* engine);
+/* Runs n solves to completion with one engine. */
+int fd2d_run_batch_device(const fd2d_solve* solves, int32_t n, void* stream);
+/* Upper bound of the device bytes one solve allocates inside the library (the caller's arrays
+ * are not counted): the figure an admission policy charges against an HBM budget. */
+int fd2d_solve_device_bytes(const fd2d_solve* solve, int64_t* bytes);
+
 /*
  * Steps per launch of the time loop (tuning; it has no counterpart in the reference). 1 (the
  * default) launches every step on its own; 2 advances two consecutive time steps per launch

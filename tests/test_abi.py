@@ -70,3 +70,46 @@ def test_matlab_dims():
     assert abi.matlab_dims((5,)) == (5, 1, 1)
     assert abi.matlab_dims((5, 2)) == (5, 2, 1)
     assert abi.matlab_dims(np.zeros((2, 3, 4)).shape) == (2, 3, 4)
+
+
+def _solve_struct(c, solver, mode=0.0, **arrays):
+    """An fd2d_solve whose descriptors carry sizes only (the byte estimate reads no data)."""
+    from fd2d_noise_amd.batch import CSolve
+    keep = [c.to_c()]
+    s = CSolve()
+    s.solver = solver
+    s.mode = mode
+    s.cfg = ctypes.cast(ctypes.pointer(keep[0]), ctypes.c_void_p)
+    for name, dims in arrays.items():
+        d = abi.CArray()
+        d.data = 1  # non-NULL marks the array as given; never dereferenced
+        d.dims[:] = dims
+        keep.append(d)
+        setattr(s, name, ctypes.pointer(d))
+    return s, keep
+
+
+def _bytes(s):
+    from fd2d_noise_amd import batch
+    b = ctypes.c_int64(0)
+    st = batch._bind().fd2d_solve_device_bytes(ctypes.byref(s), ctypes.byref(b))
+    return st, b.value
+
+
+def test_solve_device_bytes():
+    small, big = Config(nx=100, nz=100), Config(nx=2400, nz=2400)
+    for c in (small, big):
+        P = c.nx * c.nz
+        s, _ = _solve_struct(c, abi.FD2D_GREEN, src=(1, 2, 1), rec=(3, 2, 1))
+        st, b = _bytes(s)
+        # two ping-pong sets of v, sxy, szy and the two coefficients: 8 floats per node of the padded
+        # layout (64 x 32 tiles, one tile column and 32 rows of ghosts)
+        E = 64 * (-(-c.nx // 64) + 1) * (32 * -(-c.nz // 32) + 32)
+        assert st == abi.FD2D_OK and 8 * 4 * E <= b <= 8 * 4 * E + 64 * 1024
+        s, _ = _solve_struct(c, abi.FD2D_CORRELATION, rec=(3, 2, 1), G_fft=(c.nx, c.nz, c.n_sample),
+                             spectrum=(c.n_sample, 1, 1), source_dist=(c.nx, c.nz, 1))
+        st, bc = _bytes(s)
+        # the complex source spectrum alone is 16 bytes per node and frequency
+        assert st == abi.FD2D_OK and bc > b + 16 * P * c.n_sample
+    s, _ = _solve_struct(small, 7)
+    assert _bytes(s)[0] == abi.FD2D_EINVAL and "solver" in _lib.last_error()

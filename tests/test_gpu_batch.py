@@ -1,0 +1,191 @@
+"""Batched solves against the single-solve entry points, through the C ABI.
+
+Each solve of a batch must give exactly (bit for bit) what its own *_device call gives: the engine
+runs the same tile kernel body over the same data, only with the tiles of several solves in one
+launch. The mix covers the three solvers, two grid sizes under one FD order (heterogeneous nx/nz
+and nt in one grouped launch), a second FD order, Born solves (stepped one launch each), point and
+distributed adjoint sources, snapshots, DFT outputs, seismograms and both kernels. The
+single-solve path itself is held to the oracle by test_gpu_parity.py.
+"""
+import numpy as np
+import pytest
+
+from fd2d_noise_amd import Config, _lib, abi
+
+from helpers import blob_distribution, hetero_model, interior_points
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_mod():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    _lib.load()
+    return torch
+
+
+def _cases(torch):
+    """Inputs of a mix of solves, prepared with the single-solve path. Returns a list of
+    (kind, args, kwargs-without-outputs, output-dims-and-dtypes)."""
+    from fd2d_noise_amd import device as D
+    f32, f64, cd = torch.float32, torch.float64, torch.complex128
+    c1 = Config(order=4, nx=70, nz=52, nt=160, fw_nth=2, freq_samp=3)
+    c2 = Config(order=4, nx=50, nz=90, nt=120, fw_nth=3)
+    c3 = Config(order=8, nx=64, nz=60, nt=140, fw_nth=4, Lx=63e3, Lz=59e3)
+    rng = np.random.default_rng(901)
+    cases = []
+    for c in (c1, c2, c3):
+        mu, rho = hetero_model(rng, c)
+        mu_d, rho_d = D.to_device(mu), D.to_device(rho)
+        srcs = [interior_points(rng, c, 1, 0.4) for _ in range(2)]
+        rec = interior_points(rng, c, 3)
+        rec_d = D.to_device(rec)
+        nr = rec.shape[0]
+        spectrum = D.to_device(np.exp(-((np.asarray(c.f_sample) - 0.1) ** 2) / 0.05 ** 2)[:, None])
+        sd = D.to_device(blob_distribution(rng, c))
+        for q, src in enumerate(srcs):
+            src_d = D.to_device(src)
+            G = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
+            Gout = D.empty_like_dims((c.nx, c.nz, c.n_fw_green()), f32)
+            D.run_forward1_green(mu_d, rho_d, src_d, rec_d, 1, None, None, config=c, G_fft=G, G_out=Gout)
+            gouts = [((c.nx, c.nz, c.n_sample), cd), ((c.nx, c.nz, c.n_fw_green()), f32), ((nr, c.nt), f64)]
+            cases.append(("green", c, (mu_d, rho_d, src_d, rec_d, 1, None, None), gouts))
+            corr = D.empty_like_dims((nr, c.n_corr), f64)
+            C = D.empty_like_dims((c.nx, c.nz, c.n_fw_corr()), f32)
+            D.run_forward2_correlation(mu_d, rho_d, G, spectrum, sd, rec_d, 1, None, None, config=c,
+                                       seismograms=corr, C_out=C)
+            couts = [((nr, c.n_corr), f64), ((c.nx, c.nz, c.n_fw_corr()), f32)]
+            cases.append(("correlation", c, (mu_d, rho_d, G, spectrum, sd, rec_d, 1, None, None), couts))
+            a = D.to_host(corr)
+            stf = D.to_device(np.ascontiguousarray(a[:, ::-1] / np.abs(a).max()).astype(np.complex128))
+            nb1 = max(c.n_basis_fct, 1) + 1
+            aouts = [((c.nx, c.nz, nb1), f64), ((c.nx, c.nz, c.n_sample), cd), ((c.nx, c.nz, c.n_fw_corr()), f32),
+                     ((nr, c.n_corr), f64)]
+            # mode 1: point sources at the receivers, source and structure kernels, stf_fft
+            cases.append(("adjoint", c, (mu_d, rho_d, C, stf, rec_d, rec_d, spectrum, None, G, 1, None, None),
+                          aouts[:2] + [None, aouts[3]]))
+            if q == 0:
+                # mode 0 with the distributed source of a mode-1 run and the Green snapshots
+                K1 = D.empty_like_dims((c.nx, c.nz, nb1), f64)
+                a1 = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
+                D.run_noise_adjoint(mu_d, rho_d, C, stf, rec_d, rec_d, None, None, None, 1, None, None, config=c,
+                                    K=K1, stf_fft=a1)
+                cases.append(("adjoint", c, (mu_d, rho_d, Gout, a1, rec_d, rec_d, spectrum, sd, None, 0, None, None),
+                              [aouts[0], None, None, aouts[3]]))
+                # mode 13: saved snapshots too
+                cases.append(("adjoint", c, (mu_d, rho_d, C, stf, rec_d, rec_d, spectrum, None, G, 13, None, None),
+                              aouts))
+                if c is c1:
+                    dmu = D.to_device(0.05 * mu)
+                    Gin = D.empty_like_dims((c.nx, c.nz, c.nt), f32)
+                    cg = c.replace(fw_nth=1)
+                    D.run_forward1_green(mu_d, rho_d, src_d, rec_d, 1, None, None, config=cg,
+                                         G_fft=D.empty_like_dims((c.nx, c.nz, c.n_sample), cd), G_out=Gin)
+                    cases.append(("green", cg, (mu_d, rho_d, src_d, rec_d, 0, dmu, Gin), gouts[:1] + [None, gouts[2]]))
+                    Cin = D.empty_like_dims((c.nx, c.nz, cg.n_fw_corr()), f32)
+                    D.run_forward2_correlation(mu_d, rho_d, G, spectrum, sd, rec_d, 1, None, None, config=cg,
+                                               seismograms=D.empty_like_dims((nr, c.n_corr), f64), C_out=Cin)
+                    cases.append(("correlation", cg, (mu_d, rho_d, G, spectrum, sd, rec_d, 0, dmu, Cin),
+                                  [couts[0], None]))
+    return cases
+
+
+def _outs(torch, dims):
+    from fd2d_noise_amd import device as D
+    return [None if d is None else D.empty_like_dims(d[0], d[1]) for d in dims]
+
+
+_NAMES = {"green": ("G_fft", "G_out", "seismograms"), "correlation": ("seismograms", "C_out"),
+          "adjoint": ("K", "stf_fft", "u_out", "seismograms")}
+
+
+def _alone(kind, c, args, outs):
+    from fd2d_noise_amd import device as D
+    fn = {"green": D.run_forward1_green, "correlation": D.run_forward2_correlation,
+          "adjoint": D.run_noise_adjoint}[kind]
+    fn(*args, config=c, **dict(zip(_NAMES[kind], outs)))
+
+
+def _spec(kind, c, args, outs):
+    from fd2d_noise_amd import batch as B
+    fn = {"green": B.green, "correlation": B.correlation, "adjoint": B.adjoint}[kind]
+    return fn(*args, config=c, **dict(zip(_NAMES[kind], outs)))
+
+
+def _same(torch, tag, ref, got):
+    for q, (x, y) in enumerate(zip(ref, got)):
+        if x is None:
+            continue
+        assert torch.isfinite(torch.view_as_real(x) if x.is_complex() else x).all(), f"{tag}[{q}] not finite"
+        same = torch.equal(x, y)
+        print(f"{tag}[{q}] {tuple(x.shape)}: {'identical' if same else 'DIFFERENT'}", flush=True)
+        assert same, f"{tag}[{q}]: batched result differs from the single solve"
+
+
+@pytest.fixture(scope="module")
+def mix(torch_mod):
+    torch = torch_mod
+    cases = _cases(torch)
+    ref = []
+    for kind, c, args, dims in cases:
+        o = _outs(torch, dims)
+        _alone(kind, c, args, o)
+        ref.append(o)
+    torch.cuda.synchronize()
+    return cases, ref
+
+
+def test_run_batch_matches_single_solves(torch_mod, mix):
+    torch = torch_mod
+    from fd2d_noise_amd import batch as B
+    cases, ref = mix
+    got = [_outs(torch, d) for _, _, _, d in cases]
+    specs = [_spec(k, c, a, o) for (k, c, a, _), o in zip(cases, got)]
+    print(f"{len(specs)} solves in one batch", flush=True)
+    B.run_batch(specs)
+    torch.cuda.synchronize()
+    for q, (kind, c, args, _) in enumerate(cases):
+        _same(torch, f"{kind} {c.nx}x{c.nz} order {c.order} mode {args[-3 if kind != 'green' else 4]}",
+              ref[q], got[q])
+
+
+def test_engine_continuous_batching(torch_mod, mix):
+    """Solves joining while others run (three waves, advanced by uneven round counts)."""
+    torch = torch_mod
+    from fd2d_noise_amd import batch as B
+    cases, ref = mix
+    got = [_outs(torch, d) for _, _, _, d in cases]
+    specs = [_spec(k, c, a, o) for (k, c, a, _), o in zip(cases, got)]
+    waves = [specs[0::3], specs[1::3], specs[2::3]]
+    with B.Engine() as eng:
+        tickets = [eng.submit(s) for s in waves[0]]
+        eng.advance(37)
+        tickets += [eng.submit(s) for s in waves[1]]
+        eng.advance(101)
+        assert all(not eng.done(t) for t in tickets[len(waves[0]):])
+        tickets += [eng.submit(s) for s in waves[2]]
+        eng.run_until_done()
+        assert all(eng.done(t) for t in tickets) and eng.active == 0
+    order = list(range(0, len(specs), 3)) + list(range(1, len(specs), 3)) + list(range(2, len(specs), 3))
+    torch.cuda.synchronize()
+    for q in order:
+        kind, c, _, _ = cases[q]
+        _same(torch, f"wave case {q} {kind}", ref[q], got[q])
+
+
+def test_engine_rejects_bad_solves(torch_mod, mix):
+    torch = torch_mod
+    from fd2d_noise_amd import batch as B, device as D
+    cases, _ = mix
+    kind, c, args, dims = cases[0]
+    bad = list(args)
+    bad[0] = D.to_device(np.ones((c.nx - 1, c.nz)))  # mu of the wrong size
+    with B.Engine() as eng:
+        with pytest.raises(abi.Fd2dError) as e:
+            eng.submit(_spec(kind, c, tuple(bad), _outs(torch, dims)))
+        assert e.value.code == abi.FD2D_EINVAL
+        assert eng.advance(5) == 0
+        with pytest.raises(abi.Fd2dError):
+            eng.done(12345)
