@@ -40,7 +40,16 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nt", type=int, default=8000, help="time steps (8000 is the benchmark workload)")
+    ap.add_argument("--nt", type=int, default=None,
+                    help="time steps (default: the workload's, 8000 for the pipeline, 4000 for stations)")
+    ap.add_argument("--workload", choices=["pipeline", "stations"], default="pipeline",
+                    help="pipeline: the single large solve (the benchmark line); stations: the batched "
+                         "reference-station workload (1200x1200, order 4, 64 stations x 4 basis functions) "
+                         "through the batch engine against the plain single-solve path")
+    ap.add_argument("--stations", type=int, default=64, help="stations of the stations workload")
+    ap.add_argument("--window", type=int, default=8, help="stations in flight in the batch engine")
+    ap.add_argument("--plain", type=int, default=8, help="stations timed on the plain single-solve path")
+    ap.add_argument("--rounds", type=int, default=64, help="engine rounds between completion checks")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: up to 16 host cores)")
     ap.add_argument("--cpu-nt", type=int, default=16, help="nt of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -196,6 +205,93 @@ def cpu_baseline(w, nt_sample, threads):
                       f"sources), {el:.1f} s"}
 
 
+def stations_bench(args):
+    """Green + correlation per reference station on the shared 1200x1200 grid: the plain path (one
+    solve at a time) against the batch engine with `window` stations in flight, each station's
+    correlation submitted as soon as its Green run is done. Checks that the two give identical
+    correlation traces on the stations both ran."""
+    import torch
+    from fd2d_noise_amd import batch as B, device as D
+    from fd2d_noise_amd.workloads import config3_workload
+    w = config3_workload(nt=args.nt or 4000, n_stations=args.stations)
+    c = w["config"]
+    S, nr, P = len(w["stations"]), w["rec"].shape[0], c.nx * c.nz
+    cd, f64 = torch.complex128, torch.float64
+    print(f"# workload config3 {c.nx}x{c.nz} order {c.order} nt {c.nt} K {c.n_sample} nb {c.n_basis_fct} "
+          f"stations {S} receivers {nr}", file=sys.stderr, flush=True)
+    mu, rho, rec = D.to_device(w["mu"]), D.to_device(w["rho"]), D.to_device(w["rec"])
+    spec, sd = D.to_device(w["spectrum"]), D.to_device(w["source_dist"])
+    srcs = [D.to_device(w["stations"][i:i + 1]) for i in range(S)]
+    W = max(1, min(args.window, S))
+    G = [D.empty_like_dims((c.nx, c.nz, c.n_sample), cd) for _ in range(W)]
+    seis_b = [D.empty_like_dims((nr, c.n_corr), f64) for _ in range(S)]
+    n_plain = max(1, min(args.plain, S))
+    seis_p = [D.empty_like_dims((nr, c.n_corr), f64) for _ in range(n_plain)]
+
+    def plain(i, out):
+        D.run_forward1_green(mu, rho, srcs[i], None, 0, None, None, config=c, G_fft=G[0])
+        D.run_forward2_correlation(mu, rho, G[0], spec, sd, rec, 0, None, None, config=c, seismograms=out)
+
+    plain(0, seis_p[0])  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(n_plain):
+        plain(i, seis_p[i])
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / n_plain
+    print(f"# plain: {t_plain * 1e3:.1f} ms per station over {n_plain} stations", file=sys.stderr, flush=True)
+
+    eng = B.Engine()
+    pending, free, inflight, t_sub, lat = list(range(S)), list(range(W)), {}, {}, []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while pending or inflight:
+        while pending and free:
+            i, slot = pending.pop(0), free.pop()
+            t_sub[i] = time.perf_counter()
+            spec_g = B.green(mu, rho, srcs[i], None, 0, None, None, config=c, G_fft=G[slot])
+            inflight[eng.submit(spec_g)] = ("green", i, slot)
+        eng.advance(args.rounds)
+        for t in [t for t in inflight if eng.done(t)]:
+            stage, i, slot = inflight.pop(t)
+            if stage == "green":
+                spec_c = B.correlation(mu, rho, G[slot], spec, sd, rec, 0, None, None, config=c,
+                                       seismograms=seis_b[i])
+                inflight[eng.submit(spec_c)] = ("correlation", i, slot)
+            else:
+                free.append(slot)
+                lat.append(time.perf_counter() - t_sub[i])
+                if len(lat) % 8 == 0:
+                    print(f"# batched: {len(lat)} stations done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    eng.close()
+    same = all(torch.equal(seis_p[i], seis_b[i]) for i in range(n_plain))
+    gp = S * P * (c.nt + c.n_corr)
+    alg = S * P * (alg_bytes("green", 10 ** 9) * c.nt + alg_bytes("correlation", 10 ** 9) * c.n_corr)
+    lat_ms = np.array(lat) * 1e3
+    out = {
+        "metric": "Gpt-updates/s, correlations+kernels/s, p99 latency; FP32 vs plain, L2<=1e-5",
+        "value": gp / el, "unit": "GP-updates/s", "n_gpus": 1, "steps": S, "warmup": 1,
+        "ms_per_step": el / S * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 fields, f64 DFT accumulators", "data": "synthetic",
+        "config": {"workload": f"config3: {S} reference stations x {c.n_basis_fct} basis functions, "
+                               f"{c.nx}x{c.nz} order-{c.order} nt={c.nt}, {c.n_sample} frequencies, {nr} receivers, "
+                               f"Green+correlation per station, {W} stations in flight",
+                   "nx": c.nx, "nz": c.nz, "order": c.order, "nt": c.nt, "n_sample": c.n_sample,
+                   "stations": S, "window": W, "rounds_per_check": args.rounds},
+        "correlations_per_s": S / el, "plain_correlations_per_s": 1.0 / t_plain,
+        "batched_vs_plain": t_plain * S / el, "plain_gp_updates_per_s": P * (c.nt + c.n_corr) / t_plain,
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "identical_to_plain": bool(same), "stations_compared": n_plain,
+        "roofline_whole_job": {"bound": "hbm", "achieved": round(alg / el / 1e9, 1), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(alg / el / 1e9 / HBM_PEAK_GBS, 4),
+                               "note": "algorithmic step bytes (32 B Green, 40 B correlation per GP-update) "
+                                       "over the whole job's wall time, spectral work included"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -224,7 +320,10 @@ def main():
         el = time.perf_counter() - t0
         print(json.dumps({"copy_bytes_per_launch": 2 * 4 * n, "GBps": 2 * 4 * n * args.steps / el / 1e9}))
         return
-    w = config2_workload(nt=args.nt)
+    if args.workload == "stations":
+        stations_bench(args)
+        return
+    w = config2_workload(nt=args.nt or 8000)
     c = w["config"]
     print(f"# workload config2 {c.nx}x{c.nz} order {c.order} nt {c.nt} fw_nth {c.fw_nth} K {c.n_sample}",
           file=sys.stderr, flush=True)

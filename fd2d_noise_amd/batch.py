@@ -56,7 +56,8 @@ def _bind():
             ("fd2d_engine_done", [vp, i64, ctypes.POINTER(i32)]),
             ("fd2d_engine_destroy", [vp]),
             ("fd2d_run_batch_device", [ctypes.POINTER(CSolve), i32, vp]),
-            ("fd2d_solve_device_bytes", [ctypes.POINTER(CSolve), ctypes.POINTER(i64)])):
+            ("fd2d_solve_device_bytes", [ctypes.POINTER(CSolve), ctypes.POINTER(i64)]),
+            ("fd2d_release_cached_memory", [])):
         f = getattr(lib, name)
         f.argtypes = at
         f.restype = ctypes.c_int
@@ -85,6 +86,13 @@ class SolveSpec:
         self.c = s
         self.outputs = list(outputs)
 
+    def run_plain(self) -> None:
+        """Runs this solve alone through its own *_device entry point (device.py)."""
+        from . import device as D
+        fn = {abi.FD2D_GREEN: D.run_forward1_green, abi.FD2D_CORRELATION: D.run_forward2_correlation,
+              abi.FD2D_ADJOINT: D.run_noise_adjoint}[self.solver]
+        fn(*self.call_args, config=self.config, **self.call_outputs)
+
     def device_bytes(self) -> int:
         """Upper bound of the device memory the library allocates for this solve."""
         b = ctypes.c_int64(0)
@@ -92,26 +100,82 @@ class SolveSpec:
         return int(b.value)
 
 
+def _with_call(spec: SolveSpec, args, outputs: dict) -> SolveSpec:
+    spec.call_args, spec.call_outputs = args, outputs
+    return spec
+
+
 def green(mu, rho, src, rec, mode, dmu, G_in, *, config: Config, G_fft, G_out=None, seismograms=None) -> SolveSpec:
     """run_forward1_green (code/run_forward1_green.m:1) as a batch item."""
-    return SolveSpec(abi.FD2D_GREEN, mode, config,
-                     dict(mu=mu, rho=rho, src=src, rec=rec, dmu=dmu, born_in=G_in), [G_fft, G_out, seismograms])
+    return _with_call(SolveSpec(abi.FD2D_GREEN, mode, config,
+                                dict(mu=mu, rho=rho, src=src, rec=rec, dmu=dmu, born_in=G_in),
+                                [G_fft, G_out, seismograms]),
+                      (mu, rho, src, rec, mode, dmu, G_in), dict(G_fft=G_fft, G_out=G_out, seismograms=seismograms))
 
 
 def correlation(mu, rho, G_fft, spectrum, source_dist, rec, mode, dmu, C_in, *, config: Config, seismograms=None,
                 C_out=None) -> SolveSpec:
     """run_forward2_correlation (code/run_forward2_correlation.m:1) as a batch item."""
-    return SolveSpec(abi.FD2D_CORRELATION, mode, config,
-                     dict(mu=mu, rho=rho, G_fft=G_fft, spectrum=spectrum, source_dist=source_dist, rec=rec, dmu=dmu,
-                          born_in=C_in), [seismograms, C_out])
+    return _with_call(SolveSpec(abi.FD2D_CORRELATION, mode, config,
+                                dict(mu=mu, rho=rho, G_fft=G_fft, spectrum=spectrum, source_dist=source_dist, rec=rec,
+                                     dmu=dmu, born_in=C_in), [seismograms, C_out]),
+                      (mu, rho, G_fft, spectrum, source_dist, rec, mode, dmu, C_in),
+                      dict(seismograms=seismograms, C_out=C_out))
 
 
 def adjoint(mu, rho, u_fwd, stf, src, rec, spectrum, source_dist, G_fft, mode, dmu, u_in, *, config: Config, K=None,
             stf_fft=None, u_out=None, seismograms=None) -> SolveSpec:
     """run_noise_adjoint (code/run_noise_adjoint.m:1) as a batch item."""
-    return SolveSpec(abi.FD2D_ADJOINT, mode, config,
-                     dict(mu=mu, rho=rho, u_fwd=u_fwd, stf=stf, src=src, rec=rec, spectrum=spectrum,
-                          source_dist=source_dist, G_fft=G_fft, dmu=dmu, born_in=u_in), [K, stf_fft, u_out, seismograms])
+    return _with_call(SolveSpec(abi.FD2D_ADJOINT, mode, config,
+                                dict(mu=mu, rho=rho, u_fwd=u_fwd, stf=stf, src=src, rec=rec, spectrum=spectrum,
+                                     source_dist=source_dist, G_fft=G_fft, dmu=dmu, born_in=u_in),
+                                [K, stf_fft, u_out, seismograms]),
+                      (mu, rho, u_fwd, stf, src, rec, spectrum, source_dist, G_fft, mode, dmu, u_in),
+                      dict(K=K, stf_fft=stf_fft, u_out=u_out, seismograms=seismograms))
+
+
+def solve_bytes(solver: int, config: Config, mode: float = 0.0, n_src: int = 0, n_rec: int = 0,
+                G_fft: bool = False, spectrum_cols: int = 1, source_dist: bool = False, stf_dims=None,
+                dmu: bool = False, fft_out: bool = False) -> int:
+    """Device bytes the library allocates for a solve of this shape (fd2d_solve_device_bytes on
+    size-only descriptors): the figure an admission policy charges before the arrays exist."""
+    c = config
+    keep = []
+
+    def desc(dims):
+        d = abi.CArray()
+        d.data = 1  # marks the array as given; only its sizes are read
+        d.dims[:] = tuple(int(x) for x in dims)
+        keep.append(d)
+        return ctypes.pointer(d)
+
+    s = CSolve()
+    s.solver, s.mode = solver, float(mode)
+    cc = c.to_c()
+    s.cfg = ctypes.cast(ctypes.pointer(cc), ctypes.c_void_p)
+    if n_src:
+        s.src = desc((n_src, 2, 1))
+    if n_rec:
+        s.rec = desc((n_rec, 2, 1))
+    if G_fft:
+        s.G_fft = desc((c.nx, c.nz, c.n_sample))
+    if spectrum_cols:
+        s.spectrum = desc((c.n_sample, spectrum_cols, 1))
+    if source_dist:
+        s.source_dist = desc((c.nx, c.nz, max(c.n_basis_fct, 1)))
+    if stf_dims is not None:
+        s.stf = desc(stf_dims)
+    if dmu:
+        s.dmu = desc((c.nx, c.nz, 1))
+    m = abi.MArray()
+    if fft_out or solver == abi.FD2D_GREEN:
+        m.data = 1
+        m.dims[:] = (c.nx, c.nz, c.n_sample)
+        keep.append(m)
+        s.out[1 if solver == abi.FD2D_ADJOINT else 0] = ctypes.pointer(m)
+    b = ctypes.c_int64(0)
+    _check(_bind().fd2d_solve_device_bytes(ctypes.byref(s), ctypes.byref(b)))
+    return int(b.value)
 
 
 def run_batch(specs: Sequence[SolveSpec]) -> None:

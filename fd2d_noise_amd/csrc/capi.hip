@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <atomic>
 #include <mutex>
@@ -47,11 +48,114 @@ struct Failure {
         if (e_ != hipSuccess) raise(FD2D_EDEVICE, "%s failed: %s", #x, hipGetErrorString(e_));     \
     } while (0)
 
-// Owning device buffer.
+// Stream of the work the current entry point enqueues (StreamScope); device buffers remember it so
+// a released block is known free once that stream has passed the release point.
+thread_local hipStream_t t_stream = nullptr;
+struct StreamScope {
+    hipStream_t prev;
+    explicit StreamScope(hipStream_t s) : prev(t_stream) { t_stream = s; }
+    ~StreamScope() { t_stream = prev; }
+    StreamScope(const StreamScope&) = delete;
+    StreamScope& operator=(const StreamScope&) = delete;
+};
+
+// Process-wide cache of released device blocks, per device. hipMalloc / hipFree of multi-GB
+// buffers cost tens of milliseconds each on a device whose memory other processes used before
+// (and hipFree waits for the whole device), and every solve allocates the same few sizes again, so
+// released blocks are kept and handed out again for requests of at least 2/3 of their size. A
+// released block carries an event recorded on the stream that used it; the next user waits on the
+// event, so a block still read by queued work is never handed out early. At most `limit` bytes are
+// kept (a quarter of the device memory); fd2d_release_cached_memory() frees them all.
+struct BlockCache {
+    struct Block {
+        void* p;
+        hipEvent_t ev;
+    };
+    std::mutex mu;
+    std::multimap<size_t, Block> blocks;
+    size_t held = 0, limit = 0;
+    bool limit_set = false;
+    void* take(size_t bytes, size_t* got)
+    {
+        Block blk{nullptr, nullptr};
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto it = blocks.lower_bound(bytes);
+            if (it == blocks.end() || it->first > bytes + bytes / 2) return nullptr;
+            blk = it->second;
+            *got = it->first;
+            held -= it->first;
+            blocks.erase(it);
+        }
+        if (blk.ev) {
+            (void)hipEventSynchronize(blk.ev);
+            (void)hipEventDestroy(blk.ev);
+        }
+        return blk.p;
+    }
+    void give(void* p, size_t bytes, hipStream_t st)
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        if (!limit_set) {
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) limit = total_b / 4;
+            limit_set = true;
+        }
+        if (held + bytes > limit) {
+            lk.unlock();
+            (void)hipFree(p);
+            return;
+        }
+        hipEvent_t ev = nullptr;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(ev, st) != hipSuccess) {
+            (void)hipGetLastError();
+            if (ev) (void)hipEventDestroy(ev);
+            lk.unlock();
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(p);
+            return;
+        }
+        blocks.emplace(bytes, Block{p, ev});
+        held += bytes;
+    }
+    void clear()
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& kv : blocks) {
+            if (kv.second.ev) {
+                (void)hipEventSynchronize(kv.second.ev);
+                (void)hipEventDestroy(kv.second.ev);
+            }
+            (void)hipFree(kv.second.p);
+        }
+        blocks.clear();
+        held = 0;
+    }
+};
+
+constexpr int kMaxDevices = 64;
+BlockCache g_cache[kMaxDevices];
+
+BlockCache* device_cache()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return &g_cache[dev];
+}
+
+// Owning device buffer: a block of the device's BlockCache, else hipMalloc. An upload keeps its host
+// copy (`stage`) until the buffer is released.
 template <class T>
 struct DBuf {
     T* p = nullptr;
-    size_t n = 0;
+    size_t n = 0, bytes = 0;
+    BlockCache* cache = nullptr;
+    hipStream_t st = nullptr;
+    std::vector<unsigned char> stage;
     DBuf() = default;
     explicit DBuf(size_t count) { alloc(count); }
     DBuf(const DBuf&) = delete;
@@ -60,18 +164,41 @@ struct DBuf {
     {
         release();
         if (count == 0) return;
-        if (hipMalloc((void**)&p, count * sizeof(T)) != hipSuccess) {
-            p = nullptr;
+        bytes = count * sizeof(T);
+        cache = device_cache();
+        st = t_stream;
+        if (cache) {
+            size_t got = 0;
+            if (void* q = cache->take(bytes, &got)) {
+                p = (T*)q;
+                bytes = got;
+                n = count;
+                return;
+            }
+        }
+        if (hipMalloc((void**)&p, bytes) != hipSuccess) {
             (void)hipGetLastError();
-            raise(FD2D_ENOMEM, "device allocation of %zu bytes failed", count * sizeof(T));
+            // memory held by the cache may be what is missing: return it and try once more
+            if (cache) cache->clear();
+            if (hipMalloc((void**)&p, bytes) != hipSuccess) {
+                p = nullptr;
+                (void)hipGetLastError();
+                raise(FD2D_ENOMEM, "device allocation of %zu bytes failed", bytes);
+            }
         }
         n = count;
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (cache) cache->give(p, bytes, st);
+            else (void)hipFree(p);
+        }
         p = nullptr;
-        n = 0;
+        n = bytes = 0;
+        cache = nullptr;
+        stage.clear();
+        stage.shrink_to_fit();
     }
     ~DBuf() { release(); }
 };
@@ -241,14 +368,16 @@ void taper(const fd2d_config* cfg, const Geometry& g, std::vector<float>& ax, st
     az.assign(azd.begin(), azd.end());
 }
 
-// Copies a host table to a new device buffer; waits for the copy so the host vector may go away.
+// Copies a host table to a new device buffer on stream st; the buffer keeps the host copy the
+// transfer reads until it is released, so the caller's vector may go away at once.
 template <class T>
 void upload(DBuf<T>& d, const std::vector<T>& h, hipStream_t st)
 {
     d.alloc(h.size());
     if (h.empty()) return;
-    HIPCHK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(h.data());
+    d.stage.assign(b, b + h.size() * sizeof(T));
+    HIPCHK(hipMemcpyAsync(d.p, d.stage.data(), d.stage.size(), hipMemcpyHostToDevice, st));
 }
 
 // Phase table e^{sign i w_k t_j} * scale for the listed times, rows j, columns k.
@@ -1090,6 +1219,7 @@ void green_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_carr
                   const fd2d_carray* rec, double mode, const fd2d_carray* dmu, const fd2d_carray* G_in,
                   fd2d_marray* G_fft, fd2d_marray* G_out, fd2d_marray* seis, hipStream_t st)
 {
+    StreamScope scope(st);
     GreenJob j(cfg, mu, rho, src, rec, mode, dmu, G_in, G_fft, G_out, seis, st);
     run_job(j);
 }
@@ -1099,6 +1229,7 @@ void correlation_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2
                         const fd2d_carray* dmu, const fd2d_carray* C_in, fd2d_marray* seis, fd2d_marray* C_out,
                         hipStream_t st)
 {
+    StreamScope scope(st);
     CorrelationJob j(cfg, mu, rho, G, spectrum, sd, rec, mode, dmu, C_in, seis, C_out, st);
     run_job(j);
 }
@@ -1109,6 +1240,7 @@ void adjoint_device(const fd2d_config* cfg, const fd2d_carray* mu, const fd2d_ca
                     const fd2d_carray* dmu, const fd2d_carray* u_in, fd2d_marray* K_out, fd2d_marray* stf_fft,
                     fd2d_marray* u_out, fd2d_marray* seis, hipStream_t st)
 {
+    StreamScope scope(st);
     AdjointJob j(cfg, mu, rho, u_fwd, stf, src, rec, spectrum, sd, G, mode, dmu, u_in, K_out, stf_fft, u_out, seis, st);
     run_job(j);
 }
@@ -1128,6 +1260,13 @@ struct Entry {
     bool has_out[4]{};
     std::unique_ptr<Job> job;
     int64_t ticket = 0;
+    int slot = -1;
+    StepArgs stat[2]{};             // host copy of the static arguments written to the slot
+    hipEvent_t done_ev = nullptr;   // recorded after the solve's last work
+    ~Entry()
+    {
+        if (done_ev) (void)hipEventDestroy(done_ev);
+    }
     const fd2d_carray* cin(int q) const { return has_in[q] ? &in[q] : nullptr; }
     fd2d_marray* mout(int q) { return has_out[q] ? &out[q] : nullptr; }
 };
@@ -1181,22 +1320,34 @@ std::unique_ptr<Entry> make_entry(const fd2d_solve* s, hipStream_t st)
     return e;
 }
 
+constexpr int kEngineSlots = 1024;  // solves an engine holds at once
+
 struct Engine {
     hipStream_t st = nullptr;
-    std::vector<std::unique_ptr<Entry>> act;
+    std::vector<std::unique_ptr<Entry>> act;       // running
+    std::vector<std::unique_ptr<Entry>> draining;  // finished, their last work still queued
+    std::vector<int64_t> done_tickets;             // finished and complete
     int64_t next_ticket = 1;
     bool dirty = true;
-    DBuf<StepArgs> statics;  // per grouped job: its StepArgs for parity 0 and 1
+    // Static StepArgs of every solve (both parities) at a fixed slot, written once at submission:
+    // grouped launches index it with the slot each JobStep carries.
+    DBuf<StepArgs> slots_d;
+    std::vector<int> free_slots;
     struct Group {
         int R = 0;
         bool with_u = false;
-        int first = 0;             // index of the first member's pair in statics
         std::vector<Entry*> members;
     };
     std::vector<Group> groups;
     std::vector<Entry*> singles;
 
-    // Re-partition the active jobs after the set changed (stream already idle).
+    explicit Engine(hipStream_t s) : st(s)
+    {
+        StreamScope scope(st);
+        slots_d.alloc((size_t)2 * kEngineSlots);
+        for (int q = kEngineSlots - 1; q >= 0; --q) free_slots.push_back(q);
+    }
+    // Re-partition the running solves after the set changed (host work only).
     void rebuild()
     {
         groups.clear();
@@ -1210,7 +1361,6 @@ struct Engine {
         }
         std::stable_sort(cls.begin(), cls.end(),
                          [](const auto& x, const auto& y) { return x.first < y.first; });
-        std::vector<StepArgs> h;
         for (size_t q = 0; q < cls.size();) {
             size_t r = q;
             while (r < cls.size() && cls[r].first == cls[q].first && r - q < (size_t)kGroupMax) ++r;
@@ -1220,19 +1370,11 @@ struct Engine {
                 Group gr;
                 gr.R = cls[q].first.first;
                 gr.with_u = cls[q].first.second != 0;
-                gr.first = (int)(h.size() / 2);
-                for (size_t k = q; k < r; ++k) {
-                    Job& j = *cls[k].second->job;
-                    h.push_back(j.static_args(0));
-                    h.push_back(j.static_args(1));
-                    gr.members.push_back(cls[k].second);
-                }
+                for (size_t k = q; k < r; ++k) gr.members.push_back(cls[k].second);
                 groups.push_back(std::move(gr));
             }
             q = r;
         }
-        if (h.empty()) statics.release();
-        else upload(statics, h, st);
         dirty = false;
     }
     void step_single(Job& j)
@@ -1246,17 +1388,18 @@ struct Engine {
     void step_group(const Group& gr)
     {
         GroupArgs ga{};
-        ga.jobs = statics.p + 2 * (size_t)gr.first;
+        ga.jobs = slots_d.p;
         ga.n_jobs = (int)gr.members.size();
         int total = 0;
         for (int q = 0; q < ga.n_jobs; ++q) {
-            Job& j = *gr.members[q]->job;
+            Entry& e = *gr.members[q];
+            Job& j = *e.job;
             j.S->clear_step();
             j.plan(j.next, 0);
             const StepArgs& a = j.S->a;
             ga.tile_start[q] = total;
             total += j.S->n_tiles;
-            ga.step[q] = JobStep{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.ps_const, j.S->cur};
+            ga.step[q] = JobStep{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.ps_const, j.S->cur, e.slot};
         }
         ga.tile_start[ga.n_jobs] = total;
         ga.total_tiles = total;
@@ -1269,27 +1412,48 @@ struct Engine {
             ++j.next;
         }
     }
-    // Finish the solves that have taken their last step and drop them; true if any.
-    bool retire()
+    // Drop the solves whose last work has completed on the device.
+    void reap()
+    {
+        for (auto& e : draining) {
+            const hipError_t q = hipEventQuery(e->done_ev);
+            if (q == hipErrorNotReady) continue;
+            HIPCHK(q);
+            done_tickets.push_back(e->ticket);
+            free_slots.push_back(e->slot);
+            e.reset();
+        }
+        draining.erase(std::remove(draining.begin(), draining.end(), nullptr), draining.end());
+    }
+    // Finish the solves that have taken their last step: their final work is queued and an event
+    // marks its completion; no host wait.
+    void retire()
     {
         bool any = false;
         for (auto& e : act)
             if (e->job->next > e->job->nsteps) {
                 e->job->finish();
+                HIPCHK(hipEventCreateWithFlags(&e->done_ev, hipEventDisableTiming));
+                HIPCHK(hipEventRecord(e->done_ev, st));
+                draining.push_back(std::move(e));
                 any = true;
             }
-        if (!any) return false;
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(st));
-        act.erase(std::remove_if(act.begin(), act.end(),
-                                 [](const std::unique_ptr<Entry>& e) { return e->job->next > e->job->nsteps; }),
-                  act.end());
-        dirty = true;
-        return true;
+        if (any) {
+            act.erase(std::remove(act.begin(), act.end(), nullptr), act.end());
+            dirty = true;
+        }
+        reap();
     }
     int64_t submit(const fd2d_solve* s)
     {
+        StreamScope scope(st);
+        if (free_slots.empty()) raise(FD2D_ENOMEM, "the engine holds %d solves already", kEngineSlots);
         std::unique_ptr<Entry> e = make_entry(s, st);
+        e->slot = free_slots.back();
+        e->stat[0] = e->job->static_args(0);
+        e->stat[1] = e->job->static_args(1);
+        HIPCHK(hipMemcpyAsync(slots_d.p + 2 * (size_t)e->slot, e->stat, sizeof e->stat, hipMemcpyHostToDevice, st));
+        free_slots.pop_back();
         e->ticket = next_ticket++;
         act.push_back(std::move(e));
         dirty = true;
@@ -1297,28 +1461,35 @@ struct Engine {
     }
     void advance(int rounds)
     {
+        StreamScope scope(st);
         for (int r = 0; r < rounds && !act.empty(); ++r) {
             retire();
             if (act.empty()) break;
-            if (dirty) {
-                HIPCHK(hipStreamSynchronize(st));
-                rebuild();
-            }
+            if (dirty) rebuild();
             for (const Group& gr : groups) step_group(gr);
             for (Entry* e : singles) step_single(*e->job);
         }
         retire();
         HIPCHK(hipGetLastError());
+        if (act.empty() && !draining.empty()) {  // nothing left to overlap with: wait for the rest
+            HIPCHK(hipStreamSynchronize(st));
+            reap();
+        }
     }
-    bool active(int64_t ticket) const
+    // solves submitted and not yet complete
+    int pending() const { return (int)(act.size() + draining.size()); }
+    bool is_done(int64_t ticket)
     {
-        for (const auto& e : act)
-            if (e->ticket == ticket) return true;
-        return false;
+        reap();
+        return std::find(done_tickets.begin(), done_tickets.end(), ticket) != done_tickets.end();
     }
     ~Engine()
     {
-        if (!act.empty()) (void)hipStreamSynchronize(st);
+        (void)hipStreamSynchronize(st);
+        StreamScope scope(st);
+        act.clear();
+        draining.clear();
+        slots_d.release();
     }
 };
 
@@ -1489,6 +1660,7 @@ void blur_maps(const Geometry& g, const MapFrame& f, const double* src, double* 
 
 struct fd2d_engine {
     Engine e;
+    explicit fd2d_engine(hipStream_t s) : e(s) {}
 };
 
 extern "C" {
@@ -1505,13 +1677,20 @@ int fd2d_set_steps_per_launch(int32_t steps)
     return FD2D_OK;
 }
 
+int fd2d_release_cached_memory(void)
+{
+    return guarded([&] {
+        HIPCHK(hipDeviceSynchronize());
+        for (BlockCache& c : g_cache) c.clear();
+    });
+}
+
 int fd2d_engine_create(void* stream, fd2d_engine** engine)
 {
     return guarded([&] {
         if (!engine) raise(FD2D_EINVAL, "engine is NULL");
         require_device();
-        std::unique_ptr<fd2d_engine> e(new fd2d_engine);
-        e->e.st = as_stream(stream);
+        std::unique_ptr<fd2d_engine> e(new fd2d_engine(as_stream(stream)));
         *engine = e.release();
     });
 }
@@ -1531,7 +1710,7 @@ int fd2d_engine_advance(fd2d_engine* engine, int32_t rounds, int32_t* n_active)
         if (!engine) raise(FD2D_EINVAL, "engine is NULL");
         if (rounds < 0) raise(FD2D_EINVAL, "rounds must be >= 0");
         engine->e.advance(rounds);
-        if (n_active) *n_active = (int32_t)engine->e.act.size();
+        if (n_active) *n_active = (int32_t)engine->e.pending();
     });
 }
 
@@ -1540,7 +1719,7 @@ int fd2d_engine_done(const fd2d_engine* engine, int64_t ticket, int32_t* done)
     return guarded([&] {
         if (!engine || !done) raise(FD2D_EINVAL, "engine and done must not be NULL");
         if (ticket < 1 || ticket >= engine->e.next_ticket) raise(FD2D_EINVAL, "unknown ticket %lld", (long long)ticket);
-        *done = engine->e.active(ticket) ? 0 : 1;
+        *done = const_cast<fd2d_engine*>(engine)->e.is_done(ticket) ? 1 : 0;
     });
 }
 
@@ -1554,10 +1733,10 @@ int fd2d_run_batch_device(const fd2d_solve* solves, int32_t n, void* stream)
     return guarded([&] {
         if (n < 0 || (n > 0 && !solves)) raise(FD2D_EINVAL, "solves is NULL");
         require_device();
-        fd2d_engine e;
-        e.e.st = as_stream(stream);
-        for (int32_t q = 0; q < n; ++q) e.e.submit(&solves[q]);
-        while (!e.e.act.empty()) e.e.advance(1 << 20);
+        Engine e(as_stream(stream));
+        for (int32_t q = 0; q < n; ++q) e.submit(&solves[q]);
+        while (!e.act.empty()) e.advance(1 << 20);
+        HIPCHK(hipStreamSynchronize(e.st));
     });
 }
 
@@ -1700,6 +1879,7 @@ int fd2d_make_adjoint_sources_device(const double* u, const double* u0, const do
             raise(FD2D_EUNSUPPORTED, "no Hessian vector products for traveltime measurements (make_adjoint_sources.m:121)");
         if (deriv_order == 2 && !du) raise(FD2D_EINVAL, "deriv_order 2 needs du");
         hipStream_t st = as_stream(stream);
+        StreamScope scope(st);
         // make_adjoint_sources.m:85-99: window limits against the time axis
         const std::vector<double> th = to_host<double>(t, (size_t)nt);
         double left = win_left, right = win_right;
@@ -1755,6 +1935,7 @@ int fd2d_dot_device(const double* x, const double* y, int64_t n, double* result,
         require_device();
         if (!result || n < 0 || (n > 0 && (!x || !y))) raise(FD2D_EINVAL, "dot: null argument");
         hipStream_t st = as_stream(stream);
+        StreamScope scope(st);
         const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
         DBuf<double> part((size_t)2 * nb + 1);
         launch_dot(x, y, n, part.p, nb, part.p + 2 * nb, st);
@@ -1770,6 +1951,7 @@ int fd2d_axpby_device(double alpha, const double* x, double beta, const double* 
         require_device();
         if (n < 0 || (n > 0 && (!x || !y || !out))) raise(FD2D_EINVAL, "axpby: null argument");
         hipStream_t st = as_stream(stream);
+        StreamScope scope(st);
         if (n) launch_axpby(alpha, x, beta, y, out, n, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
@@ -1782,6 +1964,7 @@ int fd2d_axpby_f32_device(float alpha, const float* x, float beta, const float* 
         require_device();
         if (n < 0 || (n > 0 && (!x || !y || !out))) raise(FD2D_EINVAL, "axpby_f32: null argument");
         hipStream_t st = as_stream(stream);
+        StreamScope scope(st);
         if (n) launch_axpby_f32(alpha, x, beta, y, out, n, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
@@ -1797,6 +1980,7 @@ int fd2d_gaussblur2d_device(const double* Y, int32_t n1, int32_t n2, int32_t ldy
             raise(FD2D_EINVAL, "gaussblur2d: bad arguments");
         if (!(sigma1 > 0) || !(sigma2 > 0)) raise(FD2D_EINVAL, "gaussblur2d: sigma must be positive");
         hipStream_t st = as_stream(stream);
+        StreamScope scope(st);
         DBuf<double> tmp((size_t)n1 * n2);
         launch_gaussblur(Y, ldy, GY, ldg, tmp.p, x1, x2, n1, n2, sigma1, sigma2, st);
         HIPCHK(hipGetLastError());
@@ -1820,6 +2004,7 @@ int fd2d_general_source_device(const double* spectrum, int32_t n_sample, int32_t
             hi[(size_t)b - 1] = (int)std::floor(b * step) - 1;
         }
         hipStream_t st = as_stream(stream);
+        StreamScope scope(st);
         DBuf<int> lo_d, hi_d;
         upload(lo_d, lo, st);
         upload(hi_d, hi, st);
@@ -1842,6 +2027,7 @@ int fd2d_map_m_to_parameters_device(const fd2d_config* cfg, const double* m, int
         check_sigma(sigma_structure, "sigma_structure");
         const MapFrame f = map_frame(cfg, g);
         hipStream_t st = as_stream(stream);
+        StreamScope scope(st);
         const size_t n = g.P * (size_t)n_maps;
         // outside the frame the maps pass through unchanged (map_m_to_parameters.m:16)
         if (m_parameters != m) HIPCHK(hipMemcpyAsync(m_parameters, m, n * 8, hipMemcpyDeviceToDevice, st));
@@ -1868,6 +2054,7 @@ int fd2d_map_gradparameters_to_gradm_device(const fd2d_config* cfg, const double
         check_sigma(sigma_structure, "sigma_structure");
         const MapFrame f = map_frame(cfg, g);
         hipStream_t st = as_stream(stream);
+        StreamScope scope(st);
         const int64_t n = (int64_t)(g.P * (size_t)n_maps);
         // temp = 0.0 * grad_parameters (map_gradparameters_to_gradm.m:70), then the blurred frame
         launch_axpby(0.0, grad_parameters, 0.0, grad_parameters, grad_m, n, st);
@@ -1889,6 +2076,7 @@ int fd2d_regularization_device(double* j, double* g, const double* m, const doub
         if (n_src % n_w != 0 || (n_w != 1 && n_w != n_struct))
             raise(FD2D_EINVAL, "weighting must hold 1 or nx*nz values (regularization.m:115)");
         hipStream_t st = as_stream(stream);
+        StreamScope scope(st);
         const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
         DBuf<double> part((size_t)4 * nb + 2);
         launch_regularization(m, m0, w, n_w, n, n_src, alpha, beta, g, part.p, nb, part.p + 4 * nb, st);

@@ -128,13 +128,14 @@ struct JobStep {
     double* seis_col;
     float ps_const;
     int parity;  // which of the solve's two StepArgs (ping-pong direction) this step uses
+    int slot;    // the solve's StepArgs are jobs[2 * slot + parity] of the grouped launch
 };
 
 constexpr int kGroupMax = 48;  // solves per grouped launch (kernel arguments stay under 4 KB)
 
 // One time step of up to kGroupMax solves with the same order and field set in one launch: the
 // tiles of all solves are concatenated (solve j owns tiles tile_start[j] .. tile_start[j+1]-1)
-// and dealt to the XCDs in contiguous bands. jobs[2j + parity] holds solve j's StepArgs.
+// and dealt to the XCDs in contiguous bands. Solve j's StepArgs are jobs[2 * step[j].slot + parity].
 struct GroupArgs {
     const StepArgs* jobs;
     int n_jobs;

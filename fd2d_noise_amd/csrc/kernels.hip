@@ -58,6 +58,25 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned off, f
 }
 constexpr unsigned kOutside = 0x80000000u;  // a buffer offset past every compact array
 
+// Loads and stores through global-address-space pointers: the argument structs of a grouped
+// launch are read from memory, so the compiler cannot infer the address space of the pointers
+// they hold and would otherwise emit flat accesses.
+template <class T>
+__device__ __forceinline__ T gld(const T* p, int i)
+{
+    return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+__device__ __forceinline__ int2 gld(const int2* p, int i)
+{
+    const long long v = ((const __attribute__((address_space(1))) long long*)p)[i];
+    return make_int2((int)v, (int)(v >> 32));
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, int i, T v)
+{
+    ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+
 // Addressing shared by the step kernels.
 struct Grid {
     int nx, nz, pitch;
@@ -147,7 +166,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         hv[k] = G.field(a.o_v_in, xx, zz);
         hb[k] = G.coef(a.o_bv, xx, zz);
         hsy[k] = has_syn ? bload(r_syn, G.cofs(xx, zz)) : 0.f;
-        ht[k] = a.axp[xx + kGhostX] * a.azp[zz + kGhost];
+        ht[k] = gld(a.axp, xx + kGhostX) * gld(a.azp, zz + kGhost);
     }
     // LDS stores
 #pragma unroll
@@ -175,8 +194,8 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     __syncthreads();
 
     // ---------------- phase 2
-    const int e0 = a.ps_off ? a.ps_off[t] : 0, e1 = a.ps_off ? a.ps_off[t + 1] : 0;
-    const int f0 = a.rec_off ? a.rec_off[t] : 0, f1 = a.rec_off ? a.rec_off[t + 1] : 0;
+    const int e0 = a.ps_off ? gld(a.ps_off, t) : 0, e1 = a.ps_off ? gld(a.ps_off, t + 1) : 0;
+    const int f0 = a.rec_off ? gld(a.rec_off, t) : 0, f1 = a.rec_off ? gld(a.rec_off, t + 1) : 0;
     // div_s at region cell (r, c) = node (xx, zz) from the staged sigma, with the frame factors
     // fx (x-part, R <= x <= nx-R-1) and fz (z-part, R <= z <= nz-R-1), plus the point sources
     auto divs = [&](int xx, int zz, int r, int c, float fx, float fz) {
@@ -192,22 +211,22 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         if (e1 > e0) {
             const int key = node_key(xx, zz);
             for (int e = e0; e < e1; ++e) {
-                const int2 en = a.ps_ent[e];
-                if (en.x == key) ds += st.ps_amp ? st.ps_amp[en.y] : st.ps_const;
+                const int2 en = gld(a.ps_ent, e);
+                if (en.x == key) ds += st.ps_amp ? gld(st.ps_amp, en.y) : st.ps_const;
             }
         }
         return ds;
     };
     auto frame = [](int v, int lo, int hi) { return (v >= lo && v <= hi) ? 1.f : 0.f; };
     const float fxl = frame(x, R, nx - R - 1);
-    const float axl = a.axp[x + kGhostX];
+    const float axl = gld(a.axp, x + kGhostX);
 #pragma unroll
     for (int k = 0; k < NRV; ++k) {
         const int r = w + 4 * k;
         if (r >= HV) break;
         const int z = j0 - (R - 1) + r, c = lane + R - 1;
         const float ds = divs(x, z, r, c, fxl, frame(z, R, nz - R - 1)) + msy[k];
-        const float vn = (mv[k] + mb[k] * ds) * (axl * a.azp[z + kGhost]);
+        const float vn = (mv[k] + mb[k] * ds) * (axl * gld(a.azp, z + kGhost));
         if (r >= R - 1 && r < R - 1 + TZ) {  // own node: outputs (zeros outside the grid)
             G.put(a.o_v_out, x, z, vn);
             float out = vn;
@@ -224,8 +243,8 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
             if (f1 > f0 && st.seis_col) {
                 const int key = node_key(x, z);
                 for (int e = f0; e < f1; ++e) {
-                    const int2 en = a.rec_ent[e];
-                    if (en.x == key) st.seis_col[en.y] = (double)out;
+                    const int2 en = gld(a.rec_ent, e);
+                    if (en.x == key) gst(st.seis_col, en.y, (double)out);
                 }
             }
         }
@@ -293,7 +312,7 @@ template <int R, int MODE>
 __global__ __launch_bounds__(256) void k_tile(StepArgs a)
 {
     const int g = blockIdx.x;
-    const JobStep st{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.ps_const, 0};
+    const JobStep st{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.ps_const, 0, 0};
     tile_step<R, MODE>(a, st, (g % kXCD) * a.tiles_per_xcd + g / kXCD);
 }
 
@@ -305,8 +324,12 @@ __global__ __launch_bounds__(256) void k_tile_group(GroupArgs G)
     if (T >= G.total_tiles) return;
     int j = 0;
     while (j + 1 < G.n_jobs && G.tile_start[j + 1] <= T) ++j;  // wave-uniform scan
-    const JobStep& st = G.step[j];
-    tile_step<R, MODE>(G.jobs[2 * j + st.parity], st, T - G.tile_start[j]);
+    j = __builtin_amdgcn_readfirstlane(j);
+    // by-value copies: the step body then reads its arguments from registers, not through
+    // pointers that the compiler must assume its own stores may alias
+    const JobStep st = G.step[j];
+    const StepArgs a = G.jobs[2 * st.slot + st.parity];
+    tile_step<R, MODE>(a, st, T - G.tile_start[j]);
 }
 
 // ---------------------------------------------------------------------------------------------

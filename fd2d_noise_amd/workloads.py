@@ -10,6 +10,10 @@ config2: the single large solve -- 2400 x 2400 nodes on 2000 km (dx = 0.834 km),
          (30 km correlation length, max |g| = 1), rho constant, forward fields stored every 4th step,
          the default 121 frequencies (0.02:0.0015:0.2 Hz), 16 receivers 400-800 km from the centre
          station and a cross-correlation traveltime misfit against the homogeneous model's traces.
+config3: batched reference stations -- 1200 x 1200 nodes on 2000 km (dx = 1.67 km), order 4,
+         dt = 0.12 s, nt = 4000, 128 frequencies (0.01-0.2 Hz, every 5th step), 4 source basis
+         functions (seeded blobs, flat spectrum as after general_source), 64 reference stations
+         seeded uniformly inside the absorbing frame and 32 receivers shared by all stations.
 The taper is the reference's Gaussian taper (init_absbound.m:18).
 """
 from __future__ import annotations
@@ -78,3 +82,20 @@ def config2_workload(seed: int = 2, nt: int = 8000) -> dict:
         src=centre[None, :], rec=_ring(rng, centre, 16, 400e3, 800e3),
         spectrum=np.exp(-((f - 0.1) ** 2) / (2 * 0.03 ** 2))[:, None],
         source_dist=_blob(rng, c))
+
+
+def config3_workload(seed: int = 3, nt: int = 4000, n_stations: int = 64) -> dict:
+    rng = np.random.default_rng(seed)
+    c = Config(Lx=2.0e6, Lz=2.0e6, nx=1200, nz=1200, dt=0.12, nt=nt, order=4, fw_nth=4, n_basis_fct=4,
+               f_sample=np.linspace(0.01, 0.2, 128), freq_samp=5)
+    c.absorb_width = 30 * c.dx
+    w = c.absorb_width
+    inside = lambda n: np.stack([rng.uniform(w, c.Lx - w, n), rng.uniform(w, c.Lz - w, n)], axis=1)
+    stations = inside(n_stations)
+    rec = inside(32)
+    basis = np.stack([_blob(rng, c, sigma=250e3, background=0.05) for _ in range(c.n_basis_fct)], axis=2)
+    return dict(
+        name="config3", config=c,
+        mu=MU0 * np.ones((c.nx, c.nz), order="F"), rho=RHO0 * np.ones((c.nx, c.nz), order="F"),
+        stations=stations, rec=rec, spectrum=np.ones((c.n_sample, 1)),
+        source_dist=np.asfortranarray(basis))

@@ -318,10 +318,13 @@ typedef struct fd2d_solve {
  */
 typedef struct fd2d_engine fd2d_engine;
 
+/* The stream must outlive the engine. */
 int fd2d_engine_create(void* stream, fd2d_engine** engine);
 /* Validates and sets up a solve (its device memory is allocated here); *ticket names it. */
 int fd2d_engine_submit(fd2d_engine* engine, const fd2d_solve* solve, int64_t* ticket);
-/* Runs up to `rounds` rounds; *n_active (may be NULL) = solves still running afterwards. */
+/* Runs up to `rounds` rounds; *n_active (may be NULL) = solves not yet complete afterwards. When no
+ * solve is left running it waits for the finished ones, so advancing until *n_active == 0 leaves
+ * every output complete. */
 int fd2d_engine_advance(fd2d_engine* engine, int32_t rounds, int32_t* n_active);
 /* *done = 1 when the solve's outputs are complete (it has finished and the stream is synchronised). */
 int fd2d_engine_done(const fd2d_engine* engine, int64_t ticket, int32_t* done);
@@ -332,6 +335,15 @@ int fd2d_run_batch_device(const fd2d_solve* solves, int32_t n, void* stream);
 /* Upper bound of the device bytes one solve allocates inside the library (the caller's arrays
  * are not counted): the figure an admission policy charges against an HBM budget. */
 int fd2d_solve_device_bytes(const fd2d_solve* solve, int64_t* bytes);
+
+/*
+ * Device memory of the solvers comes from a per-device cache: released buffers are kept (up to a
+ * quarter of the device memory) and handed out again to later solves, which otherwise pay a
+ * hipMalloc / hipFree of multi-GB buffers per solve. The cached memory is not available to other
+ * allocators of the process (e.g. PyTorch's); this call waits for the current device and frees every
+ * cached block. Memory management; it has no counterpart in the reference.
+ */
+int fd2d_release_cached_memory(void);
 
 /*
  * Steps per launch of the time loop (tuning; it has no counterpart in the reference). 1 (the
