@@ -42,10 +42,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=None,
                     help="time steps (default: the workload's, 8000 for the pipeline, 4000 for stations)")
-    ap.add_argument("--workload", choices=["pipeline", "stations"], default="pipeline",
+    ap.add_argument("--workload", choices=["pipeline", "stations", "serving"], default="pipeline",
                     help="pipeline: the single large solve (the benchmark line); stations: the batched "
                          "reference-station workload (1200x1200, order 4, 64 stations x 4 basis functions) "
-                         "through the batch engine against the plain single-solve path")
+                         "through the batch engine against the plain single-solve path; serving: a Poisson "
+                         "stream of mixed correlation / kernel / Hessian-vector requests through the serving "
+                         "loop against the same requests run one solve at a time")
+    ap.add_argument("--rates", type=str, default="5,20,200", help="arrival rates [req/s] of the serving workload")
+    ap.add_argument("--requests", type=int, default=48, help="requests per rate of the serving workload")
     ap.add_argument("--stations", type=int, default=64, help="stations of the stations workload")
     ap.add_argument("--window", type=int, default=8, help="stations in flight in the batch engine")
     ap.add_argument("--plain", type=int, default=8, help="stations timed on the plain single-solve path")
@@ -292,6 +296,83 @@ def stations_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def serving_bench(args):
+    """The serving workload at each arrival rate: throughput (requests/s, GP-updates/s) and p50/p99
+    latency (arrival to result); then the requests of the highest rate run one solve at a time
+    through the single-solve entry points (the plain path), and every result is compared."""
+    import torch
+    from fd2d_noise_amd import device as D, serving as S
+    from fd2d_noise_amd.workloads import serving_models, serving_stream
+    models = serving_models()
+    rates = [float(r) for r in args.rates.split(",")]
+    srv = S.Server(rounds=args.rounds)
+    for name, m in models.items():
+        srv.add_model(name, m["config"], m["mu"], m["rho"], m["spectrum"], m["source_dist"])
+    print(f"# serving: {len(models)} models {sorted(models)}, budget {srv.budget / 1e9:.0f} GB", file=sys.stderr,
+          flush=True)
+
+    def requests(stream):
+        out = []
+        for off, kind, name, src, rec, seed in stream:
+            c = models[name]["config"]
+            obs = torch.zeros((c.n_corr, rec.shape[0]), dtype=torch.float64, device="cuda")
+            dmu = None
+            if kind == "hvp":
+                sgn = np.sign(np.random.default_rng(seed).standard_normal((c.nx, c.nz)))
+                dmu = D.to_device(0.01 * models[name]["mu"] * sgn)
+            out.append((off, S.Request(kind, name, src, rec, observed=obs, dmu=dmu)))
+        return out
+
+    # warm-up: one request of each kind on every model size class
+    warm = requests([(0.0, k, n, models[n]["stations"][:1], models[n]["stations"][1:4], 1)
+                     for n in sorted(models)[:2] for k in ("correlation", "kernel", "hvp")])
+    srv.serve(warm)
+    srv.done.clear()
+    runs = []
+    for rate in rates:
+        stream = requests(serving_stream(models, rate, args.requests, seed=int(rate * 1000) + 7))
+        srv.done.clear()
+        srv.gcache.clear()
+        srv.gcache_bytes = 0
+        gp0 = srv.gp_updates
+        t0 = time.perf_counter()
+        srv.serve(stream)
+        el = time.perf_counter() - t0
+        lat = srv.latencies() * 1e3
+        kinds = {k: sum(1 for _, r in stream if r.kind == k) for k in ("correlation", "kernel", "hvp")}
+        runs.append({"rate_req_s": rate, "requests": len(stream), "mix": kinds, "wall_s": el,
+                     "throughput_req_s": len(stream) / el, "gp_updates_per_s": (srv.gp_updates - gp0) / el,
+                     "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+                     "spectrum_cache_hits": srv.cache_hits})
+        print(f"# rate {rate}: {len(stream) / el:.2f} req/s, p50 {runs[-1]['latency_ms']['p50']:.0f} ms, "
+              f"p99 {runs[-1]['latency_ms']['p99']:.0f} ms", file=sys.stderr, flush=True)
+        last = list(srv.done)
+    # the plain path on the requests of the highest rate, each compared with its served result
+    t0 = time.perf_counter()
+    same = True
+    for live in last:
+        alone = S.run_alone(srv, live.req)
+        same &= all(torch.equal(alone[k], live.result[k]) for k in alone)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    top = runs[-1]
+    srv.close()
+    out = {
+        "metric": "Gpt-updates/s, correlations+kernels/s, p99 latency; FP32 vs plain, L2<=1e-5",
+        "value": top["gp_updates_per_s"], "unit": "GP-updates/s", "n_gpus": 1, "steps": top["requests"], "warmup": 1,
+        "ms_per_step": top["wall_s"] / top["requests"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 fields, f64 DFT accumulators/kernels", "data": "synthetic",
+        "config": {"workload": "serving: Poisson stream, 50% correlation / 35% correlation+structure kernel / 15% "
+                               "structure HVP, grids 200^2-400^2 (dx 2 km, nt=10 nx), order 4 or 8, fw_nth=1, "
+                               "121 frequencies, 8 receivers",
+                   "models": sorted(models), "rates": rates},
+        "runs": runs,
+        "plain_req_s": len(last) / t_plain, "served_vs_plain": t_plain / top["wall_s"],
+        "identical_to_plain": bool(same),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -322,6 +403,9 @@ def main():
         return
     if args.workload == "stations":
         stations_bench(args)
+        return
+    if args.workload == "serving":
+        serving_bench(args)
         return
     w = config2_workload(nt=args.nt or 8000)
     c = w["config"]

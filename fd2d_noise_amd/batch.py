@@ -21,6 +21,7 @@ from __future__ import annotations
 import ctypes
 from typing import List, Optional, Sequence
 
+import numpy as np
 import torch
 
 from . import _lib, abi
@@ -65,15 +66,38 @@ def _bind():
     return lib
 
 
+HOST_INPUTS = ("src", "rec", "spectrum")  # inputs the library also reads from host memory
+
+
+def _c_host(a: np.ndarray):
+    """Descriptor of a host array (MATLAB column-major float64)."""
+    d = abi.CArray()
+    d.data = a.ctypes.data
+    d.dims[:] = abi.matlab_dims(a.shape)
+    return ctypes.pointer(d)
+
+
+def _desc(name, v):
+    if isinstance(v, np.ndarray):
+        if name not in HOST_INPUTS:
+            raise TypeError(f"{name} must be a device tensor (host arrays are accepted for {HOST_INPUTS})")
+        return _c_host(v)
+    return _c(v)
+
+
 class SolveSpec:
-    """One solve: the C struct plus every object its pointers refer to."""
+    """One solve: the C struct plus every object its pointers refer to. src, rec and spectrum may be
+    host arrays (float64, MATLAB layout): the library then reads them without waiting for the
+    stream."""
 
     def __init__(self, solver: int, mode: float, config: Config, inputs: dict, outputs: Sequence):
         self.solver = solver
         self.config = config
         self.cc = config.to_c()
+        inputs = {k: (np.asfortranarray(v, dtype=np.float64) if isinstance(v, np.ndarray) else v)
+                  for k, v in inputs.items()}
         self.tensors = [t for t in list(inputs.values()) + list(outputs) if t is not None]
-        self._desc = {k: _c(v) for k, v in inputs.items()}
+        self._desc = {k: _desc(k, v) for k, v in inputs.items()}
         self._outs = [_m(t) for t in outputs] + [None] * (4 - len(outputs))
         s = CSolve()
         s.solver = solver
@@ -91,7 +115,20 @@ class SolveSpec:
         from . import device as D
         fn = {abi.FD2D_GREEN: D.run_forward1_green, abi.FD2D_CORRELATION: D.run_forward2_correlation,
               abi.FD2D_ADJOINT: D.run_noise_adjoint}[self.solver]
-        fn(*self.call_args, config=self.config, **self.call_outputs)
+        args = [D.to_device(a) if isinstance(a, np.ndarray) else a for a in self.call_args]
+        fn(*args, config=self.config, **self.call_outputs)
+
+    def time_steps(self) -> int:
+        """Time steps of this solve: nt (Green), 2nt-1 (correlation, adjoint) or nt for the
+        half-length adjoint modes 0 / 1 with a distributed source (run_noise_adjoint.m:165-167)."""
+        c = self.config
+        if self.solver == abi.FD2D_GREEN:
+            return c.nt
+        if self.solver == abi.FD2D_ADJOINT and int(self.c.mode) in (0, 1):
+            stf = self._desc.get("stf")
+            if stf is not None and stf.contents.dims[2] != 1:
+                return c.nt
+        return c.n_corr
 
     def device_bytes(self) -> int:
         """Upper bound of the device memory the library allocates for this solve."""

@@ -273,11 +273,28 @@ int nearest_node(double p, double h, int n)
     return best;
 }
 
+// True for memory the host cannot read directly (device memory); host memory, pageable or pinned,
+// is read in place.
+bool device_memory(const void* p)
+{
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;  // not known to HIP: pageable host memory
+    }
+    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeArray;
+}
+
+// A small input array (coordinates, spectra, time axes) on the host. Device memory is copied with
+// a synchronous copy, which waits for the work queued before it; host memory is read without
+// waiting.
 template <class T>
-std::vector<T> to_host(const void* dev, size_t n)
+std::vector<T> to_host(const void* src, size_t n)
 {
     std::vector<T> h(n);
-    if (n) HIPCHK(hipMemcpy(h.data(), dev, n * sizeof(T), hipMemcpyDeviceToHost));
+    if (n == 0) return h;
+    if (device_memory(src)) HIPCHK(hipMemcpy(h.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+    else std::memcpy(h.data(), src, n * sizeof(T));
     return h;
 }
 
@@ -793,8 +810,13 @@ struct Job {
     hipStream_t st = nullptr;
     std::unique_ptr<Solver> S;
     int solver = 0, nsteps = 0, next = 1;
-    bool born = false;      // Born terms need one launch per step and their own set-up per step
+    bool born = false;      // Born terms: one step per launch
     bool records = false;   // writes seismograms (receiver lists are used)
+    // Batch engine: the structure-kernel product of a step is handed to the engine (which launches
+    // the products of many solves together) instead of being launched by after().
+    bool defer_kmu = false;
+    bool has_kmu = false;
+    KmuJob kmu_pending{};
     virtual ~Job() = default;
     virtual void plan(int n, int sub) = 0;  // fields of step n as step `sub` of the launch
     virtual bool pairable(int n) = 0;       // may steps n, n+1 share a two-step launch
@@ -1196,7 +1218,14 @@ struct AdjointJob : Job {
     }
     void after(int n, int k) override
     {
-        if (kfwd[k - 1]) S->kmu(kfwd[k - 1], Kd + (size_t)nb_cols * g.P, (double)g.f);
+        if (kfwd[k - 1]) {
+            if (defer_kmu) {
+                kmu_pending = KmuJob{S->u[S->cur], kfwd[k - 1], Kd + (size_t)nb_cols * g.P, (double)g.f, 0};
+                has_kmu = true;
+            } else {
+                S->kmu(kfwd[k - 1], Kd + (size_t)nb_cols * g.P, (double)g.f);
+            }
+        }
         for (int q = 0; q < k; ++q)
             if (sampled(n + q)) dft.commit();
     }
@@ -1356,8 +1385,7 @@ struct Engine {
         for (auto& e : act) {
             Job& j = *e->job;
             if (j.next > j.nsteps) continue;
-            if (j.born) singles.push_back(e.get());
-            else cls.push_back({{j.g.R, j.S->with_u ? 1 : 0}, e.get()});
+            cls.push_back({{j.g.R, j.S->with_u ? 1 : 0}, e.get()});
         }
         std::stable_sort(cls.begin(), cls.end(),
                          [](const auto& x, const auto& y) { return x.first < y.first; });
@@ -1399,7 +1427,8 @@ struct Engine {
             const StepArgs& a = j.S->a;
             ga.tile_start[q] = total;
             total += j.S->n_tiles;
-            ga.step[q] = JobStep{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.ps_const, j.S->cur, e.slot};
+            ga.step[q] = JobStep{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.born_dmu, a.born_in0, a.born_in1,
+                                 a.ps_const, j.S->cur, e.slot};
         }
         ga.tile_start[ga.n_jobs] = total;
         ga.total_tiles = total;
@@ -1410,6 +1439,35 @@ struct Engine {
             j.S->cur ^= 1;
             j.after(j.next, 1);
             ++j.next;
+        }
+    }
+    // The structure-kernel products the solves' last steps left, one launch per FD order.
+    void launch_products()
+    {
+        for (int R = 1; R <= 4; ++R) {
+            KmuGroupArgs ka{};
+            ka.jobs = slots_d.p;
+            int total = 0;
+            auto flush = [&]() {
+                if (ka.n_jobs == 0) return;
+                ka.tile_start[ka.n_jobs] = total;
+                ka.total_tiles = total;
+                ka.tiles_per_xcd = (total + kXCD - 1) / kXCD;
+                launch_kmu_group(R, ka, st);
+                ka.n_jobs = 0;
+                total = 0;
+            };
+            for (auto& e : act) {
+                Job& j = *e->job;
+                if (!j.has_kmu || j.g.R != R) continue;
+                ka.tile_start[ka.n_jobs] = total;
+                ka.job[ka.n_jobs] = j.kmu_pending;
+                ka.job[ka.n_jobs].slot = e->slot;
+                total += j.S->n_tiles;
+                j.has_kmu = false;
+                if (++ka.n_jobs == kGroupMax) flush();
+            }
+            flush();
         }
     }
     // Drop the solves whose last work has completed on the device.
@@ -1449,6 +1507,7 @@ struct Engine {
         StreamScope scope(st);
         if (free_slots.empty()) raise(FD2D_ENOMEM, "the engine holds %d solves already", kEngineSlots);
         std::unique_ptr<Entry> e = make_entry(s, st);
+        e->job->defer_kmu = true;
         e->slot = free_slots.back();
         e->stat[0] = e->job->static_args(0);
         e->stat[1] = e->job->static_args(1);
@@ -1468,6 +1527,7 @@ struct Engine {
             if (dirty) rebuild();
             for (const Group& gr : groups) step_group(gr);
             for (Entry* e : singles) step_single(*e->job);
+            launch_products();
         }
         retire();
         HIPCHK(hipGetLastError());

@@ -126,6 +126,9 @@ struct JobStep {
     float* snap;
     float* dsamp;
     double* seis_col;
+    const float* born_dmu;  // stress-phase Born terms of this step (null: none)
+    const float* born_in0;
+    const float* born_in1;
     float ps_const;
     int parity;  // which of the solve's two StepArgs (ping-pong direction) this step uses
     int slot;    // the solve's StepArgs are jobs[2 * slot + parity] of the grouped launch
@@ -153,6 +156,26 @@ inline int pair_halo_hi(int R) { return 3 * R - 1; }
 void launch_step(int R, bool with_u, const StepArgs& a, hipStream_t s);
 void launch_pair(int R, bool with_u, const PairArgs& p, hipStream_t s);
 void launch_group(int R, bool with_u, const GroupArgs& g, hipStream_t s);
+
+// Structure-kernel products of up to kGroupMax solves (same FD order) in one launch; solve j owns
+// tiles tile_start[j] .. tile_start[j+1]-1 and its geometry is jobs[2 * job[j].slot].
+struct KmuJob {
+    const float* u;    // displacement after the step (padded array origin)
+    const float* fwd;  // forward snapshot (compact)
+    double* kmu;       // K_mu accumulator (compact)
+    double scale;
+    int slot;
+};
+struct KmuGroupArgs {
+    const StepArgs* jobs;
+    int n_jobs;
+    int total_tiles;
+    int tiles_per_xcd;
+    int tile_start[kGroupMax + 1];
+    KmuJob job[kGroupMax];
+};
+static_assert(sizeof(KmuGroupArgs) <= 4096, "grouped-launch arguments exceed the 4 KB kernel-argument limit");
+void launch_kmu_group(int R, const KmuGroupArgs& g, hipStream_t s);
 // K_mu -= scale * (D u . D fwd) over the grid (u padded, fwd compact), after a step.
 void launch_kmu(int R, const StepArgs& a, const float* u, const float* fwd, double* kmu, double scale,
                 hipStream_t s);

@@ -269,10 +269,10 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     for (int k = 0; k < NT; ++k) rm[k] = G.coef(a.o_dtmu, x, j0 + w + 4 * k);
     const float fxs = frame(x, R - 1, nx - R - 1);  // dx_v frame
     const int c = lane + R - 1;
-    const bool born = a.born_dmu != nullptr;
-    const auto r_b0 = rsrc(born ? (const void*)a.born_in0 : (const void*)a.fbase, born ? cb : 0u);
-    const auto r_b1 = rsrc(born ? (const void*)a.born_in1 : (const void*)a.fbase, born ? cb : 0u);
-    const auto r_bd = rsrc(born ? (const void*)a.born_dmu : (const void*)a.fbase, born ? cb : 0u);
+    const bool born = st.born_dmu != nullptr;
+    const auto r_b0 = rsrc(born ? (const void*)st.born_in0 : (const void*)a.fbase, born ? cb : 0u);
+    const auto r_b1 = rsrc(born ? (const void*)st.born_in1 : (const void*)a.fbase, born ? cb : 0u);
+    const auto r_bd = rsrc(born ? (const void*)st.born_dmu : (const void*)a.fbase, born ? cb : 0u);
 #pragma unroll
     for (int k = 0; k < NT; ++k) {
         const int lz = w + 4 * k, z = j0 + lz;
@@ -312,7 +312,8 @@ template <int R, int MODE>
 __global__ __launch_bounds__(256) void k_tile(StepArgs a)
 {
     const int g = blockIdx.x;
-    const JobStep st{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.ps_const, 0, 0};
+    const JobStep st{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.born_dmu, a.born_in0, a.born_in1,
+                     a.ps_const, 0, 0};
     tile_step<R, MODE>(a, st, (g % kXCD) * a.tiles_per_xcd + g / kXCD);
 }
 
@@ -650,14 +651,10 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
 // Structure-kernel product of run_noise_adjoint.m:324-331 after a step that stores it:
 // K_mu -= fw_nth * (dx_v(u) .* dx_v(u_fwd) + dz_v(u) .* dz_v(u_fwd)) with u the new displacement
 // (padded) and u_fwd the forward snapshot (compact); the products are formed in FP64.
-template <int R>
-__global__ __launch_bounds__(256) void k_kmu(StepArgs a, const float* __restrict__ u, const float* __restrict__ fwd,
-                                             double* __restrict__ kmu, double scale)
+// PF / PD: float / double pointer types (global-address-space pointers in the grouped launch)
+template <int R, class PF, class PD>
+__device__ __forceinline__ void kmu_tile(const StepArgs& a, PF u, PF fwd, PD kmu, double scale, int t)
 {
-    // the step kernel's 64 x 32 tiles in the same XCD-banded order, so the z-stencil rows a tile
-    // shares with the tiles above and below come from the same L2
-    const int g = blockIdx.x;
-    const int t = (g % kXCD) * a.tiles_per_xcd + g / kXCD;
     if (t >= a.n_tiles) return;
     const int x = (t % a.tiles_x) * kTX + (threadIdx.x & 63);
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -691,6 +688,43 @@ __global__ __launch_bounds__(256) void k_kmu(StepArgs a, const float* __restrict
             acc += (double)su * (double)sf;
         }
         kmu[c] -= acc * scale;
+    }
+}
+
+// the step kernel's 64 x 32 tiles in the same XCD-banded order, so the z-stencil rows a tile shares
+// with the tiles above and below come from the same L2
+template <int R>
+__global__ __launch_bounds__(256) void k_kmu(StepArgs a, const float* __restrict__ u, const float* __restrict__ fwd,
+                                             double* __restrict__ kmu, double scale)
+{
+    const int g = blockIdx.x;
+    kmu_tile<R>(a, u, fwd, kmu, scale, (g % kXCD) * a.tiles_per_xcd + g / kXCD);  // kernel arguments: global
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_kmu_group(KmuGroupArgs G)
+{
+    const int g = blockIdx.x;
+    const int T = (g % kXCD) * G.tiles_per_xcd + g / kXCD;
+    if (T >= G.total_tiles) return;
+    int j = 0;
+    while (j + 1 < G.n_jobs && G.tile_start[j + 1] <= T) ++j;
+    j = __builtin_amdgcn_readfirstlane(j);
+    const KmuJob jb = G.job[j];
+    const StepArgs a = G.jobs[2 * jb.slot];
+    kmu_tile<R>(a, (const __attribute__((address_space(1))) float*)jb.u,
+                (const __attribute__((address_space(1))) float*)jb.fwd,
+                (__attribute__((address_space(1))) double*)jb.kmu, jb.scale, T - G.tile_start[j]);
+}
+
+void launch_kmu_group(int R, const KmuGroupArgs& ga, hipStream_t s)
+{
+    const dim3 g((unsigned)(kXCD * ga.tiles_per_xcd)), b(256);
+    switch (R) {
+    case 1: hipLaunchKernelGGL(k_kmu_group<1>, g, b, 0, s, ga); break;
+    case 2: hipLaunchKernelGGL(k_kmu_group<2>, g, b, 0, s, ga); break;
+    case 3: hipLaunchKernelGGL(k_kmu_group<3>, g, b, 0, s, ga); break;
+    default: hipLaunchKernelGGL(k_kmu_group<4>, g, b, 0, s, ga); break;
     }
 }
 

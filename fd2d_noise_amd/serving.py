@@ -32,7 +32,6 @@ import torch
 
 from . import abi, batch as B, device as D
 from .config import Config
-from .inversion import axpby
 from .measurement import make_adjoint_sources_device
 
 f32, f64, cd = torch.float32, torch.float64, torch.complex128
@@ -46,9 +45,11 @@ class Model:
     rho: torch.Tensor
     spectrum: torch.Tensor
     source_dist: torch.Tensor
+    spectrum_host: Optional[np.ndarray] = None  # the same spectrum in host memory (read without a wait)
+    t: Optional[torch.Tensor] = None             # the correlation time axis, resident
 
 
-@dataclasses.dataclass
+@dataclasses.dataclass(eq=False)
 class Request:
     """kind: 'correlation' (Green + correlation), 'kernel' (correlation + structure kernel:
     eval_objective_and_gradient.m:52-98 for one station) or 'hvp' (the structure Hessian-vector
@@ -70,6 +71,10 @@ class Request:
 # ---------------------------------------------------------------------------------------------
 # Request programs
 # ---------------------------------------------------------------------------------------------
+def _host(a) -> np.ndarray:
+    return np.asfortranarray(np.atleast_2d(np.asarray(a, dtype=np.float64)))
+
+
 def _adjoint_stf(corr, observed, du, t, req: Request, order: str):
     _, adstf = make_adjoint_sources_device(corr, observed, du, t, req.veldis, req.measurement, order, req.window)
     stf = torch.zeros(adstf.shape, dtype=cd, device=adstf.device)
@@ -80,15 +85,15 @@ def _adjoint_stf(corr, observed, du, t, req: Request, order: str):
 def correlation_program(srv: "Server", req: Request):
     m = srv.models[req.model]
     c = m.config
-    src, rec = D.to_device(req.src), D.to_device(req.rec)
-    key = (req.model, tuple(np.asarray(req.src, dtype=np.float64).ravel()))
+    src, rec = _host(req.src), _host(req.rec)
+    key = (req.model, tuple(src.ravel()))
     G = srv.cached_spectrum(key)
     if G is None:
         G = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
         yield [B.green(m.mu, m.rho, src, None, 0, None, None, config=c, G_fft=G)]
         srv.store_spectrum(key, G)
     corr = D.empty_like_dims((req.rec.shape[0], c.n_corr), f64)
-    yield [B.correlation(m.mu, m.rho, G, m.spectrum, m.source_dist, rec, 0, None, None, config=c,
+    yield [B.correlation(m.mu, m.rho, G, m.spectrum_host, m.source_dist, rec, 0, None, None, config=c,
                          seismograms=corr)]
     return {"correlations": corr}
 
@@ -98,28 +103,26 @@ def kernel_program(srv: "Server", req: Request):
     against C, adjoint mode 0 against G with the transformed adjoint state; K = K1 + K2."""
     m = srv.models[req.model]
     c = m.config
-    src, rec = D.to_device(req.src), D.to_device(req.rec)
+    src, rec = _host(req.src), _host(req.rec)
     nr = req.rec.shape[0]
     G = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
     Gout = D.empty_like_dims((c.nx, c.nz, c.n_fw_green()), f32)
     yield [B.green(m.mu, m.rho, src, None, 1, None, None, config=c, G_fft=G, G_out=Gout)]
-    srv.store_spectrum((req.model, tuple(np.asarray(req.src, dtype=np.float64).ravel())), G)
+    srv.store_spectrum((req.model, tuple(src.ravel())), G)
     corr = D.empty_like_dims((nr, c.n_corr), f64)
     C = D.empty_like_dims((c.nx, c.nz, c.n_fw_corr()), f32)
-    yield [B.correlation(m.mu, m.rho, G, m.spectrum, m.source_dist, rec, 1, None, None, config=c,
+    yield [B.correlation(m.mu, m.rho, G, m.spectrum_host, m.source_dist, rec, 1, None, None, config=c,
                          seismograms=corr, C_out=C)]
-    t = torch.as_tensor(c.t_corr(), device="cuda")
-    stf = _adjoint_stf(corr, req.observed, None, t, req, "1st")
+    stf = _adjoint_stf(corr, req.observed, None, m.t, req, "1st")
     nbk = max(c.n_basis_fct, 1) + 1
     K1 = D.empty_like_dims((c.nx, c.nz, nbk), f64)
     A1 = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
     yield [B.adjoint(m.mu, m.rho, C, stf, rec, None, None, None, None, 1, None, None, config=c, K=K1, stf_fft=A1)]
     del C
     K2 = D.empty_like_dims((c.nx, c.nz, nbk), f64)
-    yield [B.adjoint(m.mu, m.rho, Gout, A1, rec, None, m.spectrum, m.source_dist, None, 0, None, None, config=c,
-                     K=K2)]
-    K = axpby(1.0, K1.view(-1), 1.0, K2.view(-1)).view(K1.shape)
-    return {"correlations": corr, "K": K}
+    yield [B.adjoint(m.mu, m.rho, Gout, A1, rec, None, m.spectrum_host, m.source_dist, None, 0, None, None,
+                     config=c, K=K2)]
+    return {"correlations": corr, "K": K1 + K2}
 
 
 def hvp_program(srv: "Server", req: Request):
@@ -131,9 +134,9 @@ def hvp_program(srv: "Server", req: Request):
     c = m.config
     if c.fw_nth != 1:
         raise ValueError("the Hessian-vector product needs fw_nth = 1 (eval_hessian_vector_product_2.m:75,154)")
-    src, rec = D.to_device(req.src), D.to_device(req.rec)
+    src, rec = _host(req.src), _host(req.rec)
     nr, N = req.rec.shape[0], c.n_corr
-    mu, rho, spec, sd, dmu = m.mu, m.rho, m.spectrum, m.source_dist, req.dmu
+    mu, rho, spec, sd, dmu = m.mu, m.rho, m.spectrum_host, m.source_dist, req.dmu
     nbk = max(c.n_basis_fct, 1) + 1
     G = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
     Gout = D.empty_like_dims((c.nx, c.nz, c.n_fw_green()), f32)
@@ -147,9 +150,8 @@ def hvp_program(srv: "Server", req: Request):
     c_pert = D.empty_like_dims((nr, N), f64)
     dC = D.empty_like_dims((c.nx, c.nz, c.n_fw_corr()), f32)
     yield [B.correlation(mu, rho, dG, spec, sd, rec, 1, dmu, C, config=c, seismograms=c_pert, C_out=dC)]  # :75
-    t = torch.as_tensor(c.t_corr(), device="cuda")
-    stf1 = _adjoint_stf(c_ref, req.observed, c_pert, t, req, "1st")  # :99-105
-    stf2 = _adjoint_stf(c_ref, req.observed, c_pert, t, req, "2nd")
+    stf1 = _adjoint_stf(c_ref, req.observed, c_pert, m.t, req, "1st")  # :99-105
+    stf2 = _adjoint_stf(c_ref, req.observed, c_pert, m.t, req, "2nd")
     H = torch.zeros((nbk, c.nz, c.nx), dtype=f64, device="cuda")
     KY = D.empty_like_dims((c.nx, c.nz, nbk), f64)
     uT = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
@@ -167,10 +169,9 @@ def hvp_program(srv: "Server", req: Request):
     # H = sum of the kernels; with type 'structure' only the structure map of YELLOW and BLACK
     # (:120, :157-159); GRAY and RED enter whole
     for K, part in ((KY, "structure"), (KG, None), (KB, "structure"), (KR, None)):
-        Kt = K.clone()
         if part == "structure":
-            Kt[:nbk - 1].zero_()
-        axpby(1.0, H.view(-1), 1.0, Kt.view(-1), out=H.view(-1))
+            K[:nbk - 1].zero_()
+        H += K
     return {"correlations": c_ref, "H": H}
 
 
@@ -197,7 +198,7 @@ def request_bytes(srv: "Server", req: Request) -> int:
     return 4 * spec_b + 2 * g_snap + 4 * c_snap + 4 * 16 * P + 2 * max(green, corr, adj1, adj0) + traces
 
 
-@dataclasses.dataclass
+@dataclasses.dataclass(eq=False)
 class _Live:
     req: Request
     gen: Any
@@ -227,11 +228,15 @@ class Server:
         self.gcache: "collections.OrderedDict[Any, torch.Tensor]" = collections.OrderedDict()
         self.gcache_bytes = 0
         self.cache_hits = 0
+        self.gp_updates = 0  # grid points x time steps of every solve submitted
 
     # ---- resident state ------------------------------------------------------------------------
     def add_model(self, name: str, config: Config, mu, rho, spectrum, source_dist) -> None:
         dev = lambda a: a if torch.is_tensor(a) else D.to_device(a)
-        self.models[name] = Model(config, dev(mu), dev(rho), dev(spectrum), dev(source_dist))
+        spec_h = D.to_host(spectrum) if torch.is_tensor(spectrum) else np.asarray(spectrum, dtype=np.float64)
+        self.models[name] = Model(config, dev(mu), dev(rho), dev(spectrum), dev(source_dist),
+                                  np.asfortranarray(spec_h, dtype=np.float64),
+                                  torch.as_tensor(config.t_corr(), device="cuda"))
 
     def cached_spectrum(self, key):
         G = self.gcache.get(key)
@@ -297,6 +302,7 @@ class Server:
             return
         live.specs = specs
         live.tickets = [self.engine.submit(s) for s in specs]
+        self.gp_updates += sum(s.config.nx * s.config.nz * s.time_steps() for s in specs)
 
     def step(self) -> None:
         """Admits what fits, advances the engine, resumes requests whose stage has completed."""

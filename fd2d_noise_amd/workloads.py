@@ -14,6 +14,11 @@ config3: batched reference stations -- 1200 x 1200 nodes on 2000 km (dx = 1.67 k
          dt = 0.12 s, nt = 4000, 128 frequencies (0.01-0.2 Hz, every 5th step), 4 source basis
          functions (seeded blobs, flat spectrum as after general_source), 64 reference stations
          seeded uniformly inside the absorbing frame and 32 receivers shared by all stations.
+serving: a Poisson stream of mixed requests -- 50% correlation, 35% correlation + structure kernel,
+         15% structure Hessian-vector product -- on resident models of three grid sizes (200, 283 and
+         400 nodes square, log-spaced; dx = 2 km, nt = 10 nx) at FD order 4 or 8, vs = 4 km/s times
+         (1 + 0.05 g) with seeded random fields g, forward fields stored every step (the HVP needs
+         it), 8 candidate reference stations per model and 8 receivers per request.
 The taper is the reference's Gaussian taper (init_absbound.m:18).
 """
 from __future__ import annotations
@@ -99,3 +104,44 @@ def config3_workload(seed: int = 3, nt: int = 4000, n_stations: int = 64) -> dic
         mu=MU0 * np.ones((c.nx, c.nz), order="F"), rho=RHO0 * np.ones((c.nx, c.nz), order="F"),
         stations=stations, rec=rec, spectrum=np.ones((c.n_sample, 1)),
         source_dist=np.asfortranarray(basis))
+
+
+def serving_models(seed: int = 5, sizes=(200, 283, 400), orders=(4, 8)) -> dict:
+    """Resident models of the serving workload: name -> dict(config, mu, rho, spectrum, source_dist,
+    stations)."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    for n in sizes:
+        for order in orders:
+            dx = 2000.0
+            c = Config(Lx=dx * (n - 1), Lz=dx * (n - 1), nx=n, nz=n, dt=0.1, nt=10 * n, order=order, fw_nth=1,
+                       freq_samp=5)
+            c.absorb_width = 20 * c.dx
+            g = gaussian_random_field(rng, n, n, c.dx, 30e3)
+            vs = 4000.0 * (1 + 0.05 * g)
+            rho = RHO0 * np.ones((n, n), order="F")
+            f = np.asarray(c.f_sample)
+            w = c.absorb_width
+            st = np.stack([rng.uniform(w, c.Lx - w, 8), rng.uniform(w, c.Lz - w, 8)], axis=1)
+            out[f"n{n}_o{order}"] = dict(config=c, mu=np.asfortranarray(rho * vs ** 2), rho=rho,
+                                         spectrum=np.exp(-((f - 0.1) ** 2) / (2 * 0.03 ** 2))[:, None],
+                                         source_dist=_blob(rng, c, sigma=0.2 * c.Lx), stations=st)
+    return out
+
+
+def serving_stream(models: dict, rate: float, n_requests: int, seed: int = 6, mix=(0.5, 0.35, 0.15)):
+    """(arrival offset [s], kind, model name, src (1 x 2), rec (8 x 2), seed) tuples of a Poisson stream."""
+    rng = np.random.default_rng(seed)
+    names = sorted(models)
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    kinds = rng.choice(["correlation", "kernel", "hvp"], size=n_requests, p=list(mix))
+    out = []
+    for q in range(n_requests):
+        name = names[rng.integers(len(names))]
+        m = models[name]
+        c = m["config"]
+        w = c.absorb_width
+        src = m["stations"][rng.integers(len(m["stations"]))][None, :]
+        rec = np.stack([rng.uniform(w, c.Lx - w, 8), rng.uniform(w, c.Lz - w, 8)], axis=1)
+        out.append((float(t[q]), str(kinds[q]), name, src, rec, int(rng.integers(1 << 30))))
+    return out

@@ -152,7 +152,10 @@ int fd2d_run_noise_adjoint(const fd2d_config* cfg,
                            fd2d_marray* seismograms);
 
 /* Device-resident forms: identical arguments, every data pointer is a device pointer, and the work
- * runs on `stream` (a hipStream_t, NULL for the null stream). No host<->device copies are made. */
+ * runs on `stream` (a hipStream_t, NULL for the null stream). No host<->device copies of fields are
+ * made. The small inputs src, rec and spectrum may also be in host memory: they are then read in
+ * place, without the wait for queued work that reading them from device memory implies (the batch
+ * engine and serving loops pass them this way). */
 int fd2d_run_forward1_green_device(const fd2d_config* cfg,
                                    const fd2d_carray* mu, const fd2d_carray* rho,
                                    const fd2d_carray* src, const fd2d_carray* rec, double mode,
