@@ -57,6 +57,7 @@ EXPORTS = (
     "fd2d_regularization_device", "fd2d_general_source_device", "fd2d_axpby_f32_device",
     "fd2d_engine_create", "fd2d_engine_submit", "fd2d_engine_advance", "fd2d_engine_done", "fd2d_engine_destroy",
     "fd2d_run_batch_device", "fd2d_solve_device_bytes", "fd2d_release_cached_memory",
+    "fd2d_make_adjoint_sources_queued",
 )
 
 

@@ -564,6 +564,7 @@ struct Solver {
         a.rec_ent = nullptr;
         a.seis_col = nullptr;
         a.born_dmu = a.born_in0 = a.born_in1 = nullptr;
+        a.kfwd = nullptr;
         pr.ps_amp2 = nullptr;
         pr.ps_const2 = 0.f;
         pr.syn2 = nullptr;
@@ -812,16 +813,14 @@ struct Job {
     int solver = 0, nsteps = 0, next = 1;
     bool born = false;      // Born terms: one step per launch
     bool records = false;   // writes seismograms (receiver lists are used)
-    // Batch engine: the structure-kernel product of a step is handed to the engine (which launches
-    // the products of many solves together) instead of being launched by after().
-    bool defer_kmu = false;
-    bool has_kmu = false;
-    KmuJob kmu_pending{};
     virtual ~Job() = default;
     virtual void plan(int n, int sub) = 0;  // fields of step n as step `sub` of the launch
     virtual bool pairable(int n) = 0;       // may steps n, n+1 share a two-step launch
     virtual void after(int n, int k) = 0;   // after the launch that advanced steps n .. n+k-1
     virtual void finish() = 0;              // after the last step
+    // For a step launched on its own: move the structure-kernel product that plan() scheduled
+    // after the launch into the step kernel itself.
+    virtual void fuse_products() {}
     // StepArgs with the fields that do not change from step to step, for ping-pong direction `parity`
     StepArgs static_args(int parity)
     {
@@ -851,6 +850,7 @@ void run_job(Job& j)
         const int k = (pairs && n + 1 <= j.nsteps && j.pairable(n)) ? 2 : 1;
         j.S->clear_step();
         for (int q = 0; q < k; ++q) j.plan(n + q, q);
+        if (k == 1) j.fuse_products();
         timer.begin(n, k);
         if (k == 2) j.S->step_pair();
         else j.S->step();
@@ -1138,6 +1138,10 @@ struct AdjointJob : Job {
 
         S.reset(new Solver(cfg, g, st, true));
         S->material((const double*)mu->data, (const double*)rho->data);
+        if (Kd) {
+            S->a.kacc = Kd + (size_t)nb_cols * g.P;  // K_mu (:359-367)
+            S->a.kscale = (double)g.f;
+        }
         if (stf_point) S->set_sources(sid);
         S->set_receivers(rid);
         if (records) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * N, st));
@@ -1216,16 +1220,15 @@ struct AdjointJob : Job {
     {
         return !kernel_step(n) && !(sampled(n) && sampled(n + 1)) && !(synth(n) && synth(n + 1));
     }
+    void fuse_products() override
+    {
+        S->a.kfwd = kfwd[0];
+        kfwd[0] = nullptr;
+    }
     void after(int n, int k) override
     {
-        if (kfwd[k - 1]) {
-            if (defer_kmu) {
-                kmu_pending = KmuJob{S->u[S->cur], kfwd[k - 1], Kd + (size_t)nb_cols * g.P, (double)g.f, 0};
-                has_kmu = true;
-            } else {
-                S->kmu(kfwd[k - 1], Kd + (size_t)nb_cols * g.P, (double)g.f);
-            }
-        }
+        // a product left for after the launch: the second step of a two-step launch
+        if (kfwd[k - 1]) S->kmu(kfwd[k - 1], Kd + (size_t)nb_cols * g.P, (double)g.f);
         for (int q = 0; q < k; ++q)
             if (sampled(n + q)) dft.commit();
     }
@@ -1409,6 +1412,7 @@ struct Engine {
     {
         j.S->clear_step();
         j.plan(j.next, 0);
+        j.fuse_products();
         j.S->step();
         j.after(j.next, 1);
         ++j.next;
@@ -1424,11 +1428,12 @@ struct Engine {
             Job& j = *e.job;
             j.S->clear_step();
             j.plan(j.next, 0);
+            j.fuse_products();
             const StepArgs& a = j.S->a;
             ga.tile_start[q] = total;
             total += j.S->n_tiles;
             ga.step[q] = JobStep{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.born_dmu, a.born_in0, a.born_in1,
-                                 a.ps_const, j.S->cur, e.slot};
+                                 a.kfwd, a.ps_const, j.S->cur, e.slot};
         }
         ga.tile_start[ga.n_jobs] = total;
         ga.total_tiles = total;
@@ -1439,35 +1444,6 @@ struct Engine {
             j.S->cur ^= 1;
             j.after(j.next, 1);
             ++j.next;
-        }
-    }
-    // The structure-kernel products the solves' last steps left, one launch per FD order.
-    void launch_products()
-    {
-        for (int R = 1; R <= 4; ++R) {
-            KmuGroupArgs ka{};
-            ka.jobs = slots_d.p;
-            int total = 0;
-            auto flush = [&]() {
-                if (ka.n_jobs == 0) return;
-                ka.tile_start[ka.n_jobs] = total;
-                ka.total_tiles = total;
-                ka.tiles_per_xcd = (total + kXCD - 1) / kXCD;
-                launch_kmu_group(R, ka, st);
-                ka.n_jobs = 0;
-                total = 0;
-            };
-            for (auto& e : act) {
-                Job& j = *e->job;
-                if (!j.has_kmu || j.g.R != R) continue;
-                ka.tile_start[ka.n_jobs] = total;
-                ka.job[ka.n_jobs] = j.kmu_pending;
-                ka.job[ka.n_jobs].slot = e->slot;
-                total += j.S->n_tiles;
-                j.has_kmu = false;
-                if (++ka.n_jobs == kGroupMax) flush();
-            }
-            flush();
         }
     }
     // Drop the solves whose last work has completed on the device.
@@ -1507,7 +1483,6 @@ struct Engine {
         StreamScope scope(st);
         if (free_slots.empty()) raise(FD2D_ENOMEM, "the engine holds %d solves already", kEngineSlots);
         std::unique_ptr<Entry> e = make_entry(s, st);
-        e->job->defer_kmu = true;
         e->slot = free_slots.back();
         e->stat[0] = e->job->static_args(0);
         e->stat[1] = e->job->static_args(1);
@@ -1527,7 +1502,6 @@ struct Engine {
             if (dirty) rebuild();
             for (const Group& gr : groups) step_group(gr);
             for (Entry* e : singles) step_single(*e->job);
-            launch_products();
         }
         retire();
         HIPCHK(hipGetLastError());
@@ -1722,6 +1696,45 @@ struct fd2d_engine {
     Engine e;
     explicit fd2d_engine(hipStream_t s) : e(s) {}
 };
+
+namespace {
+int adjoint_sources_entry(const double* u, const double* u0, const double* du, const double* t, const double* t_host,
+                          int64_t nr, int64_t nt, int32_t veldis, int32_t measurement, int32_t deriv_order,
+                          double win_left, double win_right, double* misfit, double* adstf, void* stream, bool wait)
+{
+    return guarded([&] {
+        require_device();
+        if (nr < 1 || nt < 2 || !u || !u0 || !t || !misfit || !adstf) raise(FD2D_EINVAL, "u, u_0, t, misfit, adstf and nt >= 2 are required");
+        if (measurement < 0 || measurement > 3) raise(FD2D_EINVAL, "unknown measurement %d", measurement);
+        if (deriv_order != 1 && deriv_order != 2) raise(FD2D_EINVAL, "deriv_order must be 1 or 2");
+        if (measurement == FD2D_CC_TIME_SHIFT && deriv_order == 2)
+            raise(FD2D_EUNSUPPORTED, "no Hessian vector products for traveltime measurements (make_adjoint_sources.m:121)");
+        if (deriv_order == 2 && !du) raise(FD2D_EINVAL, "deriv_order 2 needs du");
+        hipStream_t st = as_stream(stream);
+        StreamScope scope(st);
+        // make_adjoint_sources.m:85-99: window limits against the time axis
+        const std::vector<double> th = t_host ? std::vector<double>(t_host, t_host + nt) : to_host<double>(t, (size_t)nt);
+        double left = win_left, right = win_right;
+        if (measurement != FD2D_WAVEFORM_DIFFERENCE) {
+            if (left < 0) {
+                int64_t k = 0;
+                while (k < nt && th[(size_t)k] != 0.0) ++k;
+                if (k + 1 >= nt) raise(FD2D_EINVAL, "time axis has no t == 0 sample");
+                left = th[(size_t)k + 1];
+            }
+            if (right > th[(size_t)nt - 1]) right = th[(size_t)nt - 1];
+            if (left > th[(size_t)nt - 1]) raise(FD2D_EINVAL, "time series too short? (make_adjoint_sources.m:99)");
+        }
+        DBuf<double> scratch((size_t)3 * nr * nt);
+        AdjArgs a{u, u0, du, t, nr, nt, th[1] - th[0] >= 0 ? th[1] - th[0] : th[0] - th[1], veldis ? 1 : 0,
+                  measurement, deriv_order, left, right, misfit, adstf, scratch.p};
+        launch_adjoint_sources(a, st);
+        HIPCHK(hipGetLastError());
+        if (wait) HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1930,36 +1943,21 @@ int fd2d_make_adjoint_sources_device(const double* u, const double* u0, const do
                                      int64_t nt, int32_t veldis, int32_t measurement, int32_t deriv_order,
                                      double win_left, double win_right, double* misfit, double* adstf, void* stream)
 {
-    return guarded([&] {
-        require_device();
-        if (nr < 1 || nt < 2 || !u || !u0 || !t || !misfit || !adstf) raise(FD2D_EINVAL, "u, u_0, t, misfit, adstf and nt >= 2 are required");
-        if (measurement < 0 || measurement > 3) raise(FD2D_EINVAL, "unknown measurement %d", measurement);
-        if (deriv_order != 1 && deriv_order != 2) raise(FD2D_EINVAL, "deriv_order must be 1 or 2");
-        if (measurement == FD2D_CC_TIME_SHIFT && deriv_order == 2)
-            raise(FD2D_EUNSUPPORTED, "no Hessian vector products for traveltime measurements (make_adjoint_sources.m:121)");
-        if (deriv_order == 2 && !du) raise(FD2D_EINVAL, "deriv_order 2 needs du");
-        hipStream_t st = as_stream(stream);
-        StreamScope scope(st);
-        // make_adjoint_sources.m:85-99: window limits against the time axis
-        const std::vector<double> th = to_host<double>(t, (size_t)nt);
-        double left = win_left, right = win_right;
-        if (measurement != FD2D_WAVEFORM_DIFFERENCE) {
-            if (left < 0) {
-                int64_t k = 0;
-                while (k < nt && th[(size_t)k] != 0.0) ++k;
-                if (k + 1 >= nt) raise(FD2D_EINVAL, "time axis has no t == 0 sample");
-                left = th[(size_t)k + 1];
-            }
-            if (right > th[(size_t)nt - 1]) right = th[(size_t)nt - 1];
-            if (left > th[(size_t)nt - 1]) raise(FD2D_EINVAL, "time series too short? (make_adjoint_sources.m:99)");
-        }
-        DBuf<double> scratch((size_t)3 * nr * nt);
-        AdjArgs a{u, u0, du, t, nr, nt, th[1] - th[0] >= 0 ? th[1] - th[0] : th[0] - th[1], veldis ? 1 : 0,
-                  measurement, deriv_order, left, right, misfit, adstf, scratch.p};
-        launch_adjoint_sources(a, st);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(st));
-    });
+    return adjoint_sources_entry(u, u0, du, t, nullptr, nr, nt, veldis, measurement, deriv_order, win_left, win_right,
+                                 misfit, adstf, stream, true);
+}
+
+int fd2d_make_adjoint_sources_queued(const double* u, const double* u0, const double* du, const double* t,
+                                     const double* t_host, int64_t nr, int64_t nt, int32_t veldis, int32_t measurement,
+                                     int32_t deriv_order, double win_left, double win_right, double* misfit,
+                                     double* adstf, void* stream)
+{
+    if (!t_host) {
+        g_err = "t_host is required";
+        return FD2D_EINVAL;
+    }
+    return adjoint_sources_entry(u, u0, du, t, t_host, nr, nt, veldis, measurement, deriv_order, win_left, win_right,
+                                 misfit, adstf, stream, false);
 }
 
 int fd2d_make_adjoint_sources(const double* u, const double* u0, const double* du, const double* t, int64_t nr,

@@ -101,6 +101,11 @@ struct StepArgs {
     const float* born_dmu;   // dmu as FP32 (compact) for C_in / u_in Born terms, or null
     const float* born_in0;   // C_in(:,:,n)
     const float* born_in1;   // C_in(:,:,n+1)
+
+    // structure-kernel product fused into the step (adjoint runs): K_mu -= kscale * (D u . D kfwd)
+    const float* kfwd;       // forward snapshot of this step (compact), or null
+    double* kacc;            // K_mu accumulator (compact, FP64)
+    double kscale;           // fw_nth
 };
 
 inline int node_key_host(int x, int z) { return ((z + 8) << 16) | (x + 8); }
@@ -129,12 +134,13 @@ struct JobStep {
     const float* born_dmu;  // stress-phase Born terms of this step (null: none)
     const float* born_in0;
     const float* born_in1;
+    const float* kfwd;      // forward snapshot of this step's structure-kernel product (null: none)
     float ps_const;
     int parity;  // which of the solve's two StepArgs (ping-pong direction) this step uses
     int slot;    // the solve's StepArgs are jobs[2 * slot + parity] of the grouped launch
 };
 
-constexpr int kGroupMax = 48;  // solves per grouped launch (kernel arguments stay under 4 KB)
+constexpr int kGroupMax = 40;  // solves per grouped launch (kernel arguments stay under 4 KB)
 
 // One time step of up to kGroupMax solves with the same order and field set in one launch: the
 // tiles of all solves are concatenated (solve j owns tiles tile_start[j] .. tile_start[j+1]-1)
@@ -157,25 +163,6 @@ void launch_step(int R, bool with_u, const StepArgs& a, hipStream_t s);
 void launch_pair(int R, bool with_u, const PairArgs& p, hipStream_t s);
 void launch_group(int R, bool with_u, const GroupArgs& g, hipStream_t s);
 
-// Structure-kernel products of up to kGroupMax solves (same FD order) in one launch; solve j owns
-// tiles tile_start[j] .. tile_start[j+1]-1 and its geometry is jobs[2 * job[j].slot].
-struct KmuJob {
-    const float* u;    // displacement after the step (padded array origin)
-    const float* fwd;  // forward snapshot (compact)
-    double* kmu;       // K_mu accumulator (compact)
-    double scale;
-    int slot;
-};
-struct KmuGroupArgs {
-    const StepArgs* jobs;
-    int n_jobs;
-    int total_tiles;
-    int tiles_per_xcd;
-    int tile_start[kGroupMax + 1];
-    KmuJob job[kGroupMax];
-};
-static_assert(sizeof(KmuGroupArgs) <= 4096, "grouped-launch arguments exceed the 4 KB kernel-argument limit");
-void launch_kmu_group(int R, const KmuGroupArgs& g, hipStream_t s);
 // K_mu -= scale * (D u . D fwd) over the grid (u padded, fwd compact), after a step.
 void launch_kmu(int R, const StepArgs& a, const float* u, const float* fwd, double* kmu, double scale,
                 hipStream_t s);

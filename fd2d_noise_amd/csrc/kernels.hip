@@ -120,6 +120,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     const unsigned cb = a.compact_bytes;
     const bool has_syn = st.syn != nullptr;
     const auto r_syn = rsrc(has_syn ? (const void*)st.syn : (const void*)a.fbase, has_syn ? cb : 0u);
+    const bool kp = HAS_U && st.kfwd != nullptr;  // this step takes a structure-kernel product
 
     // ---------------- phase 1
     constexpr int NRV = (HV + 3) / 4, NRZ = (HSZ + 3) / 4;
@@ -127,14 +128,14 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     float msx[NRV], msz[NRZ];
     float mv[NRV], mb[NRV], mu[NRV], msy[NRV];
     float hsx[NHS], hsz[NHZ];
-    float hv[NHV], hb[NHV], hsy[NHV], ht[NHV];
+    float hv[NHV], hb[NHV], hsy[NHV], ht[NHV], hu[NHV];
 #pragma unroll
     for (int k = 0; k < NRV; ++k) {
         const int r = w + 4 * k, z = j0 - (R - 1) + r;
         msx[k] = G.field(a.o_sxy_in, x, z);
         mv[k] = G.field(a.o_v_in, x, z);
         mb[k] = G.coef(a.o_bv, x, z);
-        mu[k] = (HAS_U && r >= R - 1 && r < R - 1 + TZ) ? G.field(a.o_u_in, x, z) : 0.f;
+        mu[k] = (HAS_U && ((r >= R - 1 && r < R - 1 + TZ) || kp)) ? G.field(a.o_u_in, x, z) : 0.f;
         msy[k] = has_syn ? bload(r_syn, G.cofs(x, z)) : 0.f;
     }
 #pragma unroll
@@ -167,6 +168,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         hb[k] = G.coef(a.o_bv, xx, zz);
         hsy[k] = has_syn ? bload(r_syn, G.cofs(xx, zz)) : 0.f;
         ht[k] = gld(a.axp, xx + kGhostX) * gld(a.azp, zz + kGhost);
+        hu[k] = (HAS_U && kp) ? G.field(a.o_u_in, xx, zz) : 0.f;
     }
     // LDS stores
 #pragma unroll
@@ -306,13 +308,82 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         G.put(a.o_sxy_out, x, z, snx);
         G.put(a.o_szy_out, x, z, snz);
     }
+
+    // ---------------- phase 4 (steps with a structure-kernel product)
+    // K_mu -= kscale * (D u . D u_fwd) with the displacement of this step, the forward snapshot kfwd
+    // and the dx_v / dz_v stencils on their frames (run_noise_adjoint.m:324-331). The displacement
+    // (in vv, in place) and the snapshot (in sx, row pitch WV) are staged in LDS over the velocity
+    // region, which covers the (R-1, R) halo of the stencils, so no other tile's output is read;
+    // the sums are those of k_kmu, term for term.
+    if (kp) {
+        const auto r_fw = rsrc(st.kfwd, cb);
+        float fm[NRV], fh[NHV];
+#pragma unroll
+        for (int k = 0; k < NRV; ++k) fm[k] = bload(r_fw, G.cofs(x, j0 - (R - 1) + w + 4 * k));
+#pragma unroll
+        for (int k = 0; k < NHV; ++k) {
+            const int idx = min(tid + 256 * k, HV * HCV - 1);
+            const int r = idx / HCV, h = idx - r * HCV;
+            fh[k] = bload(r_fw, G.cofs(i0 - (R - 1) + (h < R - 1 ? h : h + kTX), j0 - (R - 1) + r));
+        }
+        __syncthreads();  // phase 3 has read vv, sx and sz
+#pragma unroll
+        for (int k = 0; k < NRV; ++k) {
+            const int r = w + 4 * k;
+            if (r >= HV) break;
+            float* cell = vv + r * WV + lane + R - 1;
+            *cell = mu[k] + *cell * a.dt;
+            sx[r * WV + lane + R - 1] = fm[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NHV; ++k) {
+            const int idx = min(tid + 256 * k, HV * HCV - 1);
+            const int r = idx / HCV, h = idx - r * HCV;
+            const int o = tid + 256 * k < HV * HCV ? r * WV + (h < R - 1 ? h : h + kTX) : HV * WV;
+            vv[o] = hu[k] + vv[o] * a.dt;
+            sx[o] = fh[k];
+        }
+        __syncthreads();
+        const bool xin = x >= R - 1 && x <= nx - R - 1;
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const int lz = w + 4 * k, z = j0 + lz;
+            const unsigned co = G.cofs(x, z);
+            const bool zin = z >= R - 1 && z <= nz - R - 1;
+            if (co == kOutside || (!xin && !zin)) continue;
+            const int o = (lz + R - 1) * WV + c;
+            const float* urow = vv + o;
+            const float* frow = sx + o;
+            double acc = 0.0;
+            if (xin) {
+                float su = 0.f, sf = 0.f;
+#pragma unroll
+                for (int m = 1; m <= R; ++m) {
+                    su += a.cx[m - 1] * (urow[m] - urow[1 - m]);
+                    sf += a.cx[m - 1] * (frow[m] - frow[1 - m]);
+                }
+                acc += (double)su * (double)sf;
+            }
+            if (zin) {
+                float su = 0.f, sf = 0.f;
+#pragma unroll
+                for (int m = 1; m <= R; ++m) {
+                    su += a.cz[m - 1] * (urow[m * WV] - urow[(1 - m) * WV]);
+                    sf += a.cz[m - 1] * (frow[m * WV] - frow[(1 - m) * WV]);
+                }
+                acc += (double)su * (double)sf;
+            }
+            const int i = (int)(co / 4u);
+            gst(a.kacc, i, gld(a.kacc, i) - acc * a.kscale);
+        }
+    }
 }
 
 template <int R, int MODE>
 __global__ __launch_bounds__(256) void k_tile(StepArgs a)
 {
     const int g = blockIdx.x;
-    const JobStep st{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.born_dmu, a.born_in0, a.born_in1,
+    const JobStep st{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.born_dmu, a.born_in0, a.born_in1, a.kfwd,
                      a.ps_const, 0, 0};
     tile_step<R, MODE>(a, st, (g % kXCD) * a.tiles_per_xcd + g / kXCD);
 }
@@ -699,33 +770,6 @@ __global__ __launch_bounds__(256) void k_kmu(StepArgs a, const float* __restrict
 {
     const int g = blockIdx.x;
     kmu_tile<R>(a, u, fwd, kmu, scale, (g % kXCD) * a.tiles_per_xcd + g / kXCD);  // kernel arguments: global
-}
-
-template <int R>
-__global__ __launch_bounds__(256) void k_kmu_group(KmuGroupArgs G)
-{
-    const int g = blockIdx.x;
-    const int T = (g % kXCD) * G.tiles_per_xcd + g / kXCD;
-    if (T >= G.total_tiles) return;
-    int j = 0;
-    while (j + 1 < G.n_jobs && G.tile_start[j + 1] <= T) ++j;
-    j = __builtin_amdgcn_readfirstlane(j);
-    const KmuJob jb = G.job[j];
-    const StepArgs a = G.jobs[2 * jb.slot];
-    kmu_tile<R>(a, (const __attribute__((address_space(1))) float*)jb.u,
-                (const __attribute__((address_space(1))) float*)jb.fwd,
-                (__attribute__((address_space(1))) double*)jb.kmu, jb.scale, T - G.tile_start[j]);
-}
-
-void launch_kmu_group(int R, const KmuGroupArgs& ga, hipStream_t s)
-{
-    const dim3 g((unsigned)(kXCD * ga.tiles_per_xcd)), b(256);
-    switch (R) {
-    case 1: hipLaunchKernelGGL(k_kmu_group<1>, g, b, 0, s, ga); break;
-    case 2: hipLaunchKernelGGL(k_kmu_group<2>, g, b, 0, s, ga); break;
-    case 3: hipLaunchKernelGGL(k_kmu_group<3>, g, b, 0, s, ga); break;
-    default: hipLaunchKernelGGL(k_kmu_group<4>, g, b, 0, s, ga); break;
-    }
 }
 
 void launch_kmu(int R, const StepArgs& a, const float* u, const float* fwd, double* kmu, double scale, hipStream_t s)

@@ -30,6 +30,11 @@ def _bind():
                                           ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]
     g.restype = ctypes.c_int
+    q = lib.fd2d_make_adjoint_sources_queued
+    q.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p]
+    q.restype = ctypes.c_int
     return f, g
 
 
@@ -59,17 +64,30 @@ def make_adjoint_sources(u, u_0, du, t, veldis, measurement, src=None, rec=None,
 
 
 def make_adjoint_sources_device(u, u_0, du, t, veldis, measurement, deriv_order="1st", window=(100.0, 200.0),
-                                misfit=None, adstf=None):
-    """Device form: u, u_0, du (nt, nr) float64 CUDA tensors (MATLAB nr x nt), t an nt tensor."""
+                                misfit=None, adstf=None, t_host=None):
+    """Device form: u, u_0, du (nt, nr) float64 CUDA tensors (MATLAB nr x nt), t an nt tensor. With
+    t_host (the same time axis as a host float64 array) the work is only enqueued on the current
+    stream (fd2d_make_adjoint_sources_queued); without it the call waits for the stream."""
     import torch
-    _, g = _bind()
+    lib = _lib.load()
+    _bind()
     nt, nr = u.shape
     misfit = torch.zeros(nr, dtype=torch.float64, device=u.device) if misfit is None else misfit
     adstf = torch.zeros((nt, nr), dtype=torch.float64, device=u.device) if adstf is None else adstf
     v, m, o = _codes(veldis, measurement, deriv_order)
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    st = g(u.data_ptr(), u_0.data_ptr(), du.data_ptr() if du is not None else None, t.data_ptr(), nr, nt, v, m, o,
-           float(window[0]), float(window[1]), misfit.data_ptr(), adstf.data_ptr(), stream)
+    dup = du.data_ptr() if du is not None else None
+    if t_host is not None:
+        th = np.ascontiguousarray(t_host, dtype=np.float64)
+        if th.size != nt:
+            raise ValueError("t_host must hold nt samples")
+        st = lib.fd2d_make_adjoint_sources_queued(u.data_ptr(), u_0.data_ptr(), dup, t.data_ptr(), th.ctypes.data, nr,
+                                                  nt, v, m, o, float(window[0]), float(window[1]), misfit.data_ptr(),
+                                                  adstf.data_ptr(), stream)
+    else:
+        st = lib.fd2d_make_adjoint_sources_device(u.data_ptr(), u_0.data_ptr(), dup, t.data_ptr(), nr, nt, v, m, o,
+                                                  float(window[0]), float(window[1]), misfit.data_ptr(),
+                                                  adstf.data_ptr(), stream)
     if st != abi.FD2D_OK:
         raise abi.Fd2dError(st, _lib.last_error())
     return misfit, adstf

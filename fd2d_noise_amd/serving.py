@@ -75,8 +75,9 @@ def _host(a) -> np.ndarray:
     return np.asfortranarray(np.atleast_2d(np.asarray(a, dtype=np.float64)))
 
 
-def _adjoint_stf(corr, observed, du, t, req: Request, order: str):
-    _, adstf = make_adjoint_sources_device(corr, observed, du, t, req.veldis, req.measurement, order, req.window)
+def _adjoint_stf(corr, observed, du, m: "Model", req: Request, order: str):
+    _, adstf = make_adjoint_sources_device(corr, observed, du, m.t, req.veldis, req.measurement, order, req.window,
+                                           t_host=m.config.t_corr())
     stf = torch.zeros(adstf.shape, dtype=cd, device=adstf.device)
     torch.view_as_real(stf)[..., 0].copy_(adstf)
     return stf
@@ -113,7 +114,7 @@ def kernel_program(srv: "Server", req: Request):
     C = D.empty_like_dims((c.nx, c.nz, c.n_fw_corr()), f32)
     yield [B.correlation(m.mu, m.rho, G, m.spectrum_host, m.source_dist, rec, 1, None, None, config=c,
                          seismograms=corr, C_out=C)]
-    stf = _adjoint_stf(corr, req.observed, None, m.t, req, "1st")
+    stf = _adjoint_stf(corr, req.observed, None, m, req, "1st")
     nbk = max(c.n_basis_fct, 1) + 1
     K1 = D.empty_like_dims((c.nx, c.nz, nbk), f64)
     A1 = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
@@ -150,8 +151,8 @@ def hvp_program(srv: "Server", req: Request):
     c_pert = D.empty_like_dims((nr, N), f64)
     dC = D.empty_like_dims((c.nx, c.nz, c.n_fw_corr()), f32)
     yield [B.correlation(mu, rho, dG, spec, sd, rec, 1, dmu, C, config=c, seismograms=c_pert, C_out=dC)]  # :75
-    stf1 = _adjoint_stf(c_ref, req.observed, c_pert, m.t, req, "1st")  # :99-105
-    stf2 = _adjoint_stf(c_ref, req.observed, c_pert, m.t, req, "2nd")
+    stf1 = _adjoint_stf(c_ref, req.observed, c_pert, m, req, "1st")  # :99-105
+    stf2 = _adjoint_stf(c_ref, req.observed, c_pert, m, req, "2nd")
     H = torch.zeros((nbk, c.nz, c.nx), dtype=f64, device="cuda")
     KY = D.empty_like_dims((c.nx, c.nz, nbk), f64)
     uT = D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)

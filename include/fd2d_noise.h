@@ -225,6 +225,14 @@ int fd2d_make_adjoint_sources_device(const double* u, const double* u0, const do
                                      int32_t deriv_order, double win_left, double win_right, double* misfit,
                                      double* adstf, void* stream);
 
+/* The same, enqueued on `stream` without waiting for it: t_host is the time axis t in host memory
+ * (read for the window limits), t stays on the device for the kernels. For callers that queue
+ * solves and caller work on one stream (the serving loop). */
+int fd2d_make_adjoint_sources_queued(const double* u, const double* u0, const double* du, const double* t,
+                                     const double* t_host, int64_t nr, int64_t nt, int32_t veldis,
+                                     int32_t measurement, int32_t deriv_order, double win_left, double win_right,
+                                     double* misfit, double* adstf, void* stream);
+
 /* x'*y of two device FP64 vectors (n = 0: 0, pointers may be NULL) with compensated summation (the inner products of
  * inversion/optlib/optlib_lbfgs.m:51,92,102,110,175 and optlib_wolfe.m:59,85,105); result to host. */
 int fd2d_dot_device(const double* x, const double* y, int64_t n, double* result, void* stream);
