@@ -3,9 +3,9 @@
 Each solve of a batch must give exactly (bit for bit) what its own *_device call gives: the engine
 runs the same tile kernel body over the same data, only with the tiles of several solves in one
 launch. The mix covers the three solvers, two grid sizes under one FD order (heterogeneous nx/nz
-and nt in one grouped launch), a second FD order, Born solves (stepped one launch each), point and
-distributed adjoint sources, snapshots, DFT outputs, seismograms and both kernels. The
-single-solve path itself is held to the oracle by test_gpu_parity.py.
+and nt in one grouped launch), a second FD order, Born solves, point and distributed adjoint
+sources, snapshots, DFT outputs, seismograms and both kernels (the structure-kernel product fused
+into the grouped step). The single-solve path itself is held to the oracle by test_gpu_parity.py.
 """
 import numpy as np
 import pytest
@@ -189,3 +189,34 @@ def test_engine_rejects_bad_solves(torch_mod, mix):
         assert eng.advance(5) == 0
         with pytest.raises(abi.Fd2dError):
             eng.done(12345)
+
+
+def test_calculate_data_matches_station_loop(torch_mod, tmp_path):
+    """formats.calculate_data (code/calculate_data.m:180-223, stations batched) equals the
+    per-station loop of single solves, and its files load back as the inversion reads them."""
+    torch = torch_mod
+    from fd2d_noise_amd import device as D, formats as F
+    c = Config(order=4, nx=64, nz=56, nt=110, n_basis_fct=2)
+    rng = np.random.default_rng(77)
+    mu, rho = hetero_model(rng, c)
+    array = interior_points(rng, c, 5)
+    ref = array[[0, 2, 4]]
+    spectrum = np.ones((c.n_sample, 1))
+    sd = blob_distribution(rng, c, nb=2)
+    out = F.calculate_data(c, mu, rho, spectrum, sd, array, ref, out_dir=str(tmp_path))
+    rows = []
+    for i in range(ref.shape[0]):
+        rec = array[~np.all(array == ref[i], axis=1)]
+        G = D.empty_like_dims((c.nx, c.nz, c.n_sample), torch.complex128)
+        D.run_forward1_green(D.to_device(mu), D.to_device(rho), D.to_device(ref[i:i + 1]), None, 0, None, None,
+                             config=c, G_fft=G)
+        tr = D.empty_like_dims((rec.shape[0], c.n_corr), torch.float64)
+        D.run_forward2_correlation(D.to_device(mu), D.to_device(rho), G, D.to_device(spectrum), D.to_device(sd),
+                                   D.to_device(rec), 0, None, None, config=c, seismograms=tr)
+        rows.append(D.to_host(tr))
+    ref_data = np.concatenate(rows, axis=0)
+    assert out["c_data"].shape == (3 * 4, c.n_corr) and np.array_equal(out["c_data"], ref_data)
+    dat = F.load_data(str(tmp_path / "data_3_ref_2.mat"))
+    net = F.load_array(str(tmp_path / "array_3_ref.mat"))
+    assert np.array_equal(dat["c_data"], ref_data) and dat["t"].shape == (1, c.n_corr)
+    assert np.array_equal(net["ref_stat"], ref) and np.array_equal(net["array"], array)
