@@ -44,12 +44,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=None,
                     help="time steps (default: the workload's, 8000 for the pipeline, 4000 for stations)")
-    ap.add_argument("--workload", choices=["pipeline", "stations", "serving"], default="pipeline",
+    ap.add_argument("--workload", choices=["pipeline", "stations", "serving", "inversion"], default="pipeline",
                     help="pipeline: the single large solve (the benchmark line); stations: the batched "
                          "reference-station workload (1200x1200, order 4, 64 stations x 4 basis functions) "
                          "through the batch engine against the plain single-solve path; serving: a Poisson "
                          "stream of mixed correlation / kernel / Hessian-vector requests through the serving "
                          "loop against the same requests run one solve at a time")
+    ap.add_argument("--iterations", type=int, default=10, help="L-BFGS iterations of the inversion workload")
     ap.add_argument("--rates", type=str, default="5,20,200", help="arrival rates [req/s] of the serving workload")
     ap.add_argument("--requests", type=int, default=48, help="requests per rate of the serving workload")
     ap.add_argument("--stations", type=int, default=64, help="stations of the stations workload")
@@ -376,6 +377,85 @@ def serving_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def inversion_bench(args):
+    """Structure inversion of the 20-station ring: observed correlations from the true model
+    (calculate_data, stations batched), one objective + gradient evaluation of the initial model on
+    the plain path (stations one after another) and batched (compared bit for bit), then L-BFGS
+    iterations with the batched problem."""
+    import torch
+    from fd2d_noise_amd import formats as F, inversion as V
+    from fd2d_noise_amd.workloads import config4_workload
+    w = config4_workload(nt=args.nt or 3000)
+    c = w["config"]
+    P, n_ref = c.nx * c.nz, w["ref_stat"].shape[0]
+    print(f"# workload config4 {c.nx}x{c.nz} order {c.order} nt {c.nt} stations {n_ref}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    obs = F.calculate_data(c, w["mu_true"], w["rho"], w["spectrum"], w["source_dist"], w["array"], w["ref_stat"])
+    t_data = time.perf_counter() - t0
+    print(f"# observed data: {t_data:.1f} s", file=sys.stderr, flush=True)
+    usr = V.UsrPar(config=c, ref_stat=w["ref_stat"], array=w["array"], c_data=obs["c_data"], t=c.t_corr(),
+                   rho=w["rho"], spectrum=w["spectrum"], type="structure", mu_0=w["mu_0"])
+    m0 = torch.as_tensor(np.concatenate([np.ones(P), np.zeros(P)]), device="cuda")
+    steps_eval = n_ref * (c.nt + c.n_corr + c.n_corr + c.nt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    j1, g1, _ = V.NoiseProblem(usr).eval_objective_and_gradient(m0)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    prob = V.NoiseProblem(usr, batched=True)
+    t0 = time.perf_counter()
+    j2, g2, _ = prob.eval_objective_and_gradient(m0)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    same = (j1 == j2) and bool(torch.equal(g1, g2))
+    print(f"# objective+gradient: plain {t_plain:.2f} s, batched {t_batch:.2f} s, identical {same}",
+          file=sys.stderr, flush=True)
+
+    class Counting:
+        def __init__(self):
+            self.n = 0
+
+        def eval_objective_and_gradient(self, m, name=""):
+            self.n += 1
+            return prob.eval_objective_and_gradient(m, name)
+
+        def eval_objective(self, m, name=""):
+            self.n += 1
+            return prob.eval_objective(m, name)
+
+        def eval_grad_objective(self, m, name=""):
+            self.n += 1
+            return prob.eval_grad_objective(m, name)
+    cp = Counting()
+    # first step: a 1% relative perturbation of the structure map at most (start_inversion.m:118-122
+    # sets init_step_length per inversion type by hand; optlib_lbfgs.m:119-121 leaves this guess open)
+    sigma0 = 0.01 / float(g2.abs().max())
+    t0 = time.perf_counter()
+    flag, m, usr = V.optlib_lbfgs(m0, V.Options(max_iterations=args.iterations, successive_change=0.0,
+                                                init_step_length=sigma0), usr,
+                                  problem=cp, rng=np.random.default_rng(0))
+    torch.cuda.synchronize()
+    t_inv = time.perf_counter() - t0
+    objs = [float(mod["objective"]) for mod in usr.model]
+    print(f"# L-BFGS: {len(objs) - 1} iterations, {cp.n} evaluations, {t_inv:.1f} s, objective {objs[0]:.4g} -> "
+          f"{objs[-1]:.4g}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "Gpt-updates/s, correlations+kernels/s, p99 latency; FP32 vs plain, L2<=1e-5",
+        "value": steps_eval * P / t_batch, "unit": "GP-updates/s", "n_gpus": 1, "steps": len(objs) - 1,
+        "warmup": 0, "ms_per_step": t_inv / max(len(objs) - 1, 1) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 fields, f64 DFT accumulators/kernels/L-BFGS", "data": "synthetic",
+        "config": {"workload": f"config4: structure inversion, {c.nx}x{c.nz} order-{c.order} nt={c.nt}, fw_nth=1, "
+                               f"{n_ref}-station ring, {c.n_sample} frequencies, L-BFGS {args.iterations} iterations",
+                   "nx": c.nx, "nz": c.nz, "nt": c.nt, "stations": n_ref},
+        "objective_gradient_s": {"plain": t_plain, "batched": t_batch, "batched_vs_plain": t_plain / t_batch,
+                                 "identical": same},
+        "lbfgs": {"flag": int(flag), "iterations": len(objs) - 1, "evaluations": cp.n, "wall_s": t_inv,
+                  "objectives": objs},
+        "observed_data_s": t_data,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -409,6 +489,9 @@ def main():
         return
     if args.workload == "serving":
         serving_bench(args)
+        return
+    if args.workload == "inversion":
+        inversion_bench(args)
         return
     w = config2_workload(nt=args.nt or 8000)
     c = w["config"]

@@ -275,3 +275,30 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+def run_programs(programs: Sequence, rounds: int = 64) -> list:
+    """Runs solve programs together through one engine and returns their results in order. A
+    program is a generator that yields the solves of its next stage (a list of SolveSpec), is
+    resumed once all of them are complete, and returns its result; the stages of all programs share
+    the engine's grouped launches."""
+    progs = list(programs)
+    results = [None] * len(progs)
+    pending = {}
+    with Engine() as eng:
+        def resume(q):
+            try:
+                specs = next(progs[q])
+            except StopIteration as fin:
+                results[q] = fin.value
+                pending.pop(q, None)
+                return
+            pending[q] = (specs, [eng.submit(s) for s in specs])
+        for q in range(len(progs)):
+            resume(q)
+        while pending:
+            eng.advance(rounds)
+            for q in [q for q, (_, ts) in pending.items() if all(eng.done(t) for t in ts)]:
+                resume(q)
+    torch.cuda.synchronize()
+    return results

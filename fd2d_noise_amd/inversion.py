@@ -234,11 +234,14 @@ class NoiseProblem:
     sum in HBM; the gradient is mapped back to m and regularised. For type 'source' the G_fft of
     each station is kept in HBM across calls, as the reference keeps G_fft_<i>.mat (:44-53)."""
 
-    def __init__(self, usr: UsrPar):
+    def __init__(self, usr: UsrPar, batched: bool = False):
         import torch
         from . import device as D
         usr.check()
         self.usr, self.D, self.torch = usr, D, torch
+        # batched: the solves of all reference stations run together through one batch engine (the
+        # reference's parfor over stations, eval_objective_and_gradient.m:54); results are identical
+        self.batched = batched
         c = usr.config
         self.c = c
         self.rho = D.to_device(usr.rho)
@@ -272,7 +275,101 @@ class NoiseProblem:
                     stf=torch.zeros((c.n_corr, nr), dtype=cd, device="cuda"),
                     A1=D.empty_like_dims((c.nx, c.nz, c.n_sample), cd))
 
+    def _station_program(self, i, mu, rho, spec, source_dist, only_objective):
+        """The solves of reference station i (eval_objective_and_gradient.m:54-104) as a program for
+        batch.run_programs: yields each stage's solves, returns (traces, j_source_n, j_structure_n,
+        [kernels])."""
+        from . import batch as B
+        from .measurement import make_adjoint_sources_device
+        torch, D, c, usr = self.torch, self.D, self.c, self.usr
+        src = np.asfortranarray(self.ref[i:i + 1])
+        rec = np.asfortranarray(self.recs[i])
+        nr = self.n_rec
+        spec_h = np.asfortranarray(D.to_host(spec)) if torch.is_tensor(spec) else spec
+        corr = D.empty_like_dims((nr, c.n_corr), torch.float64)
+        if usr.type == "source":
+            if i not in self.g_cache:
+                G_fft = D.empty_like_dims((c.nx, c.nz, c.n_sample), torch.complex128)
+                yield [B.green(mu, rho, src, None, 0, None, None, config=c, G_fft=G_fft)]
+                self.g_cache[i] = G_fft
+            G_fft = self.g_cache[i]
+            yield [B.correlation(mu, rho, G_fft, spec_h, source_dist, rec, 0, None, None, config=c, seismograms=corr)]
+        else:
+            G_fft = D.empty_like_dims((c.nx, c.nz, c.n_sample), torch.complex128)
+            G = D.empty_like_dims((c.nx, c.nz, c.n_fw_green()), torch.float32)
+            yield [B.green(mu, rho, src, None, 1, None, None, config=c, G_fft=G_fft, G_out=G)]
+            C = D.empty_like_dims((c.nx, c.nz, c.n_fw_corr()), torch.float32)
+            yield [B.correlation(mu, rho, G_fft, spec_h, source_dist, rec, 1, None, None, config=c, seismograms=corr,
+                                 C_out=C)]
+        jn_s, a_s = make_adjoint_sources_device(corr, self.c_data[i], self.zeros_du, self.t, usr.veldis,
+                                                usr.measurement_source, "1st", usr.window)
+        jn_t, a_t = make_adjoint_sources_device(corr, self.c_data[i], self.zeros_du, self.t, usr.veldis,
+                                                usr.measurement_structure, "1st", usr.window)
+        w = float(usr.kernel_weighting)
+        adstf = axpby(1.0 - w, a_s, w, a_t)  # :81-82
+        jn_s_h, jn_t_h = jn_s.cpu().numpy(), jn_t.cpu().numpy()
+        if only_objective:
+            return corr, jn_s_h, jn_t_h, []
+        nbk = max(c.n_basis_fct, 1) + 1
+        stf = torch.zeros((c.n_corr, nr), dtype=torch.complex128, device="cuda")
+        torch.view_as_real(stf)[..., 0].copy_(adstf)  # complex(adstf)
+        K1 = D.empty_like_dims((c.nx, c.nz, nbk), torch.float64)
+        if usr.type == "source":
+            yield [B.adjoint(mu, rho, None, stf, rec, None, spec_h, None, G_fft, 0, None, None, config=c, K=K1)]
+            return corr, jn_s_h, jn_t_h, [K1]
+        A1 = D.empty_like_dims((c.nx, c.nz, c.n_sample), torch.complex128)
+        yield [B.adjoint(mu, rho, C, stf, rec, None, spec_h if usr.type == "joint" else None, None,
+                         G_fft if usr.type == "joint" else None, 1, None, None, config=c, K=K1, stf_fft=A1)]
+        del C
+        K2 = D.empty_like_dims((c.nx, c.nz, nbk), torch.float64)
+        yield [B.adjoint(mu, rho, G, A1, rec, None, spec_h, source_dist, None, 0, None, None, config=c, K=K2)]
+        return corr, jn_s_h, jn_t_h, [K1, K2]
+
+    def _eval_batched(self, m, only_objective):
+        from . import batch as B
+        torch, c, usr = self.torch, self.c, self.usr
+        Pm = map_m_to_parameters(m, usr)
+        n_maps = Pm.shape[0]
+        nb_k = max(c.n_basis_fct, 1) + 1
+        source_dist, mu = Pm[:n_maps - 1], Pm[n_maps - 1]
+        n_ref, nr, N = self.ref.shape[0], self.n_rec, c.n_corr
+        # stations per batch: what fits in 70% of the free device memory (the arrays a station holds
+        # at its peak: spectra, forward snapshots, the library's slabs and spectra)
+        P, K = c.nx * c.nz, c.n_sample
+        per = 16 * P * K * 3 + 4 * P * (c.n_fw_green() + c.n_fw_corr()) + 4 * P * 512 * 2 + 8 * P * 8
+        free, _ = torch.cuda.mem_get_info()
+        chunk = max(1, min(n_ref, int(0.7 * free) // per))
+        out = []
+        for i0 in range(0, n_ref, chunk):
+            out += B.run_programs([self._station_program(i, mu, self.rho, self.spectrum, source_dist, only_objective)
+                                   for i in range(i0, min(n_ref, i0 + chunk))])
+        c_all = torch.zeros((N, n_ref * nr), dtype=torch.float64, device="cuda")
+        grad = torch.zeros((nb_k, c.nz, c.nx), dtype=torch.float64, device="cuda")
+        w = float(usr.kernel_weighting)
+        j = j_source = j_structure = 0.0
+        misfits = []
+        for i, (corr, jn_s_h, jn_t_h, Ks) in enumerate(out):  # station order, as the sequential loop
+            c_all[:, i * nr:(i + 1) * nr].copy_(corr)
+            j_n = (1.0 - w) * jn_s_h + w * jn_t_h
+            misfits.append(j_n)
+            for K in Ks:
+                axpby(1.0, grad.view(-1), 1.0, K.view(-1), out=grad.view(-1))
+            j = j + float(np.sum(j_n))
+            j_source = j_source + float(np.sum(jn_s_h))
+            j_structure = j_structure + float(np.sum(jn_t_h))
+        self.last_misfits = {"total": j, "source": j_source, "structure": j_structure, "per_station": misfits}
+        if only_objective:
+            j, _ = regularization(j, None, m, usr)
+            return j, 0, c_all
+        if grad.shape[0] != n_maps:
+            raise ValueError(f"m holds {n_maps} maps but the kernels {grad.shape[0]} (n_basis_fct + 1)")
+        g = map_gradparameters_to_gradm(grad, usr)
+        j, g = regularization(j, g, m, usr)
+        return j, g, c_all
+
     def eval_objective_and_gradient(self, m, name: str = "", only_objective: bool = False):
+        if self.batched:
+            return self._eval_batched(m, only_objective)
         from .measurement import make_adjoint_sources_device
         torch, D, c, usr = self.torch, self.D, self.c, self.usr
         Pm = map_m_to_parameters(m, usr)

@@ -14,6 +14,11 @@ config3: batched reference stations -- 1200 x 1200 nodes on 2000 km (dx = 1.67 k
          dt = 0.12 s, nt = 4000, 128 frequencies (0.01-0.2 Hz, every 5th step), 4 source basis
          functions (seeded blobs, flat spectrum as after general_source), 64 reference stations
          seeded uniformly inside the absorbing frame and 32 receivers shared by all stations.
+config4: inversion -- 600 x 600 nodes on 2000 km (dx = 3.34 km), order 4, dt = 0.25 s, nt = 3000, 91
+         frequencies in 0.01-0.10 Hz, forward fields stored every step, a 20-station ring (radius
+         600 km) whose stations are each other's receivers, a uniform noise source, and a true model
+         with 10 seeded Gaussian vs anomalies (radius 40-120 km, +-8%) inside the ring; the initial
+         model is homogeneous (4 km/s).
 serving: a Poisson stream of mixed requests -- 50% correlation, 35% correlation + structure kernel,
          15% structure Hessian-vector product -- on resident models of three grid sizes (200, 283 and
          400 nodes square, log-spaced; dx = 2 km, nt = 10 nx) at FD order 4 or 8, vs = 4 km/s times
@@ -145,3 +150,30 @@ def serving_stream(models: dict, rate: float, n_requests: int, seed: int = 6, mi
         rec = np.stack([rng.uniform(w, c.Lx - w, 8), rng.uniform(w, c.Lz - w, 8)], axis=1)
         out.append((float(t[q]), str(kinds[q]), name, src, rec, int(rng.integers(1 << 30))))
     return out
+
+
+def config4_workload(seed: int = 4, nt: int = 3000, n_stations: int = 20) -> dict:
+    rng = np.random.default_rng(seed)
+    c = Config(Lx=2.0e6, Lz=2.0e6, nx=600, nz=600, dt=0.25, nt=nt, order=4, fw_nth=1,
+               f_sample=np.linspace(0.01, 0.10, 91), freq_samp=5)
+    c.absorb_width = 30 * c.dx
+    centre = np.array([c.Lx / 2, c.Lz / 2])
+    ang = 2 * np.pi * np.arange(n_stations) / n_stations
+    ring = np.stack([centre[0] + 600e3 * np.cos(ang), centre[1] + 600e3 * np.sin(ang)], axis=1)
+    X = c.x()[:, None]
+    Z = c.z()[None, :]
+    anomaly = np.zeros((c.nx, c.nz))
+    for _ in range(10):
+        r = rng.uniform(0, 450e3)
+        a = rng.uniform(0, 2 * np.pi)
+        rad = rng.uniform(40e3, 120e3)
+        amp = rng.choice([-1.0, 1.0]) * 0.08
+        x0, z0 = centre[0] + r * np.cos(a), centre[1] + r * np.sin(a)
+        anomaly += amp * np.exp(-((X - x0) ** 2 + (Z - z0) ** 2) / (2 * rad ** 2))
+    vs_true = 4000.0 * (1 + anomaly)
+    rho = RHO0 * np.ones((c.nx, c.nz), order="F")
+    f = np.asarray(c.f_sample)
+    return dict(name="config4", config=c, rho=rho, mu_true=np.asfortranarray(rho * vs_true ** 2),
+                mu_0=MU0, array=ring, ref_stat=ring.copy(),
+                spectrum=np.exp(-((f - 0.05) ** 2) / (2 * 0.02 ** 2))[:, None],
+                source_dist=np.ones((c.nx, c.nz), order="F"))

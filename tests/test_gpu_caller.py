@@ -234,6 +234,33 @@ def test_noise_objective_and_gradient_device(orc, typ):
     assert jo == pytest.approx(j, rel=1e-12)
 
 
+@pytest.mark.parametrize("typ", ["source", "structure", "joint"])
+def test_noise_objective_batched_equals_sequential(orc, typ):
+    """All stations through one batch engine (NoiseProblem(batched=True)) give the sequential
+    station loop's objective, gradient and traces bit for bit."""
+    import torch
+    from fd2d_noise_amd import inversion as V
+    from oracle import inversion as OI
+    from test_oracle_inversion import _noise_usr
+    usr, m_true, _ = _noise_usr(typ)
+    # three reference stations, each recorded at the other three points of the array
+    usr.array = np.array([[2.0e4, 3.0e4], [4.0e4, 3.5e4], [3.5e4, 2.0e4], [2.6e4, 4.0e4]])
+    usr.ref_stat = usr.array[:3].copy()
+    c = usr.config
+    P = c.nx * c.nz
+    usr.c_data = np.zeros((9, c.n_corr))
+    _, _, cd = OI.eval_objective_and_gradient(m_true, usr, orc, only_objective=True)
+    usr.c_data = cd
+    assert usr.c_data.shape[0] == 9
+    m = torch.as_tensor(np.concatenate([np.ones(P), np.zeros(P)]), device="cuda")
+    j1, g1, c1 = V.NoiseProblem(usr).eval_objective_and_gradient(m)
+    j2, g2, c2 = V.NoiseProblem(usr, batched=True).eval_objective_and_gradient(m)
+    print(f"{typ}: batched j {j2!r} sequential {j1!r}")
+    assert j1 == j2 and torch.equal(g1, g2) and torch.equal(c1, c2)
+    jo, _ = V.NoiseProblem(usr, batched=True).eval_objective(m)
+    assert jo == V.NoiseProblem(usr).eval_objective(m)[0]
+
+
 def test_noise_lbfgs_decreases_misfit(orc):
     import torch
     from fd2d_noise_amd import inversion as V
