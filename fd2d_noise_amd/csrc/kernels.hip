@@ -420,10 +420,11 @@ __global__ __launch_bounds__(256) void k_tile_group(GroupArgs G)
 // Per-thread layout as in k_tile: lane l owns column i0 + l of every region's main block, wave w
 // the rows w, w+4, ...; halo columns are packed lists over the 256 threads.
 template <int R, int MODE>
-__global__ __launch_bounds__(256) void k_pair(PairArgs pa)
+__global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
 {
     constexpr bool HAS_U = MODE >= 1;
     constexpr int TZ = kTileZ;
+    constexpr int PT = kPairThreads, PW = PT / 64;  // threads and wavefronts of the workgroup
     constexpr int L1 = 3 * R - 2, R1 = 3 * R - 1, LS = 4 * R - 2;
     constexpr int W1 = kTX + L1 + R1, H1 = TZ + L1 + R1;
     constexpr int WSX = kTX + 2 * LS, HSZ = TZ + 2 * LS;
@@ -457,34 +458,34 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     auto frame = [](int v, int lo, int hi) { return (v >= lo && v <= hi) ? 1.f : 0.f; };
 
     // ---------------- phase A: stage sigma, load the step-n velocity operands of V1
-    constexpr int NM1 = (H1 + 3) / 4, NH1 = (H1 * HC1 + 255) / 256;
-    constexpr int NMZ = (HSZ + 3) / 4;
-    constexpr int NHX = (H1 * HCS + 255) / 256, NHZ = (HSZ * HC1 + 255) / 256;
+    constexpr int NM1 = (H1 + PW - 1) / PW, NH1 = (H1 * HC1 + PT - 1) / PT;
+    constexpr int NMZ = (HSZ + PW - 1) / PW;
+    constexpr int NHX = (H1 * HCS + PT - 1) / PT, NHZ = (HSZ * HC1 + PT - 1) / PT;
     float mv[NM1], mb[NM1], msy[NM1], mu[NM1];
     float hv[NH1], hb[NH1], ht[NH1], hsy[NH1];
     {
         float sxm[NM1], sxh[NHX], szm[NMZ], szh[NHZ];
 #pragma unroll
-        for (int k = 0; k < NM1; ++k) sxm[k] = G.field(a.o_sxy_in, x, j0 - L1 + w + 4 * k);
+        for (int k = 0; k < NM1; ++k) sxm[k] = G.field(a.o_sxy_in, x, j0 - L1 + w + PW * k);
 #pragma unroll
         for (int k = 0; k < NHX; ++k) {
-            const int idx = min(tid + 256 * k, H1 * HCS - 1);
+            const int idx = min(tid + PT * k, H1 * HCS - 1);
             const int r = idx / HCS, h = idx - r * HCS;
             const int c = h < LS ? h : h + kTX;
             sxh[k] = G.field(a.o_sxy_in, i0 - LS + c, j0 - L1 + r);
         }
 #pragma unroll
-        for (int k = 0; k < NMZ; ++k) szm[k] = G.field(a.o_szy_in, x, j0 - LS + w + 4 * k);
+        for (int k = 0; k < NMZ; ++k) szm[k] = G.field(a.o_szy_in, x, j0 - LS + w + PW * k);
 #pragma unroll
         for (int k = 0; k < NHZ; ++k) {
-            const int idx = min(tid + 256 * k, HSZ * HC1 - 1);
+            const int idx = min(tid + PT * k, HSZ * HC1 - 1);
             const int r = idx / HC1, h = idx - r * HC1;
             const int c = h < L1 ? h : h + kTX;
             szh[k] = G.field(a.o_szy_in, i0 - L1 + c, j0 - LS + r);
         }
 #pragma unroll
         for (int k = 0; k < NM1; ++k) {
-            const int r = w + 4 * k, z = j0 - L1 + r;
+            const int r = w + PW * k, z = j0 - L1 + r;
             mv[k] = G.field(a.o_v_in, x, z);
             mb[k] = G.coef(a.o_bv, x, z);
             mu[k] = (HAS_U && r >= L1 && r < L1 + TZ) ? G.field(a.o_u_in, x, z) : 0.f;
@@ -492,7 +493,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
         }
 #pragma unroll
         for (int k = 0; k < NH1; ++k) {
-            const int idx = min(tid + 256 * k, H1 * HC1 - 1);
+            const int idx = min(tid + PT * k, H1 * HC1 - 1);
             const int r = idx / HC1, h = idx - r * HC1;
             const int c = h < L1 ? h : h + kTX;
             const int xx = i0 - L1 + c, zz = j0 - L1 + r;
@@ -503,23 +504,23 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
         }
 #pragma unroll
         for (int k = 0; k < NM1; ++k) {
-            const int r = w + 4 * k;
+            const int r = w + PW * k;
             if (r < H1) SX[r * WSX + lane + LS] = sxm[k];
         }
 #pragma unroll
         for (int k = 0; k < NHX; ++k) {
-            const int idx = tid + 256 * k;
+            const int idx = tid + PT * k;
             const int r = idx / HCS, h = idx - r * HCS;
             SX[idx < H1 * HCS ? r * WSX + (h < LS ? h : h + kTX) : H1 * WSX] = sxh[k];
         }
 #pragma unroll
         for (int k = 0; k < NMZ; ++k) {
-            const int r = w + 4 * k;
+            const int r = w + PW * k;
             if (r < HSZ) SZ[r * W1 + lane + L1] = szm[k];
         }
 #pragma unroll
         for (int k = 0; k < NHZ; ++k) {
-            const int idx = tid + 256 * k;
+            const int idx = tid + PT * k;
             const int r = idx / HC1, h = idx - r * HC1;
             SZ[idx < HSZ * HC1 ? r * W1 + (h < L1 ? h : h + kTX) : HSZ * W1] = szh[k];
         }
@@ -559,21 +560,21 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     };
     // dt*mu of the sigma updates, prefetched: main columns over the sigma_zy rows (which contain
     // the sigma_xy rows and the tile), and the two packed halo lists
-    constexpr int NMD = (HS1Z + 3) / 4;
-    constexpr int NDX = (HS1X * HCX1 + 255) / 256, NDZ = (HS1Z * HCZ1 + 255) / 256;
+    constexpr int NMD = (HS1Z + PW - 1) / PW;
+    constexpr int NDX = (HS1X * HCX1 + PT - 1) / PT, NDZ = (HS1Z * HCZ1 + PT - 1) / PT;
     float dmm[NMD], dmx[NDX], dmz[NDZ];
 #pragma unroll
-    for (int k = 0; k < NMD; ++k) dmm[k] = G.coef(a.o_dtmu, x, j0 - (2 * R - 1) + w + 4 * k);
+    for (int k = 0; k < NMD; ++k) dmm[k] = G.coef(a.o_dtmu, x, j0 - (2 * R - 1) + w + PW * k);
 #pragma unroll
     for (int k = 0; k < NDX; ++k) {
-        const int idx = min(tid + 256 * k, HS1X * HCX1 - 1);
+        const int idx = min(tid + PT * k, HS1X * HCX1 - 1);
         const int r = idx / HCX1, h = idx - r * HCX1;
         const int xx = h < 2 * R - 1 ? i0 - (2 * R - 1) + h : i0 + kTX + (h - (2 * R - 1));
         dmx[k] = G.coef(a.o_dtmu, xx, j0 - (R - 1) + r);
     }
 #pragma unroll
     for (int k = 0; k < NDZ; ++k) {
-        const int idx = min(tid + 256 * k, HS1Z * HCZ1 - 1);
+        const int idx = min(tid + PT * k, HS1Z * HCZ1 - 1);
         const int r = idx / HCZ1, h = idx - r * HCZ1;
         const int xx = h < R - 1 ? i0 - (R - 1) + h : i0 + kTX + (h - (R - 1));
         dmz[k] = G.coef(a.o_dtmu, xx, j0 - (2 * R - 1) + r);
@@ -584,7 +585,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     float un1[NM1];  // displacement after step n at the tile's own rows
 #pragma unroll
     for (int k = 0; k < NM1; ++k) {
-        const int r = w + 4 * k;
+        const int r = w + PW * k;
         un1[k] = 0.f;
         if (r >= H1) break;
         const int z = j0 - L1 + r, c = lane + L1;
@@ -608,13 +609,13 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     }
 #pragma unroll
     for (int k = 0; k < NH1; ++k) {
-        const int idx = min(tid + 256 * k, H1 * HC1 - 1);
+        const int idx = min(tid + PT * k, H1 * HC1 - 1);
         const int r = idx / HC1, h = idx - r * HC1;
         const int c = h < L1 ? h : h + kTX;
         const int xx = i0 - L1 + c, zz = j0 - L1 + r;
         const float ds = divs(xx, zz, r, c, frame(xx, R, nx - R - 1), frame(zz, R, nz - R - 1), a.ps_amp, a.ps_const) +
                          (syn1 ? hsy[k] : 0.f);
-        VV[tid + 256 * k < H1 * HC1 ? r * W1 + c : H1 * W1] = (hv[k] + hb[k] * ds) * ht[k];
+        VV[tid + PT * k < H1 * HC1 ? r * W1 + c : H1 * W1] = (hv[k] + hb[k] * ds) * ht[k];
     }
     __syncthreads();
 
@@ -636,7 +637,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     };
 #pragma unroll
     for (int k = 0; k < NMD; ++k) {
-        const int q = w + 4 * k;
+        const int q = w + PW * k;
         if (q >= HS1Z) break;
         const int z = j0 - (2 * R - 1) + q;
         const int rv = z - (j0 - L1), c = lane + L1;
@@ -645,30 +646,30 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     }
 #pragma unroll
     for (int k = 0; k < NDX; ++k) {
-        const int idx = min(tid + 256 * k, HS1X * HCX1 - 1);
+        const int idx = min(tid + PT * k, HS1X * HCX1 - 1);
         const int r = idx / HCX1, h = idx - r * HCX1;
         const int xx = h < 2 * R - 1 ? i0 - (2 * R - 1) + h : i0 + kTX + (h - (2 * R - 1));
         const int zz = j0 - (R - 1) + r;
         const int rv = zz - (j0 - L1), cv = xx - (i0 - L1);
         const float upd = dmx[k] * (dxv(rv, cv) * frame(xx, R - 1, nx - R - 1));
-        if (tid + 256 * k < HS1X * HCX1) SX[rv * WSX + cv + R] += upd;
+        if (tid + PT * k < HS1X * HCX1) SX[rv * WSX + cv + R] += upd;
     }
 #pragma unroll
     for (int k = 0; k < NDZ; ++k) {
-        const int idx = min(tid + 256 * k, HS1Z * HCZ1 - 1);
+        const int idx = min(tid + PT * k, HS1Z * HCZ1 - 1);
         const int r = idx / HCZ1, h = idx - r * HCZ1;
         const int xx = h < R - 1 ? i0 - (R - 1) + h : i0 + kTX + (h - (R - 1));
         const int zz = j0 - (2 * R - 1) + r;
         const int rv = zz - (j0 - L1), cv = xx - (i0 - L1);
         const float upd = dmz[k] * (dzv(rv, cv) * frame(zz, R - 1, nz - R - 1));
-        if (tid + 256 * k < HS1Z * HCZ1) SZ[(rv + R) * W1 + cv] += upd;
+        if (tid + PT * k < HS1Z * HCZ1) SZ[(rv + R) * W1 + cv] += upd;
     }
     __syncthreads();
 
     // ---------------- phase D: step-(n+1) velocity on V2, in place
 #pragma unroll
     for (int k = 0; k < NM1; ++k) {
-        const int r = w + 4 * k;
+        const int r = w + PW * k;
         if (r >= H1) break;
         if (r < V2R0 || r > V2R1) continue;
         const int z = j0 - L1 + r, c = lane + L1;
@@ -694,10 +695,10 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     }
 #pragma unroll
     for (int k = 0; k < NH1; ++k) {
-        const int idx = min(tid + 256 * k, H1 * HC1 - 1);
+        const int idx = min(tid + PT * k, H1 * HC1 - 1);
         const int r = idx / HC1, h = idx - r * HC1;
         const int c = h < L1 ? h : h + kTX;
-        const bool in2 = tid + 256 * k < H1 * HC1 && r >= V2R0 && r <= V2R1 && c >= V2C0 && c <= V2C1;
+        const bool in2 = tid + PT * k < H1 * HC1 && r >= V2R0 && r <= V2R1 && c >= V2C0 && c <= V2C1;
         const int xx = i0 - L1 + c, zz = j0 - L1 + r;
         const float ds = divs(xx, zz, r, c, frame(xx, R, nx - R - 1), frame(zz, R, nz - R - 1), pa.ps_amp2,
                               pa.ps_const2) + (syn1 ? 0.f : hsy[k]);
@@ -709,7 +710,7 @@ __global__ __launch_bounds__(256) void k_pair(PairArgs pa)
     // ---------------- phase E: step-(n+1) sigma on the tile
 #pragma unroll
     for (int k = 0; k < NMD; ++k) {
-        const int q = w + 4 * k;
+        const int q = w + PW * k;
         if (q >= HS1Z) break;
         const int z = j0 - (2 * R - 1) + q;
         if (z < j0 || z >= j0 + TZ) continue;
@@ -809,7 +810,7 @@ __global__ __launch_bounds__(256) void k_born_green(StepArgs a, float* sxy, floa
 
 void launch_pair(int R, bool with_u, const PairArgs& p, hipStream_t s)
 {
-    const dim3 g((unsigned)(kXCD * p.a.tiles_per_xcd)), b(256);
+    const dim3 g((unsigned)(kXCD * p.a.tiles_per_xcd)), b(kPairThreads);
 #define FD2D_PAIR(RR)                                                                            \
     case RR:                                                                                     \
         if (with_u) hipLaunchKernelGGL((k_pair<RR, 1>), g, b, 0, s, p);                          \
