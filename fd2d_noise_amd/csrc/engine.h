@@ -27,6 +27,7 @@ constexpr int kGhostX = 32; // ghost columns: node x = 0 starts a 128-byte line
 constexpr int kTX = 64;     // tile width: one wavefront per row
 constexpr int kXCD = 8;     // MI355X accelerator complex dies (workgroups are dealt round-robin)
 constexpr int kTileZ = 32;  // rows of a step tile
+constexpr int kTileThreads = 512;  // threads of a step workgroup (8 wavefronts, rows w, w+8, ...)
 constexpr int kPairThreads = 512;  // threads of a two-step workgroup (8 wavefronts, rows w, w+8, ...)
 constexpr int kMaxDim = 32000;  // node keys pack (x+8, z+8) into 16 bits each
 

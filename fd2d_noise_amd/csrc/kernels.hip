@@ -100,6 +100,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
 {
     constexpr bool HAS_U = MODE >= 1;
     constexpr int TZ = kTileZ;
+    constexpr int TT = kTileThreads, TW = TT / 64;  // threads and wavefronts of the workgroup
     constexpr int WV = kTX + 2 * R - 1;     // velocity region: columns i0-(R-1) .. i0+63+R
     constexpr int HV = TZ + 2 * R - 1;      //                  rows    j0-(R-1) .. j0+TZ-1+R
     constexpr int WSX = WV + 2 * R - 1;     // sigma_xy region: columns i0-(2R-1) .. i0+63+2R-1
@@ -123,15 +124,15 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     const bool kp = HAS_U && st.kfwd != nullptr;  // this step takes a structure-kernel product
 
     // ---------------- phase 1
-    constexpr int NRV = (HV + 3) / 4, NRZ = (HSZ + 3) / 4;
-    constexpr int NHS = (HV * HCS + 255) / 256, NHZ = (HSZ * HCV + 255) / 256, NHV = (HV * HCV + 255) / 256;
+    constexpr int NRV = (HV + TW - 1) / TW, NRZ = (HSZ + TW - 1) / TW;
+    constexpr int NHS = (HV * HCS + TT - 1) / TT, NHZ = (HSZ * HCV + TT - 1) / TT, NHV = (HV * HCV + TT - 1) / TT;
     float msx[NRV], msz[NRZ];
     float mv[NRV], mb[NRV], mu[NRV], msy[NRV];
     float hsx[NHS], hsz[NHZ];
     float hv[NHV], hb[NHV], hsy[NHV], ht[NHV], hu[NHV];
 #pragma unroll
     for (int k = 0; k < NRV; ++k) {
-        const int r = w + 4 * k, z = j0 - (R - 1) + r;
+        const int r = w + TW * k, z = j0 - (R - 1) + r;
         msx[k] = G.field(a.o_sxy_in, x, z);
         mv[k] = G.field(a.o_v_in, x, z);
         mb[k] = G.coef(a.o_bv, x, z);
@@ -140,27 +141,27 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     }
 #pragma unroll
     for (int k = 0; k < NRZ; ++k) {
-        const int r = w + 4 * k, z = j0 - (2 * R - 1) + r;
+        const int r = w + TW * k, z = j0 - (2 * R - 1) + r;
         msz[k] = G.field(a.o_szy_in, x, z);
     }
     // packed halo lists: idle lanes of the last round repeat the last cell and store to the sink
 #pragma unroll
     for (int k = 0; k < NHS; ++k) {
-        const int idx = min(tid + 256 * k, HV * HCS - 1);
+        const int idx = min(tid + TT * k, HV * HCS - 1);
         const int r = idx / HCS, h = idx - r * HCS;
         const int c = h < 2 * R - 1 ? h : h + kTX;            // sigma_xy region column
         hsx[k] = G.field(a.o_sxy_in, i0 - (2 * R - 1) + c, j0 - (R - 1) + r);
     }
 #pragma unroll
     for (int k = 0; k < NHZ; ++k) {
-        const int idx = min(tid + 256 * k, HSZ * HCV - 1);
+        const int idx = min(tid + TT * k, HSZ * HCV - 1);
         const int r = idx / HCV, h = idx - r * HCV;
         const int c = h < R - 1 ? h : h + kTX;                // velocity-region column
         hsz[k] = G.field(a.o_szy_in, i0 - (R - 1) + c, j0 - (2 * R - 1) + r);
     }
 #pragma unroll
     for (int k = 0; k < NHV; ++k) {
-        const int idx = min(tid + 256 * k, HV * HCV - 1);
+        const int idx = min(tid + TT * k, HV * HCV - 1);
         const int r = idx / HCV, h = idx - r * HCV;
         const int c = h < R - 1 ? h : h + kTX;
         const int xx = i0 - (R - 1) + c, zz = j0 - (R - 1) + r;
@@ -173,23 +174,23 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     // LDS stores
 #pragma unroll
     for (int k = 0; k < NRV; ++k) {
-        const int r = w + 4 * k;
+        const int r = w + TW * k;
         if (r < HV) sx[r * WSX + lane + 2 * R - 1] = msx[k];
     }
 #pragma unroll
     for (int k = 0; k < NRZ; ++k) {
-        const int r = w + 4 * k;
+        const int r = w + TW * k;
         if (r < HSZ) sz[r * WV + lane + R - 1] = msz[k];
     }
 #pragma unroll
     for (int k = 0; k < NHS; ++k) {
-        const int idx = tid + 256 * k;
+        const int idx = tid + TT * k;
         const int r = idx / HCS, h = idx - r * HCS;
         sx[idx < HV * HCS ? r * WSX + (h < 2 * R - 1 ? h : h + kTX) : HV * WSX] = hsx[k];
     }
 #pragma unroll
     for (int k = 0; k < NHZ; ++k) {
-        const int idx = tid + 256 * k;
+        const int idx = tid + TT * k;
         const int r = idx / HCV, h = idx - r * HCV;
         sz[idx < HSZ * HCV ? r * WV + (h < R - 1 ? h : h + kTX) : HSZ * WV] = hsz[k];
     }
@@ -224,7 +225,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     const float axl = gld(a.axp, x + kGhostX);
 #pragma unroll
     for (int k = 0; k < NRV; ++k) {
-        const int r = w + 4 * k;
+        const int r = w + TW * k;
         if (r >= HV) break;
         const int z = j0 - (R - 1) + r, c = lane + R - 1;
         const float ds = divs(x, z, r, c, fxl, frame(z, R, nz - R - 1)) + msy[k];
@@ -254,21 +255,21 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     }
 #pragma unroll
     for (int k = 0; k < NHV; ++k) {
-        const int idx = min(tid + 256 * k, HV * HCV - 1);
+        const int idx = min(tid + TT * k, HV * HCV - 1);
         const int r = idx / HCV, h = idx - r * HCV;
         const int c = h < R - 1 ? h : h + kTX;
         const int xx = i0 - (R - 1) + c, zz = j0 - (R - 1) + r;
         const float ds = divs(xx, zz, r, c, frame(xx, R, nx - R - 1), frame(zz, R, nz - R - 1)) + hsy[k];
         const float vn = (hv[k] + hb[k] * ds) * ht[k];
-        vv[tid + 256 * k < HV * HCV ? r * WV + c : HV * WV] = vn;
+        vv[tid + TT * k < HV * HCV ? r * WV + c : HV * WV] = vn;
     }
     __syncthreads();
 
     // ---------------- phase 3
-    constexpr int NT = TZ / 4;
+    constexpr int NT = TZ / TW;
     float rm[NT];
 #pragma unroll
-    for (int k = 0; k < NT; ++k) rm[k] = G.coef(a.o_dtmu, x, j0 + w + 4 * k);
+    for (int k = 0; k < NT; ++k) rm[k] = G.coef(a.o_dtmu, x, j0 + w + TW * k);
     const float fxs = frame(x, R - 1, nx - R - 1);  // dx_v frame
     const int c = lane + R - 1;
     const bool born = st.born_dmu != nullptr;
@@ -277,7 +278,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     const auto r_bd = rsrc(born ? (const void*)st.born_dmu : (const void*)a.fbase, born ? cb : 0u);
 #pragma unroll
     for (int k = 0; k < NT; ++k) {
-        const int lz = w + 4 * k, z = j0 + lz;
+        const int lz = w + TW * k, z = j0 + lz;
         const int r = lz + R - 1;
         const float fzs = frame(z, R - 1, nz - R - 1);  // dz_v frame
         const float dm = rm[k];
@@ -319,17 +320,17 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         const auto r_fw = rsrc(st.kfwd, cb);
         float fm[NRV], fh[NHV];
 #pragma unroll
-        for (int k = 0; k < NRV; ++k) fm[k] = bload(r_fw, G.cofs(x, j0 - (R - 1) + w + 4 * k));
+        for (int k = 0; k < NRV; ++k) fm[k] = bload(r_fw, G.cofs(x, j0 - (R - 1) + w + TW * k));
 #pragma unroll
         for (int k = 0; k < NHV; ++k) {
-            const int idx = min(tid + 256 * k, HV * HCV - 1);
+            const int idx = min(tid + TT * k, HV * HCV - 1);
             const int r = idx / HCV, h = idx - r * HCV;
             fh[k] = bload(r_fw, G.cofs(i0 - (R - 1) + (h < R - 1 ? h : h + kTX), j0 - (R - 1) + r));
         }
         __syncthreads();  // phase 3 has read vv, sx and sz
 #pragma unroll
         for (int k = 0; k < NRV; ++k) {
-            const int r = w + 4 * k;
+            const int r = w + TW * k;
             if (r >= HV) break;
             float* cell = vv + r * WV + lane + R - 1;
             *cell = mu[k] + *cell * a.dt;
@@ -337,9 +338,9 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         }
 #pragma unroll
         for (int k = 0; k < NHV; ++k) {
-            const int idx = min(tid + 256 * k, HV * HCV - 1);
+            const int idx = min(tid + TT * k, HV * HCV - 1);
             const int r = idx / HCV, h = idx - r * HCV;
-            const int o = tid + 256 * k < HV * HCV ? r * WV + (h < R - 1 ? h : h + kTX) : HV * WV;
+            const int o = tid + TT * k < HV * HCV ? r * WV + (h < R - 1 ? h : h + kTX) : HV * WV;
             vv[o] = hu[k] + vv[o] * a.dt;
             sx[o] = fh[k];
         }
@@ -347,7 +348,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         const bool xin = x >= R - 1 && x <= nx - R - 1;
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
-            const int lz = w + 4 * k, z = j0 + lz;
+            const int lz = w + TW * k, z = j0 + lz;
             const unsigned co = G.cofs(x, z);
             const bool zin = z >= R - 1 && z <= nz - R - 1;
             if (co == kOutside || (!xin && !zin)) continue;
@@ -380,7 +381,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
 }
 
 template <int R, int MODE>
-__global__ __launch_bounds__(256) void k_tile(StepArgs a)
+__global__ __launch_bounds__(kTileThreads) void k_tile(StepArgs a)
 {
     const int g = blockIdx.x;
     const JobStep st{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, a.born_dmu, a.born_in0, a.born_in1, a.kfwd,
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(256) void k_tile(StepArgs a)
 }
 
 template <int R, int MODE>
-__global__ __launch_bounds__(256) void k_tile_group(GroupArgs G)
+__global__ __launch_bounds__(kTileThreads) void k_tile_group(GroupArgs G)
 {
     const int g = blockIdx.x;
     const int T = (g % kXCD) * G.tiles_per_xcd + g / kXCD;
@@ -827,7 +828,7 @@ void launch_pair(int R, bool with_u, const PairArgs& p, hipStream_t s)
 
 void launch_group(int R, bool with_u, const GroupArgs& ga, hipStream_t s)
 {
-    const dim3 g((unsigned)(kXCD * ga.tiles_per_xcd)), b(256);
+    const dim3 g((unsigned)(kXCD * ga.tiles_per_xcd)), b(kTileThreads);
 #define FD2D_GROUP(RR)                                                                           \
     case RR:                                                                                     \
         if (with_u) hipLaunchKernelGGL((k_tile_group<RR, 1>), g, b, 0, s, ga);                   \
@@ -844,7 +845,7 @@ void launch_group(int R, bool with_u, const GroupArgs& ga, hipStream_t s)
 
 void launch_step(int R, bool with_u, const StepArgs& a, hipStream_t s)
 {
-    const dim3 g((unsigned)(kXCD * a.tiles_per_xcd)), b(256);
+    const dim3 g((unsigned)(kXCD * a.tiles_per_xcd)), b(kTileThreads);
 #define FD2D_TILE(RR)                                                                            \
     case RR:                                                                                     \
         if (with_u) hipLaunchKernelGGL((k_tile<RR, 1>), g, b, 0, s, a);                          \
