@@ -34,9 +34,11 @@ def _cases(torch):
     c1 = Config(order=4, nx=70, nz=52, nt=160, fw_nth=2, freq_samp=3)
     c2 = Config(order=4, nx=50, nz=90, nt=120, fw_nth=3)
     c3 = Config(order=8, nx=64, nz=60, nt=140, fw_nth=4, Lx=63e3, Lz=59e3)
+    # 25 x 35 = 875 tiles: large enough to keep its own launches inside the engine
+    c4 = Config(order=4, nx=1600, nz=1100, nt=30, fw_nth=3, Lx=1.6e6, Lz=1.1e6, dt=0.1)
     rng = np.random.default_rng(901)
     cases = []
-    for c in (c1, c2, c3):
+    for c in (c1, c2, c3, c4):
         mu, rho = hetero_model(rng, c)
         mu_d, rho_d = D.to_device(mu), D.to_device(rho)
         srcs = [interior_points(rng, c, 1, 0.4) for _ in range(2)]
@@ -59,7 +61,9 @@ def _cases(torch):
             couts = [((nr, c.n_corr), f64), ((c.nx, c.nz, c.n_fw_corr()), f32)]
             cases.append(("correlation", c, (mu_d, rho_d, G, spectrum, sd, rec_d, 1, None, None), couts))
             a = D.to_host(corr)
-            stf = D.to_device(np.ascontiguousarray(a[:, ::-1] / np.abs(a).max()).astype(np.complex128))
+            scale = np.abs(a).max()  # zero on the large grid: no wave reaches its receivers in 30 steps
+            adj = a[:, ::-1] / scale if scale > 0 else rng.standard_normal(a.shape)
+            stf = D.to_device(np.ascontiguousarray(adj).astype(np.complex128))
             nb1 = max(c.n_basis_fct, 1) + 1
             aouts = [((c.nx, c.nz, nb1), f64), ((c.nx, c.nz, c.n_sample), cd), ((c.nx, c.nz, c.n_fw_corr()), f32),
                      ((nr, c.n_corr), f64)]
