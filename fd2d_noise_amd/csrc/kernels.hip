@@ -171,6 +171,20 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         ht[k] = gld(a.axp, xx + kGhostX) * gld(a.azp, zz + kGhost);
         hu[k] = (HAS_U && kp) ? G.field(a.o_u_in, xx, zz) : 0.f;
     }
+    // the forward snapshot of a product step, over the velocity region (used in phase 4; issued
+    // here so its latency hides behind phases 1-3)
+    float fm[NRV], fh[NHV];
+    if (kp) {
+        const auto r_fw = rsrc(st.kfwd, cb);
+#pragma unroll
+        for (int k = 0; k < NRV; ++k) fm[k] = bload(r_fw, G.cofs(x, j0 - (R - 1) + w + TW * k));
+#pragma unroll
+        for (int k = 0; k < NHV; ++k) {
+            const int idx = min(tid + TT * k, HV * HCV - 1);
+            const int r = idx / HCV, h = idx - r * HCV;
+            fh[k] = bload(r_fw, G.cofs(i0 - (R - 1) + (h < R - 1 ? h : h + kTX), j0 - (R - 1) + r));
+        }
+    }
     // LDS stores
 #pragma unroll
     for (int k = 0; k < NRV; ++k) {
@@ -317,15 +331,13 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     // region, which covers the (R-1, R) halo of the stencils, so no other tile's output is read;
     // the sums are those of k_kmu, term for term.
     if (kp) {
-        const auto r_fw = rsrc(st.kfwd, cb);
-        float fm[NRV], fh[NHV];
+        // the K_mu values of the tile's own nodes, loaded before the barrier
+        const bool xin = x >= R - 1 && x <= nx - R - 1;
+        double kold[NT];
 #pragma unroll
-        for (int k = 0; k < NRV; ++k) fm[k] = bload(r_fw, G.cofs(x, j0 - (R - 1) + w + TW * k));
-#pragma unroll
-        for (int k = 0; k < NHV; ++k) {
-            const int idx = min(tid + TT * k, HV * HCV - 1);
-            const int r = idx / HCV, h = idx - r * HCV;
-            fh[k] = bload(r_fw, G.cofs(i0 - (R - 1) + (h < R - 1 ? h : h + kTX), j0 - (R - 1) + r));
+        for (int k = 0; k < NT; ++k) {
+            const unsigned co = G.cofs(x, j0 + w + TW * k);
+            kold[k] = co != kOutside ? gld(a.kacc, (int)(co / 4u)) : 0.0;
         }
         __syncthreads();  // phase 3 has read vv, sx and sz
 #pragma unroll
@@ -345,7 +357,6 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
             sx[o] = fh[k];
         }
         __syncthreads();
-        const bool xin = x >= R - 1 && x <= nx - R - 1;
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
             const int lz = w + TW * k, z = j0 + lz;
@@ -374,8 +385,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
                 }
                 acc += (double)su * (double)sf;
             }
-            const int i = (int)(co / 4u);
-            gst(a.kacc, i, gld(a.kacc, i) - acc * a.kscale);
+            gst(a.kacc, (int)(co / 4u), kold[k] - acc * a.kscale);
         }
     }
 }
