@@ -62,8 +62,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-every", type=int, default=7,
                     help="time every n-th launch (7: coprime with fw_nth, so kernel steps enter in proportion)")
-    ap.add_argument("--steps-per-launch", type=int, choices=[1, 2], default=1,
-                    help="time steps per launch of the time loop (2: temporal blocking)")
+    ap.add_argument("--steps-per-launch", type=int, choices=[0, 1, 2], default=0,
+                    help="time steps per launch of the time loop (2: temporal blocking; 0: tuned per grid "
+                         "shape by the first solve of that shape)")
     ap.add_argument("--only", choices=["green", "correlation", "adjoint1", "adjoint0", "copy"], default=None,
                     help="profiling aid: time one solver of the pipeline alone (or a 1 GiB device copy "
                          "used to calibrate HBM byte counters); the default is the whole pipeline")

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <algorithm>
+#include <array>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -721,7 +722,7 @@ hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 // ---------------------------------------------------------------------------------------------
 std::mutex g_timing_mu;
 int g_timing_every = 0;
-std::atomic<int> g_steps_per_launch{1};  // fd2d_set_steps_per_launch
+std::atomic<int> g_steps_per_launch{0};  // fd2d_set_steps_per_launch: 0 = tuned per shape
 fd2d_timing g_timing{};
 
 struct StepTimer {
@@ -842,11 +843,60 @@ struct Job {
     }
 };
 
+// Launch-shape autotuner. With fd2d_set_steps_per_launch(0) (the default) the first solve of each
+// shape (grid, FD order, field set) times its own early launches both ways -- kTuneLaunches single
+// steps, then kTuneLaunches two-step launches -- and the faster shape is kept for the rest of that
+// solve and every later solve of the shape. Both shapes give the same results bit for bit, so the
+// choice changes only the time. Solves with Born terms, or too short to time, use single steps.
+constexpr int kTuneLaunches = 24;
+constexpr int kTuneWarm = 12;
+std::mutex g_tune_mu;
+std::map<std::array<int, 4>, int> g_tuned;  // (nx, nz, R, with_u) -> steps per launch
+
+std::array<int, 4> shape_key(const Job& j) { return {j.g.nx, j.g.nz, j.g.R, j.S->with_u ? 1 : 0}; }
+
+int tuned_steps(const Job& j)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(shape_key(j));
+    return it == g_tuned.end() ? 0 : it->second;
+}
+
+void record_tuned(const Job& j, int steps)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[shape_key(j)] = steps;
+}
+
+// steps per launch of solve j: 1 or 2, or 0 when this solve should time both
+int launch_steps(const Job& j)
+{
+    const int mode = g_steps_per_launch.load();
+    if (j.born) return 1;
+    if (mode == 1 || mode == 2) return mode;
+    const int t = tuned_steps(j);
+    if (t) return t;
+    return j.nsteps >= 8 * kTuneLaunches + kTuneWarm ? 0 : 1;
+}
+
 void run_job(Job& j)
 {
     StepTimer timer(j.solver, j.g.P, j.st);
-    const bool pairs = g_steps_per_launch.load() == 2 && !j.born;
+    int steps = launch_steps(j);
+    // tuning: phase 0 = single steps, phase 1 = pairs, each over kTuneLaunches launches, after
+    // kTuneWarm launches that hold the solve's one-off work (the first synthesised source block)
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int phase = steps == 0 ? -2 : -1, launches = 0, advanced[2] = {0, 0};
+    auto mark = [&](int q) {
+        HIPCHK(hipEventCreate(&ev[q]));
+        HIPCHK(hipEventRecord(ev[q], j.st));
+    };
     for (int n = 1; n <= j.nsteps;) {
+        if (phase == -2 && n > kTuneWarm) {
+            mark(0);
+            phase = 0;
+        }
+        const bool pairs = phase == 1 || (phase == -1 && steps == 2);
         const int k = (pairs && n + 1 <= j.nsteps && j.pairable(n)) ? 2 : 1;
         j.S->clear_step();
         for (int q = 0; q < k; ++q) j.plan(n + q, q);
@@ -857,7 +907,29 @@ void run_job(Job& j)
         timer.end(n, k);
         j.after(n, k);
         n += k;
+        if (phase >= 0) {
+            advanced[phase] += k;
+            if (++launches == kTuneLaunches) {
+                mark(2 * phase + 1);
+                launches = 0;
+                if (phase == 0) {
+                    mark(2);
+                    phase = 1;
+                } else {
+                    HIPCHK(hipEventSynchronize(ev[3]));
+                    float t0 = 0.f, t1 = 0.f;
+                    HIPCHK(hipEventElapsedTime(&t0, ev[0], ev[1]));
+                    HIPCHK(hipEventElapsedTime(&t1, ev[2], ev[3]));
+                    // pairs only when clearly faster per time step
+                    steps = (t1 / advanced[1] < 0.97f * (t0 / advanced[0])) ? 2 : 1;
+                    record_tuned(j, steps);
+                    phase = -1;
+                }
+            }
+        }
     }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
     j.finish();
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(j.st));
@@ -1422,14 +1494,23 @@ struct Engine {
         }
         dirty = false;
     }
+    // A solve with its own launch: one step, or two when its shape's launch shape (tuned by an
+    // earlier solve, or set by fd2d_set_steps_per_launch) is two steps and the pair is allowed.
     void step_single(Job& j)
     {
+        const int mode = g_steps_per_launch.load();
+        const int shape = (mode == 1 || mode == 2) ? mode : tuned_steps(j);
+        const int k = (shape == 2 && !j.born && j.next + 1 <= j.nsteps && j.pairable(j.next)) ? 2 : 1;
         j.S->clear_step();
-        j.plan(j.next, 0);
-        j.fuse_products();
-        j.S->step();
-        j.after(j.next, 1);
-        ++j.next;
+        for (int q = 0; q < k; ++q) j.plan(j.next + q, q);
+        if (k == 1) {
+            j.fuse_products();
+            j.S->step();
+        } else {
+            j.S->step_pair();
+        }
+        j.after(j.next, k);
+        j.next += k;
     }
     void step_group(const Group& gr)
     {
@@ -1756,8 +1837,8 @@ const char* fd2d_last_error(void) { return g_err.c_str(); }
 
 int fd2d_set_steps_per_launch(int32_t steps)
 {
-    if (steps != 1 && steps != 2) {
-        g_err = "steps per launch must be 1 or 2";
+    if (steps != 0 && steps != 1 && steps != 2) {
+        g_err = "steps per launch must be 0 (tuned per shape), 1 or 2";
         return FD2D_EINVAL;
     }
     g_steps_per_launch.store(steps);

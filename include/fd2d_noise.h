@@ -357,11 +357,12 @@ int fd2d_solve_device_bytes(const fd2d_solve* solve, int64_t* bytes);
 int fd2d_release_cached_memory(void);
 
 /*
- * Steps per launch of the time loop (tuning; it has no counterpart in the reference). 1 (the
- * default) launches every step on its own; 2 advances two consecutive time steps per launch
- * (temporal blocking) wherever the two steps need at most one DFT sample, at most one synthesised
- * source slice, no Born term and no structure-kernel product after the first of them. Results are
- * the same either way.
+ * Steps per launch of the time loop (tuning; it has no counterpart in the reference). 1 launches
+ * every step on its own; 2 advances two consecutive time steps per launch (temporal blocking)
+ * wherever the two steps need at most one DFT sample, at most one synthesised source slice, no Born
+ * term and no structure-kernel product after the first of them; 0 (the default) lets the first
+ * solve of each shape (grid, FD order, field set) time both on its own first launches and keeps the
+ * faster for that shape. Results are the same in every case, bit for bit.
  * Returns FD2D_EINVAL for other values. Applies to solves started afterwards.
  */
 int fd2d_set_steps_per_launch(int32_t steps);

@@ -25,7 +25,7 @@ def gpu():
         pytest.skip("no GPU")
     _lib.load()
     yield
-    _lib.set_steps_per_launch(1)
+    _lib.set_steps_per_launch(0)
 
 
 def both(fn):
@@ -33,7 +33,7 @@ def both(fn):
     for k in (1, 2):
         _lib.set_steps_per_launch(k)
         out.append(fn())
-    _lib.set_steps_per_launch(1)
+    _lib.set_steps_per_launch(0)
     return out
 
 
@@ -91,3 +91,24 @@ def test_adjoint_pair(orc, mode, fw_nth):
     else:
         one, two = both(lambda: fd.run_noise_adjoint(mu, rho, C, stf, rec, rec, spectrum, None, G, mode, config=c))
     _cmp(f"adjoint mode {mode} f {fw_nth}", one, two)
+
+
+@pytest.mark.parametrize("order", [4, 8])
+def test_tuned_launch_shape_matches_single_steps(orc, order):
+    """Mode 0: the first solve of a shape times single and two-step launches on its own early steps
+    and keeps the faster; every output equals one step per launch."""
+    c = Config(order=order, nx=96, nz=88, nt=150, fw_nth=2, Lx=95e3, Lz=87e3)
+    rng = np.random.default_rng(730 + order)
+    mu, rho = hetero_model(rng, c)
+    src = interior_points(rng, c, 1, 0.4)
+    rec = interior_points(rng, c, 3)
+    G, _, _ = orc.run_forward1_green(mu, rho, src, rec, 0, config=c)
+    spectrum = rng.uniform(0.5, 1.5, (c.n_sample, 1))
+    sd = blob_distribution(rng, c)
+    outs = []
+    for k in (1, 0, 0):  # the second mode-0 run uses the shape tuned by the first
+        _lib.set_steps_per_launch(k)
+        outs.append(fd.run_forward2_correlation(mu, rho, G, spectrum, sd, rec, 1, config=c))
+    _lib.set_steps_per_launch(0)
+    _cmp(f"tuned order {order}", outs[0], outs[1])
+    _cmp(f"tuned again order {order}", outs[0], outs[2])
