@@ -816,12 +816,14 @@ struct Job {
     bool records = false;   // writes seismograms (receiver lists are used)
     virtual ~Job() = default;
     virtual void plan(int n, int sub) = 0;  // fields of step n as step `sub` of the launch
-    virtual bool pairable(int n) = 0;       // may steps n, n+1 share a two-step launch
+    virtual bool pairable(int n) const = 0;  // may steps n, n+1 share a two-step launch
     virtual void after(int n, int k) = 0;   // after the launch that advanced steps n .. n+k-1
     virtual void finish() = 0;              // after the last step
     // For a step launched on its own: move the structure-kernel product that plan() scheduled
     // after the launch into the step kernel itself.
     virtual void fuse_products() {}
+    // whether step n (not yet planned) carries a structure-kernel product
+    virtual bool product_at(int) const { return false; }
     // StepArgs with the fields that do not change from step to step, for ping-pong direction `parity`
     StepArgs static_args(int parity)
     {
@@ -1007,7 +1009,7 @@ struct GreenJob : Job {
         if (sampled(n)) *sl.dsamp = dft.next_slot();
         if (records) S->receivers(sub, sg + (size_t)nr * (n - 1));
     }
-    bool pairable(int n) override { return !(sampled(n) && sampled(n + 1)); }
+    bool pairable(int n) const override { return !(sampled(n) && sampled(n + 1)); }
     void after(int n, int k) override
     {
         for (int q = 0; q < k; ++q)
@@ -1114,7 +1116,7 @@ struct CorrelationJob : Job {
         }
         if (records) S->receivers(sub, sg + (size_t)nr * (n - 1));
     }
-    bool pairable(int n) override { return !(n % g.s == 0 && (n + 1) % g.s == 0); }
+    bool pairable(int n) const override { return !(n % g.s == 0 && (n + 1) % g.s == 0); }
     void after(int, int) override {}
     void finish() override {}
 };
@@ -1259,6 +1261,7 @@ struct AdjointJob : Job {
         }
     }
     bool kernel_step(int n) const { return Kd && n_ufwd && n_ufwd >= i_fw_in + 1 && n % g.f == 0; }  // :324
+    bool product_at(int n) const override { return kernel_step(n); }
     bool sampled(int n) const { return need_dft && n % g.s == 0; }
     bool synth(int n) const { return !(stf_point && !sid.empty()) && dist_src && n % g.s == 0; }
     void plan(int n, int sub) override
@@ -1288,7 +1291,7 @@ struct AdjointJob : Job {
     }
     // A pair needs the structure-kernel product, if any, after its second step (the displacement
     // of its first step is not kept), at most one DFT sample and at most one synthesised source.
-    bool pairable(int n) override
+    bool pairable(int n) const override
     {
         return !kernel_step(n) && !(sampled(n) && sampled(n + 1)) && !(synth(n) && synth(n + 1));
     }
@@ -1494,13 +1497,11 @@ struct Engine {
         }
         dirty = false;
     }
-    // A solve with its own launch: one step, or two when its shape's launch shape (tuned by an
-    // earlier solve, or set by fd2d_set_steps_per_launch) is two steps and the pair is allowed.
+    // A solve with its own launch: one step, or two when its launch shape (shape_of) is two steps
+    // and the pair is allowed.
     void step_single(Job& j)
     {
-        const int mode = g_steps_per_launch.load();
-        const int shape = (mode == 1 || mode == 2) ? mode : tuned_steps(j);
-        const int k = (shape == 2 && !j.born && j.next + 1 <= j.nsteps && j.pairable(j.next)) ? 2 : 1;
+        const int k = (shape_of(j) == 2 && !j.born && j.next + 1 <= j.nsteps && j.pairable(j.next)) ? 2 : 1;
         j.S->clear_step();
         for (int q = 0; q < k; ++q) j.plan(j.next + q, q);
         if (k == 1) {
@@ -1512,14 +1513,41 @@ struct Engine {
         j.after(j.next, k);
         j.next += k;
     }
+    // Steps per launch the engine uses for solve j: the tuned shape of its grid when an earlier solve
+    // has timed it (or the fd2d_set_steps_per_launch setting), else two steps at FD order 2 / 4 and
+    // one at 6 / 8 (measured: two-step launches are faster at order 4 on every grid tried, equal at
+    // order 8 on the large benchmark grid).
+    int shape_of(const Job& j) const
+    {
+        const int mode = g_steps_per_launch.load();
+        if (mode == 1 || mode == 2) return mode;
+        const int t = tuned_steps(j);
+        return t ? t : (j.g.R <= 2 ? 2 : 1);
+    }
+    // whether solve j takes steps next, next+1 in one grouped two-step launch: allowed by the pair
+    // rules, no Born terms, and no structure-kernel product at the second step (which would need
+    // its own launch after the pair)
+    bool pairs_now(const Job& j) const
+    {
+        return shape_of(j) == 2 && !j.born && j.next + 1 <= j.nsteps && j.pairable(j.next) &&
+               !j.product_at(j.next + 1);
+    }
     void step_group(const Group& gr)
+    {
+        std::vector<Entry*> one, two;
+        for (Entry* e : gr.members) (pairs_now(*e->job) ? two : one).push_back(e);
+        if (!one.empty()) launch_singles(gr, one);
+        for (size_t q = 0; q < two.size(); q += kPairGroupMax)
+            launch_pairs(gr, std::vector<Entry*>(two.begin() + q, two.begin() + std::min(two.size(), q + kPairGroupMax)));
+    }
+    void launch_singles(const Group& gr, const std::vector<Entry*>& members)
     {
         GroupArgs ga{};
         ga.jobs = slots_d.p;
-        ga.n_jobs = (int)gr.members.size();
+        ga.n_jobs = (int)members.size();
         int total = 0;
         for (int q = 0; q < ga.n_jobs; ++q) {
-            Entry& e = *gr.members[q];
+            Entry& e = *members[q];
             Job& j = *e.job;
             j.S->clear_step();
             j.plan(j.next, 0);
@@ -1535,10 +1563,40 @@ struct Engine {
         ga.tiles_per_xcd = (total + kXCD - 1) / kXCD;
         launch_group(gr.R, gr.with_u, ga, st);
         for (int q = 0; q < ga.n_jobs; ++q) {
-            Job& j = *gr.members[q]->job;
+            Job& j = *members[q]->job;
             j.S->cur ^= 1;
             j.after(j.next, 1);
             ++j.next;
+        }
+    }
+    void launch_pairs(const Group& gr, const std::vector<Entry*>& members)
+    {
+        PairGroupArgs ga{};
+        ga.jobs = slots_d.p;
+        ga.n_jobs = (int)members.size();
+        int total = 0;
+        for (int q = 0; q < ga.n_jobs; ++q) {
+            Entry& e = *members[q];
+            Job& j = *e.job;
+            j.S->clear_step();
+            j.plan(j.next, 0);
+            j.plan(j.next + 1, 1);
+            const StepArgs& a = j.S->a;
+            const PairArgs& p = j.S->pr;
+            ga.tile_start[q] = total;
+            total += j.S->n_tiles;
+            ga.step[q] = PairStep{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, p.ps_amp2, p.syn2, p.snap2, p.dsamp2,
+                                  p.seis_col2, a.ps_const, p.ps_const2, j.S->cur, e.slot};
+        }
+        ga.tile_start[ga.n_jobs] = total;
+        ga.total_tiles = total;
+        ga.tiles_per_xcd = (total + kXCD - 1) / kXCD;
+        launch_pair_group(gr.R, gr.with_u, ga, st);
+        for (int q = 0; q < ga.n_jobs; ++q) {
+            Job& j = *members[q]->job;
+            j.S->cur ^= 1;  // a pair swaps the state buffers once, as one step does
+            j.after(j.next, 2);
+            j.next += 2;
         }
     }
     // Drop the solves whose last work has completed on the device.

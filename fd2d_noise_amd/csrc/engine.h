@@ -157,6 +157,35 @@ struct GroupArgs {
 };
 static_assert(sizeof(GroupArgs) <= 4096, "grouped-launch arguments exceed the 4 KB kernel-argument limit");
 
+// The per-step fields of one solve in a grouped two-step launch: those of steps n and n+1.
+struct PairStep {
+    const float* ps_amp;
+    const float* syn;
+    float* snap;
+    float* dsamp;
+    double* seis_col;
+    const float* ps_amp2;
+    const float* syn2;
+    float* snap2;
+    float* dsamp2;
+    double* seis_col2;
+    float ps_const;
+    float ps_const2;
+    int parity;
+    int slot;
+};
+constexpr int kPairGroupMax = 36;  // solves per grouped two-step launch (arguments under 4 KB)
+struct PairGroupArgs {
+    const StepArgs* jobs;
+    int n_jobs;
+    int total_tiles;
+    int tiles_per_xcd;
+    int tile_start[kPairGroupMax + 1];
+    PairStep step[kPairGroupMax];
+};
+static_assert(sizeof(PairGroupArgs) <= 4096, "grouped-launch arguments exceed the 4 KB kernel-argument limit");
+void launch_pair_group(int R, bool with_u, const PairGroupArgs& g, hipStream_t s);
+
 // halo of the first step's velocity region of a pair launch (the point-source lists cover it)
 inline int pair_halo_lo(int R) { return 3 * R - 2; }
 inline int pair_halo_hi(int R) { return 3 * R - 1; }

@@ -431,7 +431,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_group(GroupArgs G)
 // Per-thread layout as in k_tile: lane l owns column i0 + l of every region's main block, wave w
 // the rows w, w+4, ...; halo columns are packed lists over the 256 threads.
 template <int R, int MODE>
-__global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
+__device__ __forceinline__ void pair_step(const StepArgs& a, const PairStep& ps, int t)
 {
     constexpr bool HAS_U = MODE >= 1;
     constexpr int TZ = kTileZ;
@@ -451,9 +451,6 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
     __shared__ float SZ[HSZ * W1 + 1];
     __shared__ float VV[H1 * W1 + 1];
 
-    const StepArgs& a = pa.a;
-    const int g = blockIdx.x;
-    const int t = (g % kXCD) * a.tiles_per_xcd + g / kXCD;
     if (t >= a.n_tiles) return;
     const int i0 = (t % a.tiles_x) * kTX, j0 = (t / a.tiles_x) * TZ;
     const int nx = a.nx, nz = a.nz;
@@ -462,9 +459,9 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
     const int x = i0 + lane;
     const Grid G{nx, nz, a.pitch, rsrc(a.fbase, a.fbytes), rsrc(a.cbase, a.cbytes)};
     const unsigned cb = a.compact_bytes;
-    // the synthesised source of the pair belongs to step n (a.syn) or to step n+1 (syn2)
-    const float* syn = a.syn ? a.syn : pa.syn2;
-    const bool syn1 = a.syn != nullptr, has_syn = syn != nullptr;
+    // the synthesised source of the pair belongs to step n (ps.syn) or to step n+1 (syn2)
+    const float* syn = ps.syn ? ps.syn : ps.syn2;
+    const bool syn1 = ps.syn != nullptr, has_syn = syn != nullptr;
     const auto r_syn = rsrc(has_syn ? (const void*)syn : (const void*)a.fbase, has_syn ? cb : 0u);
     auto frame = [](int v, int lo, int hi) { return (v >= lo && v <= hi) ? 1.f : 0.f; };
 
@@ -511,7 +508,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
             hv[k] = G.field(a.o_v_in, xx, zz);
             hb[k] = G.coef(a.o_bv, xx, zz);
             hsy[k] = has_syn ? bload(r_syn, G.cofs(xx, zz)) : 0.f;
-            ht[k] = a.axp[xx + kGhostX] * a.azp[zz + kGhost];
+            ht[k] = gld(a.axp, xx + kGhostX) * gld(a.azp, zz + kGhost);
         }
 #pragma unroll
         for (int k = 0; k < NM1; ++k) {
@@ -539,8 +536,8 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
     __syncthreads();
 
     // ---------------- phase B: step-n velocity on V1
-    const int e0 = a.ps_off ? a.ps_off[t] : 0, e1 = a.ps_off ? a.ps_off[t + 1] : 0;
-    const int f0 = a.rec_off ? a.rec_off[t] : 0, f1 = a.rec_off ? a.rec_off[t + 1] : 0;
+    const int e0 = a.ps_off ? gld(a.ps_off, t) : 0, e1 = a.ps_off ? gld(a.ps_off, t + 1) : 0;
+    const int f0 = a.rec_off ? gld(a.rec_off, t) : 0, f1 = a.rec_off ? gld(a.rec_off, t + 1) : 0;
     auto divs = [&](int xx, int zz, int r, int c, float fx, float fz, const float* amp, float cst) {
         const float* row = SX + r * WSX + c + R;
         float s = 0.f;
@@ -554,8 +551,8 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
         if (e1 > e0) {
             const int key = node_key(xx, zz);
             for (int e = e0; e < e1; ++e) {
-                const int2 en = a.ps_ent[e];
-                if (en.x == key) ds += amp ? amp[en.y] : cst;
+                const int2 en = gld(a.ps_ent, e);
+                if (en.x == key) ds += amp ? gld(amp, en.y) : cst;
             }
         }
         return ds;
@@ -564,8 +561,8 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
         if (f1 > f0 && col) {
             const int key = node_key(xx, zz);
             for (int e = f0; e < f1; ++e) {
-                const int2 en = a.rec_ent[e];
-                if (en.x == key) col[en.y] = (double)val;
+                const int2 en = gld(a.rec_ent, e);
+                if (en.x == key) gst(col, en.y, (double)val);
             }
         }
     };
@@ -592,7 +589,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
     }
 
     const float fxl = frame(x, R, nx - R - 1);
-    const float axl = a.axp[x + kGhostX];
+    const float axl = gld(a.axp, x + kGhostX);
     float un1[NM1];  // displacement after step n at the tile's own rows
 #pragma unroll
     for (int k = 0; k < NM1; ++k) {
@@ -600,21 +597,21 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
         un1[k] = 0.f;
         if (r >= H1) break;
         const int z = j0 - L1 + r, c = lane + L1;
-        const float ds = divs(x, z, r, c, fxl, frame(z, R, nz - R - 1), a.ps_amp, a.ps_const) + (syn1 ? msy[k] : 0.f);
-        const float vn = (mv[k] + mb[k] * ds) * (axl * a.azp[z + kGhost]);
+        const float ds = divs(x, z, r, c, fxl, frame(z, R, nz - R - 1), ps.ps_amp, ps.ps_const) + (syn1 ? msy[k] : 0.f);
+        const float vn = (mv[k] + mb[k] * ds) * (axl * gld(a.azp, z + kGhost));
         if (r >= L1 && r < L1 + TZ) {  // own node: outputs of step n
             const float un = HAS_U ? mu[k] + vn * a.dt : 0.f;
             un1[k] = un;
             const unsigned co = G.cofs(x, z);
             float out = vn;
             if (HAS_U) {
-                if (a.snap) bstore(rsrc(a.snap, cb), co, mu[k]);
+                if (ps.snap) bstore(rsrc(ps.snap, cb), co, mu[k]);
                 out = un;
-            } else if (a.snap) {
-                bstore(rsrc(a.snap, cb), co, mv[k]);
+            } else if (ps.snap) {
+                bstore(rsrc(ps.snap, cb), co, mv[k]);
             }
-            if (a.dsamp) bstore(rsrc(a.dsamp, cb), co, out);
-            gather(x, z, out, a.seis_col);
+            if (ps.dsamp) bstore(rsrc(ps.dsamp, cb), co, out);
+            gather(x, z, out, ps.seis_col);
         }
         VV[r * W1 + c] = vn;
     }
@@ -624,7 +621,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
         const int r = idx / HC1, h = idx - r * HC1;
         const int c = h < L1 ? h : h + kTX;
         const int xx = i0 - L1 + c, zz = j0 - L1 + r;
-        const float ds = divs(xx, zz, r, c, frame(xx, R, nx - R - 1), frame(zz, R, nz - R - 1), a.ps_amp, a.ps_const) +
+        const float ds = divs(xx, zz, r, c, frame(xx, R, nx - R - 1), frame(zz, R, nz - R - 1), ps.ps_amp, ps.ps_const) +
                          (syn1 ? hsy[k] : 0.f);
         VV[tid + PT * k < H1 * HC1 ? r * W1 + c : H1 * W1] = (hv[k] + hb[k] * ds) * ht[k];
     }
@@ -685,8 +682,8 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
         if (r < V2R0 || r > V2R1) continue;
         const int z = j0 - L1 + r, c = lane + L1;
         const float v1 = VV[r * W1 + c];
-        const float ds = divs(x, z, r, c, fxl, frame(z, R, nz - R - 1), pa.ps_amp2, pa.ps_const2) + (syn1 ? 0.f : msy[k]);
-        const float vn = (v1 + mb[k] * ds) * (axl * a.azp[z + kGhost]);
+        const float ds = divs(x, z, r, c, fxl, frame(z, R, nz - R - 1), ps.ps_amp2, ps.ps_const2) + (syn1 ? 0.f : msy[k]);
+        const float vn = (v1 + mb[k] * ds) * (axl * gld(a.azp, z + kGhost));
         if (r >= L1 && r < L1 + TZ) {  // own node: outputs of step n+1 and the new state
             G.put(a.o_v_out, x, z, vn);
             const unsigned co = G.cofs(x, z);
@@ -694,13 +691,13 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
             if (HAS_U) {
                 const float un = un1[k] + vn * a.dt;
                 G.put(a.o_u_out, x, z, un);
-                if (pa.snap2) bstore(rsrc(pa.snap2, cb), co, un1[k]);
+                if (ps.snap2) bstore(rsrc(ps.snap2, cb), co, un1[k]);
                 out = un;
-            } else if (pa.snap2) {
-                bstore(rsrc(pa.snap2, cb), co, v1);
+            } else if (ps.snap2) {
+                bstore(rsrc(ps.snap2, cb), co, v1);
             }
-            if (pa.dsamp2) bstore(rsrc(pa.dsamp2, cb), co, out);
-            gather(x, z, out, pa.seis_col2);
+            if (ps.dsamp2) bstore(rsrc(ps.dsamp2, cb), co, out);
+            gather(x, z, out, ps.seis_col2);
         }
         VV[r * W1 + c] = vn;
     }
@@ -711,8 +708,8 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
         const int c = h < L1 ? h : h + kTX;
         const bool in2 = tid + PT * k < H1 * HC1 && r >= V2R0 && r <= V2R1 && c >= V2C0 && c <= V2C1;
         const int xx = i0 - L1 + c, zz = j0 - L1 + r;
-        const float ds = divs(xx, zz, r, c, frame(xx, R, nx - R - 1), frame(zz, R, nz - R - 1), pa.ps_amp2,
-                              pa.ps_const2) + (syn1 ? 0.f : hsy[k]);
+        const float ds = divs(xx, zz, r, c, frame(xx, R, nx - R - 1), frame(zz, R, nz - R - 1), ps.ps_amp2,
+                              ps.ps_const2) + (syn1 ? 0.f : hsy[k]);
         const float vn = (VV[r * W1 + c] + hb[k] * ds) * ht[k];
         VV[in2 ? r * W1 + c : H1 * W1] = vn;
     }
@@ -729,6 +726,30 @@ __global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
         G.put(a.o_sxy_out, x, z, SX[rv * WSX + c + R] + dmm[k] * (dxv(rv, c) * fxs));
         G.put(a.o_szy_out, x, z, SZ[(rv + R) * W1 + c] + dmm[k] * (dzv(rv, c) * frame(z, R - 1, nz - R - 1)));
     }
+}
+
+
+template <int R, int MODE>
+__global__ __launch_bounds__(kPairThreads) void k_pair(PairArgs pa)
+{
+    const int g = blockIdx.x;
+    const PairStep ps{pa.a.ps_amp, pa.a.syn, pa.a.snap, pa.a.dsamp, pa.a.seis_col, pa.ps_amp2, pa.syn2, pa.snap2,
+                      pa.dsamp2, pa.seis_col2, pa.a.ps_const, pa.ps_const2, 0, 0};
+    pair_step<R, MODE>(pa.a, ps, (g % kXCD) * pa.a.tiles_per_xcd + g / kXCD);
+}
+
+template <int R, int MODE>
+__global__ __launch_bounds__(kPairThreads) void k_pair_group(PairGroupArgs G)
+{
+    const int g = blockIdx.x;
+    const int T = (g % kXCD) * G.tiles_per_xcd + g / kXCD;
+    if (T >= G.total_tiles) return;
+    int j = 0;
+    while (j + 1 < G.n_jobs && G.tile_start[j + 1] <= T) ++j;  // wave-uniform scan
+    j = __builtin_amdgcn_readfirstlane(j);
+    const PairStep ps = G.step[j];
+    const StepArgs a = G.jobs[2 * ps.slot + ps.parity];
+    pair_step<R, MODE>(a, ps, T - G.tile_start[j]);
 }
 
 // Structure-kernel product of run_noise_adjoint.m:324-331 after a step that stores it:
@@ -817,6 +838,23 @@ __global__ __launch_bounds__(256) void k_born_green(StepArgs a, float* sxy, floa
         for (int m = 1; m <= R; ++m) s += a.cz[m - 1] * (gin[c + m * nxl] - gin[c - (m - 1) * nxl]);
         szy[p] += d * s;
     }
+}
+
+void launch_pair_group(int R, bool with_u, const PairGroupArgs& ga, hipStream_t s)
+{
+    const dim3 g((unsigned)(kXCD * ga.tiles_per_xcd)), b(kPairThreads);
+#define FD2D_PGROUP(RR)                                                                          \
+    case RR:                                                                                     \
+        if (with_u) hipLaunchKernelGGL((k_pair_group<RR, 1>), g, b, 0, s, ga);                   \
+        else hipLaunchKernelGGL((k_pair_group<RR, 0>), g, b, 0, s, ga);                          \
+        break;
+    switch (R) {
+        FD2D_PGROUP(1)
+        FD2D_PGROUP(2)
+        FD2D_PGROUP(3)
+        FD2D_PGROUP(4)
+    }
+#undef FD2D_PGROUP
 }
 
 void launch_pair(int R, bool with_u, const PairArgs& p, hipStream_t s)
