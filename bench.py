@@ -475,15 +475,18 @@ def main():
         n = 1 << 28  # 1 GiB of FP32 read + 1 GiB written per copy
         a = torch.ones(n, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
-        for _ in range(args.warmup):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        print(json.dumps({"copy_bytes_per_launch": 2 * 4 * n, "GBps": 2 * 4 * n * args.steps / el / 1e9}))
+        out = {"copy_bytes_per_launch": 2 * 4 * n}
+        # the runtime's device copy, and a streaming elementwise kernel (torch.neg) over the same bytes
+        for name, fn in (("GBps", lambda: b.copy_(a)), ("elementwise_GBps", lambda: torch.neg(a, out=b))):
+            for _ in range(max(args.warmup, 1)):
+                fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                fn()
+            torch.cuda.synchronize()
+            out[name] = 2 * 4 * n * args.steps / (time.perf_counter() - t0) / 1e9
+        print(json.dumps(out))
         return
     if args.workload == "stations":
         stations_bench(args)
