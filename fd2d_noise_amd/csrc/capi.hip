@@ -631,15 +631,32 @@ int slab_len(size_t P, size_t n_samples)
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
         while (T > 64 && T * P * sizeof(float) > free_b / 8) T /= 2;
+    // several slabs: each starts on a 16-sample boundary (the phase tables are read 8 samples at a time)
+    if (T < n_samples) T = std::max<size_t>(16, T / 16 * 16);
     return (int)T;
 }
+
+// The three bfloat16 parts of a phase table, each [rows][cols] (uint16), zero where not set.
+struct SplitTable {
+    size_t rows = 0, cols = 0;
+    std::vector<unsigned short> h, m, l;
+    void resize(size_t r, size_t c)
+    {
+        rows = r;
+        cols = c;
+        h.assign(r * c, 0);
+        m.assign(r * c, 0);
+        l.assign(r * c, 0);
+    }
+    void set(size_t r, size_t c, double v) { split_bf16((float)v, h[r * cols + c], m[r * cols + c], l[r * cols + c]); }
+};
 
 struct DftAccumulator {
     double2* acc = nullptr;
     size_t P = 0;
-    int K = 0, C_pad = 0;
+    int K = 0, C_pad = 0, Jt = 0;
     DBuf<float> slab;
-    DBuf<float> etab;  // [n_samples][C_pad]: (Re, Im) of the phase factor of each sample, FP32
+    DBuf<unsigned short> eh, em, el;  // [C_pad][Jt]: parts of (Re, Im) of each sample's phase factor
     int cnt = 0, done = 0, T = 0;
     hipStream_t st = nullptr;
     void init(double2* acc_, size_t P_, int K_, const std::vector<double2>& table, size_t n_samples, hipStream_t s)
@@ -649,15 +666,19 @@ struct DftAccumulator {
         K = K_;
         st = s;
         C_pad = spec_cols(K);
-        T = slab_len(P, n_samples);
+        T = slab_len(P, n_samples);  // one slab, or a multiple of 16 samples
+        Jt = (int)((n_samples + 15) / 16 * 16);
         slab.alloc((size_t)T * P);
-        std::vector<float> e(n_samples * (size_t)C_pad, 0.f);
+        SplitTable e;
+        e.resize(C_pad, Jt);
         for (size_t j = 0; j < n_samples; ++j)
             for (int k = 0; k < K; ++k) {
-                e[j * C_pad + 2 * k] = (float)table[j * K + k].x;
-                e[j * C_pad + 2 * k + 1] = (float)table[j * K + k].y;
+                e.set(2 * k, j, table[j * K + k].x);
+                e.set(2 * k + 1, j, table[j * K + k].y);
             }
-        upload(etab, e, st);
+        upload(eh, e.h, st);
+        upload(em, e.m, st);
+        upload(el, e.l, st);
         HIPCHK(hipMemsetAsync(acc, 0, P * (size_t)K * sizeof(double2), st));
     }
     float* next_slot() { return slab.p + (size_t)cnt * P; }
@@ -668,7 +689,7 @@ struct DftAccumulator {
     void flush()
     {
         if (cnt == 0) return;
-        launch_dft_mfma(acc, slab.p, etab.p + (size_t)done * C_pad, P, K, C_pad, cnt, st);
+        launch_dft_bf16(acc, slab.p, eh.p + done, em.p + done, el.p + done, P, K, C_pad, Jt, cnt, st);
         done += cnt;
         cnt = 0;
     }
@@ -679,7 +700,7 @@ struct Synthesiser {
     int K = 0, C_pad = 0, T = 0, J_pad = 0, n_total = 0;
     DBuf<float> slab;
     DBuf<float> a32;  // [C_pad][P]: (Re, Im) rows of the source spectrum, FP32
-    DBuf<float> et;   // per slab [C_pad][J_pad]: (Re, -Im) of the phase factors, FP32
+    DBuf<unsigned short> th, tm, tl;  // per slab [J_pad][C_pad]: parts of (Re, -Im) of the phase factors
     hipStream_t st = nullptr;
     void init(const double2* A, size_t P_, int K_, const std::vector<double2>& table, int n_samples, hipStream_t s)
     {
@@ -689,28 +710,32 @@ struct Synthesiser {
         n_total = n_samples;
         C_pad = spec_cols(K);
         T = slab_len(P, (size_t)std::max(n_samples, 1));
-        J_pad = (T + 255) / 256 * 256;
+        J_pad = (T + 63) / 64 * 64;
         slab.alloc((size_t)T * P);
         a32.alloc((size_t)C_pad * P);
         launch_spectrum_f32(a32.p, A, P, K, C_pad, st);
         const int n_slabs = (std::max(n_samples, 1) + T - 1) / T;
-        std::vector<float> e((size_t)n_slabs * C_pad * J_pad, 0.f);
+        SplitTable e;
+        e.resize((size_t)n_slabs * J_pad, C_pad);
         for (int idx = 0; idx < n_samples; ++idx) {
-            const size_t base = (size_t)(idx / T) * C_pad * J_pad + (size_t)(idx % T);
+            const size_t row = (size_t)(idx / T) * J_pad + (size_t)(idx % T);
             for (int k = 0; k < K; ++k) {
-                e[base + (size_t)(2 * k) * J_pad] = (float)table[(size_t)idx * K + k].x;
-                e[base + (size_t)(2 * k + 1) * J_pad] = -(float)table[(size_t)idx * K + k].y;
+                e.set(row, 2 * k, table[(size_t)idx * K + k].x);
+                e.set(row, 2 * k + 1, -table[(size_t)idx * K + k].y);
             }
         }
-        upload(et, e, st);
+        upload(th, e.h, st);
+        upload(tm, e.m, st);
+        upload(tl, e.l, st);
     }
     // Source slice of the idx-th sampled step (0-based); synthesises a new block when needed.
     const float* slice(int idx)
     {
         const int slot = idx % T;
-        if (slot == 0)
-            launch_synth_mfma(slab.p, a32.p, et.p + (size_t)(idx / T) * C_pad * J_pad, P, C_pad, J_pad,
-                              std::min(T, n_total - idx), st);
+        if (slot == 0) {
+            const size_t o = (size_t)(idx / T) * J_pad * C_pad;
+            launch_synth_bf16(slab.p, a32.p, th.p + o, tm.p + o, tl.p + o, P, C_pad, std::min(T, n_total - idx), st);
+        }
         return slab.p + (size_t)slot * P;
     }
 };
@@ -1695,10 +1720,10 @@ int64_t solve_bytes(const fd2d_solve* s)
     b += (int64_t)(g.nx / kTX + 2) * (g.nz / kTileZ + 2) * 8 + 8 * 9 * (n_src + n_rec);  // tile lists
     const int N = 2 * g.nt - 1;
     auto slab = [&](int64_t n) { return std::min<int64_t>(kSpecSlab, std::max<int64_t>(n, 1)); };
-    auto dft_bytes = [&](int64_t n) { return slab(n) * P * 4 + n * C_pad * 4; };
+    auto dft_bytes = [&](int64_t n) { return slab(n) * P * 4 + 3 * ((n + 15) / 16 * 16) * C_pad * 2; };
     auto syn_bytes = [&](int64_t n) {
-        const int64_t T = slab(n), J = (T + 255) / 256 * 256, slabs = (std::max<int64_t>(n, 1) + T - 1) / T;
-        return T * P * 4 + C_pad * P * 4 + slabs * C_pad * J * 4 + P * K * 16;  // slab, a32, tables, A
+        const int64_t T = slab(n), J = (T + 63) / 64 * 64, slabs = (std::max<int64_t>(n, 1) + T - 1) / T;
+        return T * P * 4 + C_pad * P * 4 + 3 * slabs * C_pad * J * 2 + P * K * 16;  // slab, a32, tables, A
     };
     const bool born = numel(s->dmu) != 0;
     if (s->solver == FD2D_GREEN) {

@@ -207,14 +207,19 @@ void launch_material(const double* mu, const double* rho, float* bv, float* dtmu
                      int pitch, double dt, hipStream_t s);
 void launch_scale_to_float(const double* in, float* out, size_t n, double scale, hipStream_t s);
 
-// acc[k][x] += sum_j samp[j][x] * e[j][k] over cnt samples: FP32 matrix-core products, FP64
-// accumulators; etab[j][2k], etab[j][2k+1] = Re, Im e[j][k] with C_pad (a multiple of 256) columns.
-void launch_dft_mfma(double2* acc, const float* samp, const float* etab, size_t P, int K, int C_pad, int cnt,
-                     hipStream_t s);
-// out[j][x] = sum_c et[c][j] * a[c][x] for j < cnt (FP32 matrix cores): with a = (Re A, Im A) rows
-// and et = (Re, -Im) of the phase factors this is Re(A[k][x] * coef[j][k]) summed over k.
-void launch_synth_mfma(float* out, const float* a, const float* et, size_t P, int C_pad, int J_pad, int cnt,
-                       hipStream_t s);
+// acc[k][x] += sum_j samp[j][x] * e[j][k] over cnt samples: matrix-core products of the three
+// bfloat16 parts of each FP32 operand (FP32-accurate), FP64 accumulators; e{h,m,l}[c][Jt] are the
+// parts of Re e[j][k] (c = 2k) and Im e[j][k] (c = 2k + 1), C_pad (a multiple of 256) rows, offset
+// to the slab's first sample (16-byte aligned).
+void launch_dft_bf16(double2* acc, const float* samp, const unsigned short* eh, const unsigned short* em,
+                     const unsigned short* el, size_t P, int K, int C_pad, int Jt, int cnt, hipStream_t s);
+// out[j][x] = sum_c et[j][c] * a[c][x] for j < cnt, the same products: with a = (Re A, Im A) rows
+// and et = (Re, -Im) of the phase factors (parts t{h,m,l}[J_pad][C_pad], J_pad >= cnt rounded up to
+// 64) this is Re(A[k][x] * coef[j][k]) summed over k.
+void launch_synth_bf16(float* out, const float* a, const unsigned short* th, const unsigned short* tm,
+                       const unsigned short* tl, size_t P, int C_pad, int cnt, hipStream_t s);
+// the three bfloat16 parts (rounded to nearest, v = h + m + l exactly) of an FP32 value
+void split_bf16(float v, unsigned short& h, unsigned short& m, unsigned short& l);
 // a[2k][x], a[2k+1][x] = Re, Im A[k][x] (FP32), zero rows 2K..C_pad-1
 void launch_spectrum_f32(float* a, const double2* A, size_t P, int K, int C_pad, hipStream_t s);
 // A[k][x] = spec_sum[k] * dist(x,k) * (conj_in ? conj(Z) : Z)[k][x], dist(x,k) = sum_b I[k,b] sd[b][x]

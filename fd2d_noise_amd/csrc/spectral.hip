@@ -1,4 +1,4 @@
-// spectral.hip -- frequency-domain slab products of the engine on the FP32 matrix cores.
+// spectral.hip -- frequency-domain slab products of the engine on the matrix cores.
 //
 // Both contractions are tall-skinny products over the P grid nodes, with the complex phase
 // factors of a slab of time samples split into real columns c = 2k (real part) and c = 2k + 1:
@@ -6,174 +6,338 @@
 //           code/run_forward1_green.m:176-183, run_noise_adjoint.m:335-346)
 //   synth:  out[j][x]  = sum_c et[c][j] * a[c][x]                  (the distributed source of the
 //           sampled steps, run_forward2_correlation.m:195-216: Re(A e^{iwt}) = Ar*cr - Ai*ci)
-// Samples are FP32 (the wavefields), the phase tables and spectra are rounded to FP32, the products
-// run on v_mfma_f32_32x32x2_f32 with FP32 accumulation over one slab, and every slab is added into
-// FP64 accumulators (flush) or written as the FP32 source slice it becomes anyway (synth).
+// Samples and spectra are FP32, the phase factors are rounded to FP32. Every FP32 operand is split
+// exactly into three bfloat16 parts, x = h + m + l (8 significant bits each, rounded to nearest),
+// and a product is formed from the six part products down to 2^-16 of the leading one,
+//   a b ~ ah bh + ah bm + am bh + am bm + ah bl + al bh,
+// which leaves out terms below 2^-23 of |a b|, the size of an FP32 product's own rounding. The part
+// products run on v_mfma_f32_32x32x16_bf16 (17x the rate of the FP32 v_mfma_f32_32x32x2_f32, so six
+// of them take a third of its time) with FP32 accumulation over one slab, and every slab is added
+// into FP64 accumulators (flush) or written as the FP32 source slice it becomes anyway (synth). The
+// phase tables are split once on the host; the samples and spectra as they are loaded.
 //
-// MFMA 32x32x2 (one wavefront): lane l supplies A[i = l % 32][k = l / 32] and B[k = l / 32][j = l % 32];
-// it receives D[i][j] for j = l % 32 and i = 8 * (r / 4) + 4 * (l / 32) + (r % 4), r = 0..15.
+// MFMA 32x32x16 (one wavefront): lane l supplies A[i = l % 32][k = 8 (l / 32) + v] and
+// B[k = 8 (l / 32) + v][j = l % 32] for v = 0..7; it receives D[i][j] for j = l % 32 and
+// i = 8 * (r / 4) + 4 * (l / 32) + (r % 4), r = 0..15.
 #include "engine.h"
+
+#include <algorithm>
+#include <cstring>
 
 namespace fd2d {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-
-__device__ __forceinline__ f32x16 mfma32(float a, float b, f32x16 d)
-{
-    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, d, 0, 0, 0);
-}
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int d_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
-// ---------------------------------------------------------------------------------------------
-// DFT flush. Workgroup = 64 nodes x 256 columns (128 frequencies); wavefront w owns columns
-// 64w..64w+63 as two 32-row MFMA blocks against the two 32-node blocks.
-// ---------------------------------------------------------------------------------------------
-constexpr int kFX = 64;    // nodes per workgroup
-constexpr int kFC = 256;   // table columns per workgroup
-constexpr int kFU = 8;     // k-steps (sample pairs) per unrolled group
-
-__global__ __launch_bounds__(256) void k_dft_mfma(double2* __restrict__ acc, const float* __restrict__ samp,
-                                                  const float* __restrict__ etab, long P, int K, int C_pad, int cnt)
+// the three bfloat16 parts of FP32 values: h = x rounded to 8 significant bits (to nearest, ties to
+// even), m the same of x - h, l = x - h - m (exact: each difference is representable, and l has at
+// most 8 bits left). Rounding to nearest keeps |m| <= 2^-8 |x| and |l| <= 2^-16 |x| with random
+// signs, so the part products left out of a product (m l, l m, l l) stay below 2^-23 of it and
+// carry no bias. split2 packs the parts of two values as the bfloat16 pairs of three 32-bit words.
+struct Split8 {
+    bf16x8 h, m, l;
+};
+__device__ __forceinline__ unsigned bf16_rne(unsigned u) { return (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u; }
+__device__ __forceinline__ void split2(float v0, float v1, unsigned& h, unsigned& m, unsigned& l)
 {
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    unsigned hb[2], mb[2], lb[2];
+    const float v[2] = {v0, v1};
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        hb[e] = bf16_rne(__float_as_uint(v[e]));
+        const float r1 = v[e] - __uint_as_float(hb[e]);
+        mb[e] = bf16_rne(__float_as_uint(r1));
+        lb[e] = __float_as_uint(r1 - __uint_as_float(mb[e])) & 0xffff0000u;
+    }
+    h = (hb[0] >> 16) | hb[1];
+    m = (mb[0] >> 16) | mb[1];
+    l = (lb[0] >> 16) | lb[1];
+}
+// 8 consecutive bfloat16 values of a pre-split table (16-byte aligned)
+__device__ __forceinline__ bf16x8 ld8(const unsigned short* p)
+{
+    return __builtin_bit_cast(bf16x8, *(const u32x4*)p);
+}
+__device__ __forceinline__ Split8 ld_split(const unsigned short* h, const unsigned short* m, const unsigned short* l,
+                                           long o)
+{
+    return {ld8(h + o), ld8(m + o), ld8(l + o)};
+}
+__device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 d)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, d, 0, 0, 0);
+}
+// d[p][nb] += A[p] B over the six part products, smallest first; the two column (or slice)
+// blocks interleave
+template <int NB>
+__device__ __forceinline__ void mfma6(f32x16 (&d)[2][NB], const Split8 (&A)[2], const Split8& b, int nb)
+{
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const Split8& a = A[p];
+            switch (t) {
+            case 0: d[p][nb] = mfma(a.l, b.h, d[p][nb]); break;
+            case 1: d[p][nb] = mfma(a.h, b.l, d[p][nb]); break;
+            case 2: d[p][nb] = mfma(a.m, b.m, d[p][nb]); break;
+            case 3: d[p][nb] = mfma(a.m, b.h, d[p][nb]); break;
+            case 4: d[p][nb] = mfma(a.h, b.m, d[p][nb]); break;
+            default: d[p][nb] = mfma(a.h, b.h, d[p][nb]); break;
+            }
+        }
+}
+
+void split_bf16(float v, unsigned short& h, unsigned short& m, unsigned short& l)
+{
+    auto bits = [](float f) {
+        unsigned u;
+        std::memcpy(&u, &f, 4);
+        return u;
+    };
+    auto value = [](unsigned u) {
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+    };
+    auto rne = [](unsigned u) { return (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u; };
+    const unsigned hb = rne(bits(v));
+    const float r1 = v - value(hb);
+    const unsigned mb = rne(bits(r1));
+    const unsigned lb = bits(r1 - value(mb)) & 0xffff0000u;
+    h = (unsigned short)(hb >> 16);
+    m = (unsigned short)(mb >> 16);
+    l = (unsigned short)(lb >> 16);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Staging of the FP32 operand shared by a workgroup's wavefronts. Per chunk of 16 contraction
+// rows, the rows of its 128 nodes are loaded kPF chunks ahead (each thread some rows of one
+// node), split into their three parts and written to LDS in the B-operand order of the MFMA: for
+// node block nb, part p, lane (half, col): the 8 values of rows 8 half .. 8 half + 7 of node
+// 32 nb + col. Two LDS buffers alternate, so one barrier per chunk orders the writes after the
+// previous reads. Each wavefront multiplies its two 32-row table blocks with all four node blocks,
+// so every table value it loads serves 128 nodes.
+// ---------------------------------------------------------------------------------------------
+constexpr int kCH = 16;   // contraction rows per chunk (one MFMA k-step)
+
+// staging area [buf][node block (NB)][part][half][col], 16 bytes each
+template <int NB>
+__device__ __forceinline__ const u32x4* b_slot(const u32x4* Bs, int buf, int nb, int half, int col)
+{
+    return Bs + ((buf * NB + nb) * 3) * 64 + half * 32 + col;  // parts 64 apart
+}
+// split RPT consecutive rows v0 .. v0 + RPT - 1 at lane position (half, col) of node block nb and
+// write their parts (RPT = 2, 4 or 8)
+template <int NB, int RPT>
+__device__ __forceinline__ void stage_rows(u32x4* Bs, int buf, int nb, int half, int col, int v0, const float (&r)[RPT])
+{
+    unsigned ph[RPT / 2], pm[RPT / 2], pl[RPT / 2];
+#pragma unroll
+    for (int q = 0; q < RPT / 2; ++q) split2(r[2 * q], r[2 * q + 1], ph[q], pm[q], pl[q]);
+    unsigned* base = (unsigned*)b_slot<NB>(Bs, buf, nb, half, col) + v0 / 2;
+#pragma unroll
+    for (int q = 0; q < RPT / 2; ++q) {
+        base[q] = ph[q];
+        base[q + 64 * 4] = pm[q];
+        base[q + 128 * 4] = pl[q];
+    }
+}
+template <int NB>
+__device__ __forceinline__ Split8 read_b(const u32x4* Bs, int buf, int nb, int lane)
+{
+    const u32x4* q = b_slot<NB>(Bs, buf, nb, lane >> 5, lane & 31);
+    return {__builtin_bit_cast(bf16x8, q[0]), __builtin_bit_cast(bf16x8, q[64]), __builtin_bit_cast(bf16x8, q[128])};
+}
+
+// ---------------------------------------------------------------------------------------------
+// DFT flush. Workgroup = 128 nodes x 256 columns (128 frequencies); wavefront w owns columns
+// 64w..64w+63 as two 32-row MFMA blocks against the four 32-node blocks. Thread t stages rows
+// 8 (t / 128) .. + 7 of every chunk of samples for node t % 128. Table parts are [C_pad][Jt]
+// (samples contiguous), offset to the slab's first sample (a multiple of 16), loaded one chunk
+// ahead.
+// ---------------------------------------------------------------------------------------------
+constexpr int kFX = 128;   // nodes per workgroup
+constexpr int kNB = kFX / 32;
+constexpr int kFC = 256;   // table columns per workgroup
+constexpr int kPFD = 2;    // chunks of samples loaded ahead
+
+__global__ __launch_bounds__(256, 2) void k_dft_bf16(double2* __restrict__ acc, const float* __restrict__ samp,
+                                                  const unsigned short* __restrict__ eh,
+                                                  const unsigned short* __restrict__ em,
+                                                  const unsigned short* __restrict__ el, long P, int K, int Jt,
+                                                  int cnt)
+{
+    __shared__ u32x4 Bs[2 * kNB * 3 * 64];  // [buf][node block][part][half][col]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const long x0 = (long)blockIdx.x * kFX;
     const int c0 = blockIdx.y * kFC + w * 64;
     const int half = lane >> 5, col = lane & 31;
-    // clamped node columns of the two B blocks (loads stay inside the slab; stores are masked)
-    const long xa = x0 + col < P ? x0 + col : P - 1;
-    const long xb = x0 + 32 + col < P ? x0 + 32 + col : P - 1;
-    const float* sa = samp + xa + (long)half * P;
-    const float* sb = samp + xb + (long)half * P;
-    const float* ea = etab + c0 + col + (long)half * C_pad;
-    const float* eb = ea + 32;
-    f32x16 d00 = {}, d01 = {}, d10 = {}, d11 = {};  // [column block][node block]
-    const int pairs = cnt >> 1;
-    int p = 0;
-    for (; p + kFU <= pairs; p += kFU) {
-        float a0[kFU], a1[kFU], b0[kFU], b1[kFU];
+    // staging: node x0 + (tid % 128) (clamped: loads stay inside the slab, stores are masked)
+    const int sn = tid & (kFX - 1), sh = tid >> 7;
+    const float* sp = samp + (x0 + sn < P ? x0 + sn : P - 1);
+    const int nch = (cnt + kCH - 1) / kCH;
+    // Loads are unconditional (rows past the slab read its last row, then count as zero samples
+    // against the next slab's factors): a conditional load would make the compiler wait for every
+    // load in flight before the first product of each chunk.
+    auto fetch = [&](int q, float (&r)[8]) {
 #pragma unroll
-        for (int u = 0; u < kFU; ++u) {
-            const long j = 2L * (p + u);
-            b0[u] = sa[j * P];
-            b1[u] = sb[j * P];
-            a0[u] = ea[j * C_pad];
-            a1[u] = eb[j * C_pad];
-        }
-#pragma unroll
-        for (int u = 0; u < kFU; ++u) {
-            d00 = mfma32(a0[u], b0[u], d00);
-            d01 = mfma32(a0[u], b1[u], d01);
-            d10 = mfma32(a1[u], b0[u], d10);
-            d11 = mfma32(a1[u], b1[u], d11);
-        }
-    }
-    for (; 2 * p < cnt; ++p) {  // remaining pairs; an odd last sample pairs with zero
-        const long j = 2L * p;
-        const bool ok = j + half < cnt;
-        const float b0 = ok ? sa[j * P] : 0.f, b1 = ok ? sb[j * P] : 0.f;
-        const float a0 = ok ? ea[j * C_pad] : 0.f, a1 = ok ? eb[j * C_pad] : 0.f;
-        d00 = mfma32(a0, b0, d00);
-        d01 = mfma32(a0, b1, d01);
-        d10 = mfma32(a1, b0, d10);
-        d11 = mfma32(a1, b1, d11);
-    }
-    // rows come in (real, imaginary) pairs: column c0 + 32cb + row, row even
-    auto put = [&](const f32x16& d, int cb, int nb) {
-        const long x = x0 + 32 * nb + col;
-        if (x >= P) return;
-#pragma unroll
-        for (int r = 0; r < 16; r += 2) {
-            const int k = (c0 + 32 * cb + d_row(r, lane)) >> 1;
-            if (k < K) {
-                double2* q = acc + (long)k * P + x;
-                double2 v = *q;
-                v.x += (double)d[r];
-                v.y += (double)d[r + 1];
-                *q = v;
-            }
+        for (int i = 0; i < 8; ++i) {
+            const int j = kCH * q + 8 * sh + i;
+            const float v = sp[(long)(j < cnt ? j : cnt - 1) * P];
+            r[i] = j < cnt ? v : 0.f;
         }
     };
-    put(d00, 0, 0);
-    put(d01, 0, 1);
-    put(d10, 1, 0);
-    put(d11, 1, 1);
+    const long ra = (long)(c0 + col) * Jt + 8 * half, rb = ra + 32L * Jt;  // the lane's two table rows
+    auto table = [&](int q, Split8 (&E)[2]) {
+        E[0] = ld_split(eh, em, el, ra + kCH * q);
+        E[1] = ld_split(eh, em, el, rb + kCH * q);
+    };
+    float raw[kPFD][8];
+#pragma unroll
+    for (int u = 0; u < kPFD; ++u) fetch(u < nch ? u : nch - 1, raw[u]);
+    Split8 E[2];
+    table(0, E);
+    f32x16 d[2][kNB] = {};  // [column block][node block]
+    for (int q0 = 0; q0 < nch; q0 += kPFD) {
+#pragma unroll
+        for (int u = 0; u < kPFD; ++u) {
+            const int q = q0 + u;
+            if (q >= nch) break;  // uniform
+            stage_rows<kNB, 8>(Bs, q & 1, sn >> 5, sh, sn & 31, 0, raw[u]);
+            fetch(q + kPFD < nch ? q + kPFD : nch - 1, raw[u]);
+            __syncthreads();
+            Split8 En[2];
+            table(q + 1 < nch ? q + 1 : q, En);
+            __builtin_amdgcn_sched_barrier(0);  // issue the next chunk's table loads before the products
+#pragma unroll
+            for (int nb = 0; nb < kNB; ++nb) mfma6<kNB>(d, E, read_b<kNB>(Bs, q & 1, nb, lane), nb);
+            E[0] = En[0];
+            E[1] = En[1];
+        }
+    }
+    // rows come in (real, imaginary) pairs: column c0 + 32cb + row, row even. The accumulators of
+    // two node blocks are read together (clamped addresses) before any is written back.
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int n2 = 0; n2 < kNB; n2 += 2) {
+            double2 v[2][8];
+            long o[2][8];
+            bool ok[2][8];
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const long x = x0 + 32 * (n2 + e) + col;
+                    const int k = (c0 + 32 * cb + d_row(2 * r, lane)) >> 1;
+                    ok[e][r] = x < P && k < K;
+                    o[e][r] = ok[e][r] ? (long)k * P + x : 0;
+                    v[e][r] = acc[o[e][r]];
+                }
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+#pragma unroll
+                for (int r = 0; r < 8; ++r)
+                    if (ok[e][r]) {
+                        v[e][r].x += (double)d[cb][n2 + e][2 * r];
+                        v[e][r].y += (double)d[cb][n2 + e][2 * r + 1];
+                        acc[o[e][r]] = v[e][r];
+                    }
+        }
 }
 
-void launch_dft_mfma(double2* acc, const float* samp, const float* etab, size_t P, int K, int C_pad, int cnt,
-                     hipStream_t s)
+void launch_dft_bf16(double2* acc, const float* samp, const unsigned short* eh, const unsigned short* em,
+                     const unsigned short* el, size_t P, int K, int C_pad, int Jt, int cnt, hipStream_t s)
 {
     if (cnt <= 0 || K <= 0) return;
     const unsigned nxb = (unsigned)((P + kFX - 1) / kFX);
     const unsigned ncg = (unsigned)(C_pad / kFC);
-    hipLaunchKernelGGL(k_dft_mfma, dim3(nxb, ncg), dim3(256), 0, s, acc, samp, etab, (long)P, K, C_pad, cnt);
+    hipLaunchKernelGGL(k_dft_bf16, dim3(nxb, ncg), dim3(256), 0, s, acc, samp, eh, em, el, (long)P, K, Jt, cnt);
 }
 
 // ---------------------------------------------------------------------------------------------
-// Synthesis. Workgroup = 64 nodes x 256 time slices; wavefront w owns slices 64w..64w+63 as two
-// 32-row MFMA blocks against the two 32-node blocks; the contraction runs over the C_pad spectrum
-// rows (zero rows pad 2K up to a multiple of 256).
+// Synthesis. Workgroup = 64 nodes x 512 time slices (8 wavefronts); wavefront w owns slices
+// 64w..64w+63 as two 32-row MFMA blocks against the two 32-node blocks (wavefronts past cnt
+// compute on slice 0 and store nothing, so that every load stays unconditional). Thread t stages
+// rows 2 (t / 64), 2 (t / 64) + 1 of every chunk of spectrum rows for node t % 64, so each
+// spectrum row is read once per block of slices; the contraction runs over the C_pad spectrum rows
+// (zero rows pad 2K up to a multiple of 256). Table parts are [J_pad][C_pad] (spectrum rows
+// contiguous) per slab, loaded at the start of each chunk.
 // ---------------------------------------------------------------------------------------------
-constexpr int kSJ = 256;  // time slices per workgroup
+constexpr int kSX = 64;    // nodes per workgroup
+constexpr int kSNB = kSX / 32;
+constexpr int kSJ = 512;   // time slices per workgroup
+constexpr int kPFS = 4;    // chunks of spectrum rows loaded ahead
 
-__global__ __launch_bounds__(256) void k_synth_mfma(float* __restrict__ out, const float* __restrict__ a,
-                                                    const float* __restrict__ et, long P, int C_pad, int J_pad,
+__global__ __launch_bounds__(512) void k_synth_bf16(float* __restrict__ out, const float* __restrict__ a,
+                                                    const unsigned short* __restrict__ th,
+                                                    const unsigned short* __restrict__ tm,
+                                                    const unsigned short* __restrict__ tl, long P, int C_pad,
                                                     int cnt)
 {
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const long x0 = (long)blockIdx.x * kFX;
+    __shared__ u32x4 Bs[2 * kSNB * 3 * 64];  // [buf][node block][part][half][col]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const long x0 = (long)blockIdx.x * kSX;
     const int j0 = blockIdx.y * kSJ + w * 64;
+    const bool active = j0 < cnt;  // wave-uniform
     const int half = lane >> 5, col = lane & 31;
-    const long xa = x0 + col < P ? x0 + col : P - 1;
-    const long xb = x0 + 32 + col < P ? x0 + 32 + col : P - 1;
-    const float* pa = a + xa + (long)half * P;      // a[c][x], row c = 2q + half
-    const float* pb = a + xb + (long)half * P;
-    const float* ta = et + j0 + col + (long)half * J_pad;  // et[c][j]
-    const float* tb = ta + 32;
-    f32x16 d00 = {}, d01 = {}, d10 = {}, d11 = {};  // [slice block][node block]
-    const int pairs = C_pad >> 1;  // C_pad is a multiple of 256
-    for (int p = 0; p < pairs; p += kFU) {
-        float a0[kFU], a1[kFU], b0[kFU], b1[kFU];
+    const float* sp = a + (x0 + lane < P ? x0 + lane : P - 1);  // a[c][x]; staging: node lane, rows 2w, 2w + 1
+    const int nch = C_pad / kCH;
+    auto fetch = [&](int q, float (&r)[2]) {
 #pragma unroll
-        for (int u = 0; u < kFU; ++u) {
-            const long c = 2L * (p + u);
-            b0[u] = pa[c * P];
-            b1[u] = pb[c * P];
-            a0[u] = ta[c * J_pad];
-            a1[u] = tb[c * J_pad];
-        }
+        for (int i = 0; i < 2; ++i) r[i] = sp[(long)(kCH * q + 2 * w + i) * P];
+    };
+    // the lane's two slices (slice 0 for the wavefronts past cnt)
+    const long ta = (long)((active ? j0 : 0) + col) * C_pad + 8 * half, tb = ta + 32L * C_pad;
+    auto table = [&](int q, Split8 (&T)[2]) {
+        T[0] = ld_split(th, tm, tl, ta + kCH * q);
+        T[1] = ld_split(th, tm, tl, tb + kCH * q);
+    };
+    float raw[kPFS][2];
 #pragma unroll
-        for (int u = 0; u < kFU; ++u) {
-            d00 = mfma32(a0[u], b0[u], d00);
-            d01 = mfma32(a0[u], b1[u], d01);
-            d10 = mfma32(a1[u], b0[u], d10);
-            d11 = mfma32(a1[u], b1[u], d11);
+    for (int u = 0; u < kPFS; ++u) fetch(u < nch ? u : nch - 1, raw[u]);
+    f32x16 d[2][kSNB] = {};  // [slice block][node block]
+    for (int q0 = 0; q0 < nch; q0 += kPFS) {
+#pragma unroll
+        for (int u = 0; u < kPFS; ++u) {
+            const int q = q0 + u;
+            if (q >= nch) break;  // uniform
+            Split8 T[2];
+            table(q, T);  // L2-resident; in flight across the staging below
+            __builtin_amdgcn_sched_barrier(0);
+            stage_rows<kSNB, 2>(Bs, q & 1, lane >> 5, w >> 2, lane & 31, 2 * (w & 3), raw[u]);
+            fetch(q + kPFS < nch ? q + kPFS : nch - 1, raw[u]);
+            __syncthreads();
+#pragma unroll
+            for (int nb = 0; nb < kSNB; ++nb) mfma6<kSNB>(d, T, read_b<kSNB>(Bs, q & 1, nb, lane), nb);
         }
     }
-    auto put = [&](const f32x16& d, int jb, int nb) {
-        const long x = x0 + 32 * nb + col;
-        if (x >= P) return;
+    if (!active) return;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int j = j0 + 32 * jb + d_row(r, lane);
-            if (j < cnt) out[(long)j * P + x] = d[r];
+    for (int jb = 0; jb < 2; ++jb)
+#pragma unroll
+        for (int nb = 0; nb < kSNB; ++nb) {
+            const long x = x0 + 32 * nb + col;
+            if (x >= P) continue;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int j = j0 + 32 * jb + d_row(r, lane);
+                if (j < cnt) out[(long)j * P + x] = d[jb][nb][r];
+            }
         }
-    };
-    put(d00, 0, 0);
-    put(d01, 0, 1);
-    put(d10, 1, 0);
-    put(d11, 1, 1);
 }
 
-void launch_synth_mfma(float* out, const float* a, const float* et, size_t P, int C_pad, int J_pad, int cnt,
-                       hipStream_t s)
+void launch_synth_bf16(float* out, const float* a, const unsigned short* th, const unsigned short* tm,
+                       const unsigned short* tl, size_t P, int C_pad, int cnt, hipStream_t s)
 {
     if (cnt <= 0) return;
-    const unsigned nxb = (unsigned)((P + kFX - 1) / kFX);
+    const unsigned nxb = (unsigned)((P + kSX - 1) / kSX);
     const unsigned njg = (unsigned)((cnt + kSJ - 1) / kSJ);
-    hipLaunchKernelGGL(k_synth_mfma, dim3(nxb, njg), dim3(256), 0, s, out, a, et, (long)P, C_pad, J_pad, cnt);
+    hipLaunchKernelGGL(k_synth_bf16, dim3(nxb, njg), dim3(kSJ), 0, s, out, a, th, tm, tl, (long)P, C_pad, cnt);
 }
 
 // a[2k][x] = Re A[k][x], a[2k+1][x] = Im A[k][x] (FP32), zero rows 2K..C_pad-1
