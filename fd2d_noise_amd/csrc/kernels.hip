@@ -8,9 +8,10 @@
 //             distributed source, v = (v + dt/rho*DS) * taper, u += v*dt, snapshot of the
 //             pre-update field, DFT sample store, receiver gather;
 //   stress:   sigma += dt*mu * D(v) (zero frames of dx_v.m / dz_v.m), Born terms.
-// The structure-kernel product K_mu -= fw_nth * (D u . D u_fwd) runs as k_kmu after the steps that
-// take it. The frequency-domain work is blocked over time: DFT samples and synthesised source slices
-// are staged as FP32 slabs and contracted with FP64 phase tables in separate launches.
+// The structure-kernel product K_mu -= fw_nth * (D u . D u_fwd) of the steps that take it is a fourth
+// phase of the same launch (k_kmu after the second step of a two-step launch). The frequency-domain
+// work is blocked over time: DFT samples and synthesised source slices are staged as FP32 slabs and
+// contracted with the phase tables on the matrix cores in separate launches (spectral.hip).
 #include "engine.h"
 
 namespace fd2d {
@@ -18,14 +19,14 @@ namespace fd2d {
 // ---------------------------------------------------------------------------------------------
 // Fused time step
 // ---------------------------------------------------------------------------------------------
-// One workgroup (4 wavefronts) advances one kTX x kTileZ tile by one step:
+// One workgroup (kTileThreads / 64 wavefronts) advances one kTX x kTileZ tile by one step:
 //   phase 1 issues, in one burst, the loads of sigma over the tile plus the halo that the velocity
 //           stencil of the halo needs and the velocity operands, then stores sigma to LDS;
 //   phase 2 computes the velocity (and displacement) on the tile plus an (R-1, R) halo in both
 //           directions -- the halo is recomputed, bit-identically, by the neighbouring tiles --
 //           and writes the outputs of the tile's own nodes;
 //   phase 3 updates sigma on the tile.
-// Lane l of every wavefront owns column i0 + l; wavefront w owns rows w, w+4, ... of each region,
+// Lane l of every wavefront owns column i0 + l; wavefront w owns rows w, w+TW, ... of each region,
 // so row offsets are wave-uniform and LDS row offsets fold into instruction immediates. The few
 // halo columns (2R-1 of the velocity region, 2(2R-1) of the sigma_xy region) are handled as one
 // packed list.
@@ -429,7 +430,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_group(GroupArgs G)
 //   SX:      columns -LS .. 63+LS, rows as VV
 //   SZ:      columns as VV,        rows -LS .. TZ-1+LS
 // Per-thread layout as in k_tile: lane l owns column i0 + l of every region's main block, wave w
-// the rows w, w+4, ...; halo columns are packed lists over the 256 threads.
+// the rows w, w+PW, ...; halo columns are packed lists over the kPairThreads threads.
 template <int R, int MODE>
 __device__ __forceinline__ void pair_step(const StepArgs& a, const PairStep& ps, int t)
 {
