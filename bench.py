@@ -60,8 +60,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: up to 16 host cores)")
     ap.add_argument("--cpu-nt", type=int, default=16, help="nt of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--timing-every", type=int, default=7,
-                    help="time every n-th launch (7: coprime with fw_nth, so kernel steps enter in proportion)")
+    ap.add_argument("--timing-every", type=int, default=31,
+                    help="time every n-th launch (31: coprime with fw_nth, so kernel steps enter in proportion; "
+                         "each timed launch adds ~6 us of event overhead to the run)")
     ap.add_argument("--steps-per-launch", type=int, choices=[0, 1, 2], default=0,
                     help="time steps per launch of the time loop (2: temporal blocking; 0: tuned per grid "
                          "shape by the first solve of that shape)")
