@@ -105,6 +105,32 @@ def test_adjoint_structure_gradient_pair(orc, fw_nth):
     assert np.all(s2[:, c.nt:] == 0)
 
 
+def test_many_frequencies(orc):
+    """181 frequencies (0.02:0.001:0.2): 2K = 362 phase-table columns padded to 512, so the DFT
+    flushes run two column groups and the synthesis contracts over 32 chunks of spectrum rows.
+    Green spectrum, correlation (synthesised source), adjoint mode 1 (stf_fft) and mode 0 (source
+    from stf_fft) against the oracle."""
+    from fd2d_noise_amd.config import matlab_colon
+    c = tiny(f_sample=matlab_colon(0.02, 0.001, 0.2), fw_nth=2)
+    assert c.n_sample == 181
+    rng = np.random.default_rng(450)
+    mu, rho, src, rec, G0, Gout0, spectrum, sd, C0, stf = _pipeline_inputs(orc, c, rng)
+    G, Gout, _ = fd.run_forward1_green(mu, rho, src, rec, 1, config=c)
+    s, C = fd.run_forward2_correlation(mu, rho, G0, spectrum, sd, rec, 1, config=c)
+    s0, _ = orc.run_forward2_correlation(mu, rho, G0, spectrum, sd, rec, 1, config=c)
+    K1, a1, _, _ = fd.run_noise_adjoint(mu, rho, C0, stf, rec, rec, None, None, None, 1, config=c)
+    K10, a10, _, _ = orc.run_noise_adjoint(mu, rho, C0, stf, rec, rec, None, None, None, 1, config=c)
+    K2, _, _, s2 = fd.run_noise_adjoint(mu, rho, Gout0, a10, rec, rec, spectrum, sd, None, 0, config=c)
+    K20, _, _, s20 = orc.run_noise_adjoint(mu, rho, Gout0, a10, rec, rec, spectrum, sd, None, 0, config=c)
+    e = {"G_fft": rel_l2(G, G0), "correlation": rel_l2(s, s0), "C": rel_l2(C, C0), "stf_fft": rel_l2(a1, a10),
+         "K1": rel_l2(K1[:, :, -1], K10[:, :, -1]), "K2": rel_l2(K2[:, :, -1], K20[:, :, -1]),
+         "adjoint seis": rel_l2(s2, s20)}
+    print("181 frequencies: " + " ".join(f"{k} {v:.2e}" for k, v in e.items()))
+    assert e["G_fft"] < TOL_FIELD and e["C"] < TOL_FIELD and e["stf_fft"] < TOL_FIELD
+    assert e["correlation"] < TOL_CORR and e["adjoint seis"] < TOL_CORR
+    assert e["K1"] < TOL_KERNEL and e["K2"] < TOL_KERNEL
+
+
 @pytest.mark.parametrize("nb", [0, 4])
 def test_adjoint_source_kernel(orc, nb):
     """Source kernel with point adjoint sources (eval_objective_and_gradient.m:91, mode 0)."""
