@@ -34,6 +34,16 @@ def load() -> ctypes.CDLL:
             f = getattr(lib, name)
             f.argtypes = at
             f.restype = ctypes.c_int
+        vp = ctypes.c_void_p
+        for name, at in (("fd2d_resident_upload", [abi.PCArray, ctypes.c_int32, ctypes.POINTER(vp)]),
+                         ("fd2d_resident_alloc", [ctypes.POINTER(ctypes.c_int64 * 3), ctypes.c_int32,
+                                                  ctypes.POINTER(vp)]),
+                         ("fd2d_resident_view", [vp, abi.PCArray, abi.PMArray]),
+                         ("fd2d_resident_download", [vp, abi.PMArray, vp]),
+                         ("fd2d_resident_free", [vp])):
+            f = getattr(lib, name)
+            f.argtypes = at
+            f.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -58,6 +68,8 @@ EXPORTS = (
     "fd2d_engine_create", "fd2d_engine_submit", "fd2d_engine_advance", "fd2d_engine_done", "fd2d_engine_destroy",
     "fd2d_run_batch_device", "fd2d_solve_device_bytes", "fd2d_release_cached_memory",
     "fd2d_make_adjoint_sources_queued",
+    "fd2d_resident_upload", "fd2d_resident_alloc", "fd2d_resident_view", "fd2d_resident_download",
+    "fd2d_resident_free",
 )
 
 

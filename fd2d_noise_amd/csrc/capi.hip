@@ -1875,6 +1875,49 @@ struct fd2d_engine {
     explicit fd2d_engine(hipStream_t s) : e(s) {}
 };
 
+// A device-resident array owned by the caller through its handle.
+struct fd2d_resident {
+    void* p = nullptr;
+    int64_t dims[3] = {0, 0, 0};
+    int32_t elem_bytes = 0;
+    size_t bytes = 0;
+    ~fd2d_resident()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+namespace {
+// validated before any device call, so argument errors are reported without a GPU
+size_t resident_bytes(const int64_t dims[3], int32_t eb)
+{
+    if (eb != 4 && eb != 8 && eb != 16) raise(FD2D_EINVAL, "elem_bytes must be 4, 8 or 16, not %d", eb);
+    double n = eb;
+    for (int q = 0; q < 3; ++q) {
+        if (dims[q] < 0) raise(FD2D_EINVAL, "negative dimension");
+        n *= (double)dims[q];
+    }
+    if (n > 4.0e18) raise(FD2D_EINVAL, "array too large");
+    return (size_t)dims[0] * (size_t)dims[1] * (size_t)dims[2] * (size_t)eb;
+}
+
+std::unique_ptr<fd2d_resident> make_resident(const int64_t dims[3], int32_t eb)
+{
+    const size_t b = resident_bytes(dims, eb);
+    require_device();
+    std::unique_ptr<fd2d_resident> r(new fd2d_resident);
+    for (int q = 0; q < 3; ++q) r->dims[q] = dims[q];
+    r->elem_bytes = eb;
+    r->bytes = b;
+    if (b && hipMalloc(&r->p, b) != hipSuccess) {
+        (void)hipGetLastError();
+        r->p = nullptr;
+        raise(FD2D_ENOMEM, "device allocation of a %zu-byte resident array failed", b);
+    }
+    return r;
+}
+}  // namespace
+
 namespace {
 int adjoint_sources_entry(const double* u, const double* u0, const double* du, const double* t, const double* t_host,
                           int64_t nr, int64_t nt, int32_t veldis, int32_t measurement, int32_t deriv_order,
@@ -1926,6 +1969,65 @@ int fd2d_set_steps_per_launch(int32_t steps)
     }
     g_steps_per_launch.store(steps);
     return FD2D_OK;
+}
+
+int fd2d_resident_upload(const fd2d_carray* host, int32_t elem_bytes, fd2d_resident** out)
+{
+    return guarded([&] {
+        if (!host || !out) raise(FD2D_EINVAL, "host and out are required");
+        const size_t b = resident_bytes(host->dims, elem_bytes);
+        if (b && !host->data) raise(FD2D_EINVAL, "host data is NULL");
+        std::unique_ptr<fd2d_resident> r = make_resident(host->dims, elem_bytes);
+        if (b) HIPCHK(hipMemcpy(r->p, host->data, b, hipMemcpyHostToDevice));
+        *out = r.release();
+    });
+}
+
+int fd2d_resident_alloc(const int64_t dims[3], int32_t elem_bytes, fd2d_resident** out)
+{
+    return guarded([&] {
+        if (!dims || !out) raise(FD2D_EINVAL, "dims and out are required");
+        std::unique_ptr<fd2d_resident> r = make_resident(dims, elem_bytes);
+        if (r->bytes) HIPCHK(hipMemset(r->p, 0, r->bytes));
+        *out = r.release();
+    });
+}
+
+int fd2d_resident_view(const fd2d_resident* r, fd2d_carray* in, fd2d_marray* out)
+{
+    return guarded([&] {
+        if (!r) raise(FD2D_EINVAL, "resident array is NULL");
+        if (in) {
+            in->data = r->p;
+            for (int q = 0; q < 3; ++q) in->dims[q] = r->dims[q];
+        }
+        if (out) {
+            out->data = r->p;
+            for (int q = 0; q < 3; ++q) out->dims[q] = r->dims[q];
+        }
+    });
+}
+
+int fd2d_resident_download(const fd2d_resident* r, fd2d_marray* host, void* stream)
+{
+    return guarded([&] {
+        if (!r || !host) raise(FD2D_EINVAL, "resident array and host are required");
+        const size_t b = resident_bytes(host->dims, r->elem_bytes);
+        if (b != r->bytes) raise(FD2D_EINVAL, "host array holds %zu bytes, the resident array %zu", b, r->bytes);
+        if (!b) return;
+        if (!host->data) raise(FD2D_EINVAL, "host data is NULL");
+        HIPCHK(hipStreamSynchronize(as_stream(stream)));
+        HIPCHK(hipMemcpy(host->data, r->p, b, hipMemcpyDeviceToHost));
+    });
+}
+
+int fd2d_resident_free(fd2d_resident* r)
+{
+    return guarded([&] {
+        if (!r) return;
+        HIPCHK(hipDeviceSynchronize());
+        delete r;
+    });
 }
 
 int fd2d_release_cached_memory(void)

@@ -348,6 +348,30 @@ int fd2d_run_batch_device(const fd2d_solve* solves, int32_t n, void* stream);
 int fd2d_solve_device_bytes(const fd2d_solve* solve, int64_t* bytes);
 
 /*
+ * Device-resident arrays: the model and Green-spectrum handles of a cache or serving tier. A handle
+ * owns device memory (on the device current at creation) holding one array in the C ABI's layout
+ * (column-major, complex interleaved), so a caller without device pointers of its own -- a MEX
+ * shim, a service in another language -- keeps mu and rho, Green spectra (G_fft, the input the
+ * correlation and adjoint runs share) and snapshots on the GPU between calls, and passes them to
+ * the *_device entry points and fd2d_solve through fd2d_resident_view. MATLAB moves every array
+ * through the MEX boundary by value, so these have no counterpart in the reference; they replace
+ * the PCIe round trip of a multi-GB G_fft between the Green and correlation runs.
+ *   elem_bytes   4 (single), 8 (double) or 16 (complex double); any other value is FD2D_EINVAL
+ */
+typedef struct fd2d_resident fd2d_resident;
+/* A resident copy of host array `host` (copied before the call returns). */
+int fd2d_resident_upload(const fd2d_carray* host, int32_t elem_bytes, fd2d_resident** out);
+/* A resident array of the given dims, zero-filled: an output of the *_device entry points. */
+int fd2d_resident_alloc(const int64_t dims[3], int32_t elem_bytes, fd2d_resident** out);
+/* Device descriptors of the array: `in` (may be NULL) for inputs, `out` (may be NULL) for outputs. */
+int fd2d_resident_view(const fd2d_resident* r, fd2d_carray* in, fd2d_marray* out);
+/* Copies the array into host memory after the work queued on `stream` (NULL: the null stream) has
+ * finished; host->dims must have as many elements as the array. */
+int fd2d_resident_download(const fd2d_resident* r, fd2d_marray* host, void* stream);
+/* Frees the handle and its device memory (waits for the device first). NULL is a no-op. */
+int fd2d_resident_free(fd2d_resident* r);
+
+/*
  * Device memory of the solvers comes from a per-device cache: released buffers are kept (up to a
  * quarter of the device memory) and handed out again to later solves, which otherwise pay a
  * hipMalloc / hipFree of multi-GB buffers per solve. The cached memory is not available to other

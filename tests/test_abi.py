@@ -113,3 +113,23 @@ def test_solve_device_bytes():
         assert st == abi.FD2D_OK and bc > b + 16 * P * c.n_sample
     s, _ = _solve_struct(small, 7)
     assert _bytes(s)[0] == abi.FD2D_EINVAL and "solver" in _lib.last_error()
+
+
+def test_resident_arguments_are_checked_before_the_device():
+    """fd2d_resident_*: element sizes other than 4/8/16, negative dims, NULL handles and host arrays
+    of the wrong size are FD2D_EINVAL; freeing NULL is a no-op (no device needed for any of it)."""
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    dims = (ctypes.c_int64 * 3)(2, 3, 1)
+    assert lib.fd2d_resident_alloc(ctypes.byref(dims), 3, ctypes.byref(h)) == abi.FD2D_EINVAL
+    assert "elem_bytes" in _lib.last_error()
+    neg = (ctypes.c_int64 * 3)(2, -3, 1)
+    assert lib.fd2d_resident_alloc(ctypes.byref(neg), 8, ctypes.byref(h)) == abi.FD2D_EINVAL
+    d = abi.CArray()
+    d.data = None
+    d.dims[:] = (4, 1, 1)
+    assert lib.fd2d_resident_upload(ctypes.byref(d), 8, ctypes.byref(h)) == abi.FD2D_EINVAL
+    assert "NULL" in _lib.last_error()
+    assert lib.fd2d_resident_view(None, None, None) == abi.FD2D_EINVAL
+    assert lib.fd2d_resident_download(None, None, None) == abi.FD2D_EINVAL
+    assert lib.fd2d_resident_free(None) == abi.FD2D_OK
