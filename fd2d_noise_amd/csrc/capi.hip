@@ -651,34 +651,134 @@ struct SplitTable {
     void set(size_t r, size_t c, double v) { split_bf16((float)v, h[r * cols + c], m[r * cols + c], l[r * cols + c]); }
 };
 
+// Split phase tables on the device, cached per device: the solves of one configuration (the
+// stations of an inversion, the requests of a served model) share a table instead of recomputing
+// its sines and cosines and uploading it again, which costs milliseconds of host time per solve.
+// An entry is keyed by everything the table is computed from; it is uploaded on the stream of the
+// solve that first needs it, and later users wait for that upload through its event. Entries no
+// solve holds are freed least recently used first once the cache exceeds kTableCacheBytes.
+struct DeviceTable {
+    unsigned short* part[3] = {nullptr, nullptr, nullptr};
+    std::vector<unsigned short> host[3];  // staging of the asynchronous upload
+    size_t n = 0;                         // elements per part
+    hipEvent_t ready = nullptr;
+    ~DeviceTable()
+    {
+        for (unsigned short* q : part)
+            if (q) (void)hipFree(q);
+        if (ready) (void)hipEventDestroy(ready);
+    }
+};
+
+constexpr size_t kTableCacheBytes = size_t(256) << 20;
+std::mutex g_table_mu;
+struct TableCache {
+    std::vector<std::pair<std::string, std::shared_ptr<DeviceTable>>> lru;  // most recent last
+    size_t bytes = 0;
+};
+// never destroyed: at process exit the HIP runtime may already be gone, and the memory goes with it
+TableCache* const g_tables = new TableCache[64];
+
+template <class T>
+void key_add(std::string& k, const T* v, size_t n)
+{
+    k.append(reinterpret_cast<const char*>(v), n * sizeof(T));
+}
+
+// The table for `key`, built by `build` (which fills a SplitTable) when it is not cached.
+template <class F>
+std::shared_ptr<const DeviceTable> cached_table(const std::string& key, hipStream_t st, F&& build)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    TableCache& c = g_tables[dev & 63];
+    std::lock_guard<std::mutex> lk(g_table_mu);
+    for (size_t q = 0; q < c.lru.size(); ++q)
+        if (c.lru[q].first == key) {
+            std::shared_ptr<DeviceTable> t = c.lru[q].second;
+            c.lru.erase(c.lru.begin() + (long)q);
+            c.lru.emplace_back(key, t);
+            HIPCHK(hipStreamWaitEvent(st, t->ready, 0));
+            return t;
+        }
+    SplitTable e;
+    build(e);
+    auto t = std::make_shared<DeviceTable>();
+    t->n = e.h.size();
+    t->host[0] = std::move(e.h);
+    t->host[1] = std::move(e.m);
+    t->host[2] = std::move(e.l);
+    for (int q = 0; q < 3; ++q) {
+        if (hipMalloc(&t->part[q], std::max<size_t>(t->n, 1) * sizeof(unsigned short)) != hipSuccess) {
+            (void)hipGetLastError();
+            t->part[q] = nullptr;
+            raise(FD2D_ENOMEM, "device allocation of a phase table failed");
+        }
+        if (t->n)
+            HIPCHK(hipMemcpyAsync(t->part[q], t->host[q].data(), t->n * sizeof(unsigned short), hipMemcpyHostToDevice,
+                                  st));
+    }
+    HIPCHK(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(t->ready, st));
+    c.lru.emplace_back(key, t);
+    c.bytes += 3 * t->n * sizeof(unsigned short);
+    for (size_t q = 0; c.bytes > kTableCacheBytes && q + 1 < c.lru.size();) {
+        if (c.lru[q].second.use_count() == 1) {  // no solve holds it, so no queued work reads it
+            c.bytes -= 3 * c.lru[q].second->n * sizeof(unsigned short);
+            c.lru.erase(c.lru.begin() + (long)q);
+        } else {
+            ++q;
+        }
+    }
+    return t;
+}
+
+// Key of the phase table e^{sign i w_k t_j} * scale over the times ts, in layout `kind` with slab
+// length T and C_pad columns.
+std::string table_key(char kind, const fd2d_config* cfg, const std::vector<double>& ts, double sign, double scale,
+                      int T, int C_pad)
+{
+    std::string k(1, kind);
+    const int32_t ints[4] = {T, C_pad, cfg->n_sample, (int32_t)ts.size()};
+    const double dbl[2] = {sign, scale};
+    key_add(k, ints, 4);
+    key_add(k, dbl, 2);
+    key_add(k, ts.data(), ts.size());
+    key_add(k, cfg->w_sample, (size_t)cfg->n_sample);
+    return k;
+}
+
 struct DftAccumulator {
     double2* acc = nullptr;
     size_t P = 0;
     int K = 0, C_pad = 0, Jt = 0;
     DBuf<float> slab;
-    DBuf<unsigned short> eh, em, el;  // [C_pad][Jt]: parts of (Re, Im) of each sample's phase factor
+    // [C_pad][Jt]: parts of (Re, Im) of each sample's phase factor
+    std::shared_ptr<const DeviceTable> tab;
     int cnt = 0, done = 0, T = 0;
     hipStream_t st = nullptr;
-    void init(double2* acc_, size_t P_, int K_, const std::vector<double2>& table, size_t n_samples, hipStream_t s)
+    // phase factors e^{sign i w_k t_j} * scale at the sample times ts
+    void init(double2* acc_, size_t P_, const fd2d_config* cfg, const std::vector<double>& ts, double sign,
+              double scale, hipStream_t s)
     {
         acc = acc_;
         P = P_;
-        K = K_;
+        K = cfg->n_sample;
         st = s;
         C_pad = spec_cols(K);
+        const size_t n_samples = ts.size();
         T = slab_len(P, n_samples);  // one slab, or a multiple of 16 samples
         Jt = (int)((n_samples + 15) / 16 * 16);
         slab.alloc((size_t)T * P);
-        SplitTable e;
-        e.resize(C_pad, Jt);
-        for (size_t j = 0; j < n_samples; ++j)
-            for (int k = 0; k < K; ++k) {
-                e.set(2 * k, j, table[j * K + k].x);
-                e.set(2 * k + 1, j, table[j * K + k].y);
-            }
-        upload(eh, e.h, st);
-        upload(em, e.m, st);
-        upload(el, e.l, st);
+        tab = cached_table(table_key('D', cfg, ts, sign, scale, T, C_pad), st, [&](SplitTable& e) {
+            const std::vector<double2> table = phase_table(cfg, ts, sign, scale);
+            e.resize(C_pad, Jt);
+            for (size_t j = 0; j < n_samples; ++j)
+                for (int k = 0; k < K; ++k) {
+                    e.set(2 * k, j, table[j * K + k].x);
+                    e.set(2 * k + 1, j, table[j * K + k].y);
+                }
+        });
         HIPCHK(hipMemsetAsync(acc, 0, P * (size_t)K * sizeof(double2), st));
     }
     float* next_slot() { return slab.p + (size_t)cnt * P; }
@@ -689,7 +789,8 @@ struct DftAccumulator {
     void flush()
     {
         if (cnt == 0) return;
-        launch_dft_bf16(acc, slab.p, eh.p + done, em.p + done, el.p + done, P, K, C_pad, Jt, cnt, st);
+        launch_dft_bf16(acc, slab.p, tab->part[0] + done, tab->part[1] + done, tab->part[2] + done, P, K, C_pad, Jt, cnt,
+                        st);
         done += cnt;
         cnt = 0;
     }
@@ -700,13 +801,17 @@ struct Synthesiser {
     int K = 0, C_pad = 0, T = 0, J_pad = 0, n_total = 0;
     DBuf<float> slab;
     DBuf<float> a32;  // [C_pad][P]: (Re, Im) rows of the source spectrum, FP32
-    DBuf<unsigned short> th, tm, tl;  // per slab [J_pad][C_pad]: parts of (Re, -Im) of the phase factors
+    // per slab [J_pad][C_pad]: parts of (Re, -Im) of the phase factors
+    std::shared_ptr<const DeviceTable> tab;
     hipStream_t st = nullptr;
-    void init(const double2* A, size_t P_, int K_, const std::vector<double2>& table, int n_samples, hipStream_t s)
+    // phase factors e^{i w_k t_j} * scale at the slice times ts
+    void init(const double2* A, size_t P_, const fd2d_config* cfg, const std::vector<double>& ts, double scale,
+              hipStream_t s)
     {
         P = P_;
-        K = K_;
+        K = cfg->n_sample;
         st = s;
+        const int n_samples = (int)ts.size();
         n_total = n_samples;
         C_pad = spec_cols(K);
         T = slab_len(P, (size_t)std::max(n_samples, 1));
@@ -714,19 +819,18 @@ struct Synthesiser {
         slab.alloc((size_t)T * P);
         a32.alloc((size_t)C_pad * P);
         launch_spectrum_f32(a32.p, A, P, K, C_pad, st);
-        const int n_slabs = (std::max(n_samples, 1) + T - 1) / T;
-        SplitTable e;
-        e.resize((size_t)n_slabs * J_pad, C_pad);
-        for (int idx = 0; idx < n_samples; ++idx) {
-            const size_t row = (size_t)(idx / T) * J_pad + (size_t)(idx % T);
-            for (int k = 0; k < K; ++k) {
-                e.set(row, 2 * k, table[(size_t)idx * K + k].x);
-                e.set(row, 2 * k + 1, -table[(size_t)idx * K + k].y);
+        tab = cached_table(table_key('S', cfg, ts, 1.0, scale, T, C_pad), st, [&](SplitTable& e) {
+            const std::vector<double2> table = phase_table(cfg, ts, 1.0, scale);
+            const int n_slabs = (std::max(n_samples, 1) + T - 1) / T;
+            e.resize((size_t)n_slabs * J_pad, C_pad);
+            for (int idx = 0; idx < n_samples; ++idx) {
+                const size_t row = (size_t)(idx / T) * J_pad + (size_t)(idx % T);
+                for (int k = 0; k < K; ++k) {
+                    e.set(row, 2 * k, table[(size_t)idx * K + k].x);
+                    e.set(row, 2 * k + 1, -table[(size_t)idx * K + k].y);
+                }
             }
-        }
-        upload(th, e.h, st);
-        upload(tm, e.m, st);
-        upload(tl, e.l, st);
+        });
     }
     // Source slice of the idx-th sampled step (0-based); synthesises a new block when needed.
     const float* slice(int idx)
@@ -734,7 +838,8 @@ struct Synthesiser {
         const int slot = idx % T;
         if (slot == 0) {
             const size_t o = (size_t)(idx / T) * J_pad * C_pad;
-            launch_synth_bf16(slab.p, a32.p, th.p + o, tm.p + o, tl.p + o, P, C_pad, std::min(T, n_total - idx), st);
+            launch_synth_bf16(slab.p, a32.p, tab->part[0] + o, tab->part[1] + o, tab->part[2] + o, P, C_pad,
+                              std::min(T, n_total - idx), st);
         }
         return slab.p + (size_t)slot * P;
     }
@@ -1019,7 +1124,7 @@ struct GreenJob : Job {
         std::vector<double> ts;
         for (int n = 1; n <= g.nt; ++n)
             if ((n + g.nt - 1) % g.s == 0) ts.push_back((n - 1) * g.dt);
-        if (Gf) dft.init(Gf, g.P, g.K, phase_table(cfg, ts, -1.0, 1.0), ts.size(), st);
+        if (Gf) dft.init(Gf, g.P, cfg, ts, -1.0, 1.0, st);
     }
     bool sampled(int n) const { return Gf != nullptr && (n + g.nt - 1) % g.s == 0; }
     void plan(int n, int sub) override
@@ -1120,7 +1225,7 @@ struct CorrelationJob : Job {
         const int n_ftc = N / g.s;
         std::vector<double> ts;
         for (int q = 1; q <= n_ftc; ++q) ts.push_back((double)(q * g.s - g.nt) * g.dt);
-        syn.init(A.p, g.P, g.K, phase_table(cfg, ts, 1.0, kInvSqrt2Pi * g.dw), n_ftc, st);
+        syn.init(A.p, g.P, cfg, ts, kInvSqrt2Pi * g.dw, st);
         if (born) {
             dmu_f.alloc(g.P);
             launch_scale_to_float((const double*)dmu->data, dmu_f.p, g.P, 1.0, st);
@@ -1266,7 +1371,7 @@ struct AdjointJob : Job {
             const int n_syn = nsteps / g.s;
             std::vector<double> ts;
             for (int q = 1; q <= n_syn; ++q) ts.push_back((double)(q * g.s - g.nt) * g.dt);
-            syn.init(A.p, g.P, g.K, phase_table(cfg, ts, 1.0, kInvSqrt2Pi * g.dw), n_syn, st);
+            syn.init(A.p, g.P, cfg, ts, kInvSqrt2Pi * g.dw, st);
         }
         // DFT of u at the sampled steps: stf_fft (:335-346) and the source kernel (:302-320) share it
         need_dft = (sfft != nullptr) || (src_kernel && Kd);
@@ -1278,7 +1383,7 @@ struct AdjointJob : Job {
             }
             std::vector<double> ts;
             for (int n = g.s; n <= nsteps; n += g.s) ts.push_back((double)(n - g.nt) * g.dt);
-            dft.init(accp, g.P, g.K, phase_table(cfg, ts, -1.0, 1.0), ts.size(), st);
+            dft.init(accp, g.P, cfg, ts, -1.0, 1.0, st);
         }
         if (born) {
             dmu_f.alloc(g.P);
@@ -2035,6 +2140,18 @@ int fd2d_release_cached_memory(void)
     return guarded([&] {
         HIPCHK(hipDeviceSynchronize());
         for (BlockCache& c : g_cache) c.clear();
+        std::lock_guard<std::mutex> lk(g_table_mu);
+        for (int d = 0; d < 64; ++d) {  // phase tables that no solve holds
+            TableCache& c = g_tables[d];
+            for (size_t q = 0; q < c.lru.size();) {
+                if (c.lru[q].second.use_count() == 1) {
+                    c.bytes -= 3 * c.lru[q].second->n * sizeof(unsigned short);
+                    c.lru.erase(c.lru.begin() + (long)q);
+                } else {
+                    ++q;
+                }
+            }
+        }
     });
 }
 

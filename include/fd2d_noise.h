@@ -374,9 +374,11 @@ int fd2d_resident_free(fd2d_resident* r);
 /*
  * Device memory of the solvers comes from a per-device cache: released buffers are kept (up to a
  * quarter of the device memory) and handed out again to later solves, which otherwise pay a
- * hipMalloc / hipFree of multi-GB buffers per solve. The cached memory is not available to other
- * allocators of the process (e.g. PyTorch's); this call waits for the current device and frees every
- * cached block. Memory management; it has no counterpart in the reference.
+ * hipMalloc / hipFree of multi-GB buffers per solve. The split phase tables of the frequency-domain
+ * work are cached too (up to 256 MB per device), so solves of one configuration compute and upload
+ * them once. The cached memory is not available to other allocators of the process (e.g.
+ * PyTorch's); this call waits for the current device and frees every cached block and every phase
+ * table no solve holds. Memory management; it has no counterpart in the reference.
  */
 int fd2d_release_cached_memory(void);
 
