@@ -329,9 +329,10 @@ def serving_bench(args):
             out.append((off, S.Request(kind, name, src, rec, observed=obs, dmu=dmu)))
         return out
 
-    # warm-up: one request of each kind on every model size class
+    # warm-up, as a server does when it loads its models: one request of each kind on every model
+    # (first-use allocations, launch-shape tuning and phase tables of each grid stay out of the runs)
     warm = requests([(0.0, k, n, models[n]["stations"][:1], models[n]["stations"][1:4], 1)
-                     for n in sorted(models)[:2] for k in ("correlation", "kernel", "hvp")])
+                     for n in sorted(models) for k in ("correlation", "kernel", "hvp")])
     srv.serve(warm)
     srv.done.clear()
     runs = []
