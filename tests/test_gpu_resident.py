@@ -80,3 +80,27 @@ def test_resident_round_trip_and_size_check():
         assert "bytes" in _lib.last_error()
     with Resident.alloc((4, 4), np.float32) as r:
         assert r.download().dtype == np.float32 and not r.download().any()
+
+
+def test_cached_tables_and_memory_release():
+    """Solves of one configuration share cached phase tables; releasing the cached memory between
+    two identical correlation runs (tables, buffers) leaves the second result unchanged, and a run
+    with other frequencies (its own tables) still matches its first run."""
+    from fd2d_noise_amd.config import matlab_colon
+    c = fd.Config(order=4, fw_nth=1)
+    c2 = fd.Config(order=4, fw_nth=1, f_sample=matlab_colon(0.03, 0.002, 0.15))
+    rng = np.random.default_rng(902)
+    mu, rho = hetero_model(rng, c)
+    src = interior_points(rng, c, 1, 0.4)
+    rec = interior_points(rng, c, 3)
+    sd = blob_distribution(rng, c)
+    out = []
+    for cfg in (c, c2, c, c2):
+        spectrum = np.exp(-((np.asarray(cfg.f_sample) - 0.1) ** 2) / 0.05 ** 2)[:, None]
+        G, _, _ = fd.run_forward1_green(mu, rho, src, rec, 0, config=cfg)
+        corr, _ = fd.run_forward2_correlation(mu, rho, G, spectrum, sd, rec, 0, config=cfg)
+        out.append((G, corr))
+        assert _lib.load().fd2d_release_cached_memory() == abi.FD2D_OK
+    for a, b in ((out[0], out[2]), (out[1], out[3])):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert out[0][0].shape[2] != out[1][0].shape[2]
