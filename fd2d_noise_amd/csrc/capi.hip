@@ -804,21 +804,24 @@ struct Synthesiser {
     // per slab [J_pad][C_pad]: parts of (Re, -Im) of the phase factors
     std::shared_ptr<const DeviceTable> tab;
     hipStream_t st = nullptr;
-    // phase factors e^{i w_k t_j} * scale at the slice times ts
-    void init(const double2* A, size_t P_, const fd2d_config* cfg, const std::vector<double>& ts, double scale,
-              hipStream_t s)
+    // the FP32 spectrum rows, for the caller to fill (launch_source_spectrum) before init
+    float* spectrum_rows(size_t P_, int K_)
     {
         P = P_;
-        K = cfg->n_sample;
+        K = K_;
+        C_pad = spec_cols(K);
+        a32.alloc((size_t)C_pad * P);
+        return a32.p;
+    }
+    // phase factors e^{i w_k t_j} * scale at the slice times ts
+    void init(const fd2d_config* cfg, const std::vector<double>& ts, double scale, hipStream_t s)
+    {
         st = s;
         const int n_samples = (int)ts.size();
         n_total = n_samples;
-        C_pad = spec_cols(K);
         T = slab_len(P, (size_t)std::max(n_samples, 1));
         J_pad = (T + 63) / 64 * 64;
         slab.alloc((size_t)T * P);
-        a32.alloc((size_t)C_pad * P);
-        launch_spectrum_f32(a32.p, A, P, K, C_pad, st);
         tab = cached_table(table_key('S', cfg, ts, 1.0, scale, T, C_pad), st, [&](SplitTable& e) {
             const std::vector<double2> table = phase_table(cfg, ts, 1.0, scale);
             const int n_slabs = (std::max(n_samples, 1) + T - 1) / T;
@@ -1167,7 +1170,6 @@ struct CorrelationJob : Job {
     float* Cout = nullptr;
     double* sg = nullptr;
     DBuf<double> I_d, ss_d;
-    DBuf<double2> A;
     Synthesiser syn;
     DBuf<float> dmu_f;
 
@@ -1218,14 +1220,13 @@ struct CorrelationJob : Job {
             for (int64_t q = 0; q < n_src; ++q) spec_sum[k] += spec[(size_t)k + (size_t)g.K * q];
         upload(I_d, interpol_matrix(g.K, nb), st);
         upload(ss_d, spec_sum, st);
-        A.alloc(g.P * (size_t)g.K);
-        launch_source_spectrum(A.p, (const double2*)G->data, 0, (const double*)sd->data, nb_cols, I_d.p, ss_d.p, g.P,
-                               g.K, st);
+        launch_source_spectrum(syn.spectrum_rows(g.P, g.K), (const double2*)G->data, 0, (const double*)sd->data,
+                               nb_cols, I_d.p, ss_d.p, g.P, g.K, syn.C_pad, st);
         // ifft_coeff of :73-85 at t(n) = (n - nt) dt for n = s, 2s, ...
         const int n_ftc = N / g.s;
         std::vector<double> ts;
         for (int q = 1; q <= n_ftc; ++q) ts.push_back((double)(q * g.s - g.nt) * g.dt);
-        syn.init(A.p, g.P, cfg, ts, kInvSqrt2Pi * g.dw, st);
+        syn.init(cfg, ts, kInvSqrt2Pi * g.dw, st);
         if (born) {
             dmu_f.alloc(g.P);
             launch_scale_to_float((const double*)dmu->data, dmu_f.p, g.P, 1.0, st);
@@ -1267,7 +1268,7 @@ struct AdjointJob : Job {
     double* sg = nullptr;
     DBuf<float> amp;
     DBuf<double> I_d, ss_d, sp_d;
-    DBuf<double2> A, acc_tmp;
+    DBuf<double2> acc_tmp;
     Synthesiser syn;
     DftAccumulator dft;
     DBuf<float> dmu_f;
@@ -1365,13 +1366,12 @@ struct AdjointJob : Job {
             for (int k = 0; k < g.K; ++k)
                 for (int64_t q = 0; q < n_src; ++q) spec_sum[k] += spec[(size_t)k + (size_t)g.K * q];
             upload(ss_d, spec_sum, st);
-            A.alloc(g.P * (size_t)g.K);
-            launch_source_spectrum(A.p, (const double2*)stf->data, 1, (const double*)sd->data, nb_cols, I_d.p, ss_d.p,
-                                   g.P, g.K, st);
+            launch_source_spectrum(syn.spectrum_rows(g.P, g.K), (const double2*)stf->data, 1, (const double*)sd->data,
+                                   nb_cols, I_d.p, ss_d.p, g.P, g.K, syn.C_pad, st);
             const int n_syn = nsteps / g.s;
             std::vector<double> ts;
             for (int q = 1; q <= n_syn; ++q) ts.push_back((double)(q * g.s - g.nt) * g.dt);
-            syn.init(A.p, g.P, cfg, ts, kInvSqrt2Pi * g.dw, st);
+            syn.init(cfg, ts, kInvSqrt2Pi * g.dw, st);
         }
         // DFT of u at the sampled steps: stf_fft (:335-346) and the source kernel (:302-320) share it
         need_dft = (sfft != nullptr) || (src_kernel && Kd);
@@ -1828,7 +1828,7 @@ int64_t solve_bytes(const fd2d_solve* s)
     auto dft_bytes = [&](int64_t n) { return slab(n) * P * 4 + 3 * ((n + 15) / 16 * 16) * C_pad * 2; };
     auto syn_bytes = [&](int64_t n) {
         const int64_t T = slab(n), J = (T + 63) / 64 * 64, slabs = (std::max<int64_t>(n, 1) + T - 1) / T;
-        return T * P * 4 + C_pad * P * 4 + 3 * slabs * C_pad * J * 2 + P * K * 16;  // slab, a32, tables, A
+        return T * P * 4 + C_pad * P * 4 + 3 * slabs * C_pad * J * 2;  // slab, spectrum rows, tables
     };
     const bool born = numel(s->dmu) != 0;
     if (s->solver == FD2D_GREEN) {

@@ -220,11 +220,10 @@ void launch_synth_bf16(float* out, const float* a, const unsigned short* th, con
                        const unsigned short* tl, size_t P, int C_pad, int cnt, hipStream_t s);
 // the three bfloat16 parts (rounded to nearest, v = h + m + l exactly) of an FP32 value
 void split_bf16(float v, unsigned short& h, unsigned short& m, unsigned short& l);
-// a[2k][x], a[2k+1][x] = Re, Im A[k][x] (FP32), zero rows 2K..C_pad-1
-void launch_spectrum_f32(float* a, const double2* A, size_t P, int K, int C_pad, hipStream_t s);
-// A[k][x] = spec_sum[k] * dist(x,k) * (conj_in ? conj(Z) : Z)[k][x], dist(x,k) = sum_b I[k,b] sd[b][x]
-void launch_source_spectrum(double2* A, const double2* Z, int conj_in, const double* sd, int n_sd,
-                            const double* I, const double* spec_sum, size_t P, int K, hipStream_t s);
+// a32[2k][x], a32[2k+1][x] = Re, Im of A[k][x] = spec_sum[k] * dist(x,k) * (conj_in ? conj(Z) : Z)[k][x]
+// (FP64, rounded to FP32), dist(x,k) = sum_b I[k,b] sd[b][x]; zero rows 2K..C_pad-1
+void launch_source_spectrum(float* a32, const double2* Z, int conj_in, const double* sd, int n_sd,
+                            const double* I, const double* spec_sum, size_t P, int K, int C_pad, hipStream_t s);
 // out[k][x] = scale * (conj ? conj(acc) : acc)
 void launch_finalize_dft(double2* out, const double2* acc, size_t n, double scale, int conj,
                          hipStream_t s);

@@ -975,8 +975,11 @@ void launch_stf_real(float* amp, const double2* stf, int64_t d0, int nq, int nst
 // ---------------------------------------------------------------------------------------------
 // The DFT flush and the synthesis of distributed sources run on the matrix cores (spectral.hip);
 // these kernels prepare their spectra and finish their results.
-__global__ void k_source_spectrum(double2* A, const double2* Z, int conj_in, const double* sd, int n_sd,
-                                  const double* I, const double* spec_sum, size_t P, int K)
+// a[2k][x], a[2k+1][x] = Re, Im of A[k][x] = spec_sum[k] * dist(x,k) * (conj_in ? conj(Z) : Z)[k][x],
+// dist(x,k) = sum_b I[k,b] sd[b][x], formed in FP64 and rounded to FP32 as the synthesis reads them;
+// rows 2K..C_pad-1 are zero.
+__global__ void k_source_spectrum(float* a32, const double2* Z, int conj_in, const double* sd, int n_sd,
+                                  const double* I, const double* spec_sum, size_t P, int K, int C_pad)
 {
     const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= P) return;
@@ -985,15 +988,17 @@ __global__ void k_source_spectrum(double2* A, const double2* Z, int conj_in, con
         for (int b = 0; b < n_sd; ++b) d += I[k + (size_t)K * b] * sd[x + P * b];
         const double a = spec_sum[k] * d;
         const double2 z = Z[(size_t)k * P + x];
-        A[(size_t)k * P + x] = make_double2(a * z.x, conj_in ? -(a * z.y) : a * z.y);
+        a32[(size_t)(2 * k) * P + x] = (float)(a * z.x);
+        a32[(size_t)(2 * k + 1) * P + x] = (float)(conj_in ? -(a * z.y) : a * z.y);
     }
+    for (int r = 2 * K; r < C_pad; ++r) a32[(size_t)r * P + x] = 0.f;
 }
 
-void launch_source_spectrum(double2* A, const double2* Z, int conj_in, const double* sd, int n_sd,
-                            const double* I, const double* spec_sum, size_t P, int K, hipStream_t s)
+void launch_source_spectrum(float* a32, const double2* Z, int conj_in, const double* sd, int n_sd,
+                            const double* I, const double* spec_sum, size_t P, int K, int C_pad, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_source_spectrum, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, A, Z, conj_in, sd,
-                       n_sd, I, spec_sum, P, K);
+    hipLaunchKernelGGL(k_source_spectrum, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, a32, Z, conj_in, sd,
+                       n_sd, I, spec_sum, P, K, C_pad);
 }
 
 __global__ void k_finalize_dft(double2* out, const double2* acc, size_t n, double scale, int conj)

@@ -340,26 +340,5 @@ void launch_synth_bf16(float* out, const float* a, const unsigned short* th, con
     hipLaunchKernelGGL(k_synth_bf16, dim3(nxb, njg), dim3(kSJ), 0, s, out, a, th, tm, tl, (long)P, C_pad, cnt);
 }
 
-// a[2k][x] = Re A[k][x], a[2k+1][x] = Im A[k][x] (FP32), zero rows 2K..C_pad-1 (grid.y = C_pad / 2)
-__global__ void k_spectrum_f32(float* __restrict__ a, const double2* __restrict__ A, long P, int K)
-{
-    const long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
-    if (x >= P) return;
-    if (k < K) {
-        const double2 v = A[(long)k * P + x];
-        a[(2L * k) * P + x] = (float)v.x;
-        a[(2L * k + 1) * P + x] = (float)v.y;
-    } else {
-        a[(2L * k) * P + x] = 0.f;
-        a[(2L * k + 1) * P + x] = 0.f;
-    }
-}
-
-void launch_spectrum_f32(float* a, const double2* A, size_t P, int K, int C_pad, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_spectrum_f32, dim3((unsigned)((P + 255) / 256), (unsigned)(C_pad / 2)), dim3(256), 0, s, a,
-                       A, (long)P, K);
-}
 
 }  // namespace fd2d

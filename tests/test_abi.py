@@ -109,8 +109,10 @@ def test_solve_device_bytes():
         s, _ = _solve_struct(c, abi.FD2D_CORRELATION, rec=(3, 2, 1), G_fft=(c.nx, c.nz, c.n_sample),
                              spectrum=(c.n_sample, 1, 1), source_dist=(c.nx, c.nz, 1))
         st, bc = _bytes(s)
-        # the complex source spectrum alone is 16 bytes per node and frequency
-        assert st == abi.FD2D_OK and bc > b + 16 * P * c.n_sample
+        # on top of the Green run's buffers: the FP32 spectrum rows (2K padded to 256) and the slab
+        # of synthesised slices (up to 512), 4 bytes per node each
+        rows = 256 * -(-2 * c.n_sample // 256)
+        assert st == abi.FD2D_OK and bc > b + 4 * P * (rows + min(512, c.n_corr // c.freq_samp))
     s, _ = _solve_struct(small, 7)
     assert _bytes(s)[0] == abi.FD2D_EINVAL and "solver" in _lib.last_error()
 
