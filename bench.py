@@ -464,7 +464,7 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus N > 1 is not supported "
                                    "(stations shard across GPUs with one all-reduce of the kernels, "
-                                   "which is not part of this build)"}))
+                                   "which this engine does not implement)"}))
         sys.exit(2)
     import torch
     from fd2d_noise_amd import _lib
