@@ -224,3 +224,31 @@ def test_errors_are_reported(orc):
         fd.run_noise_adjoint(mu, rho, None, stf3, None, None, np.ones((121, 1)), np.ones((50, 50)), None, 1,
                              config=tiny())
     assert ei.value.code == abi.FD2D_EUNSUPPORTED
+
+
+@pytest.mark.parametrize("order", [4, 8])
+def test_edge_positions(orc, order):
+    """Sources and receivers on the corners and edges of the grid, outside the domain (nearest node,
+    code/run_forward1_green.m:45-60), on tile borders (x = 63, 64 nodes) and twice on the same node:
+    Green, correlation and adjoint (mode 0, point adjoint sources) against the oracle."""
+    d0 = tiny().Lx / (tiny().nx - 1)  # the tiny set-up's spacing, so its time step stays stable
+    c = tiny(order=order, nx=130, nz=70, Lx=129 * d0, Lz=69 * d0)
+    rng = np.random.default_rng(700 + order)
+    mu, rho = hetero_model(rng, c)
+    dx = c.Lx / (c.nx - 1)
+    src = np.array([[0.0, 0.0], [63 * dx, 31 * dx], [63 * dx, 31 * dx]])
+    rec = np.array([[c.Lx, c.Lz], [-5e4, 0.5 * c.Lz], [64 * dx, 32 * dx], [0.5 * c.Lx, 2 * c.Lz], [64 * dx, 32 * dx]])
+    G, _, s = fd.run_forward1_green(mu, rho, src, rec, 0, config=c)
+    G0, _, s0 = orc.run_forward1_green(mu, rho, src, rec, 0, config=c)
+    spectrum = rng.uniform(0.5, 1.5, (c.n_sample, 1))
+    sd = blob_distribution(rng, c)
+    corr, _ = fd.run_forward2_correlation(mu, rho, G0, spectrum, sd, rec, 0, config=c)
+    corr0, _ = orc.run_forward2_correlation(mu, rho, G0, spectrum, sd, rec, 0, config=c)
+    stf = np.ascontiguousarray(corr0[:, ::-1] / np.abs(corr0).max()).astype(np.complex128)
+    K, _, _, sa = fd.run_noise_adjoint(mu, rho, None, stf, rec, rec, spectrum, None, G0, 0, config=c)
+    K0, _, _, sa0 = orc.run_noise_adjoint(mu, rho, None, stf, rec, rec, spectrum, None, G0, 0, config=c)
+    e = (rel_l2(s, s0), rel_l2(G, G0), rel_l2(corr, corr0), rel_l2(sa, sa0), rel_l2(K[:, :, 0], K0[:, :, 0]))
+    print(f"edge positions order={order}: green {e[0]:.2e} G_fft {e[1]:.2e} correlation {e[2]:.2e} "
+          f"adjoint {e[3]:.2e} K_s {e[4]:.2e}")
+    assert e[0] < TOL_FIELD and e[1] < TOL_FIELD and e[2] < TOL_CORR and e[3] < TOL_CORR and e[4] < TOL_KERNEL
+    assert np.array_equal(s[2], s[4]) and np.array_equal(corr[2], corr[4])  # the same node twice
