@@ -260,75 +260,92 @@ void launch_dft_bf16(double2* acc, const float* samp, const unsigned short* eh, 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Synthesis. Workgroup = 64 nodes x 512 time slices (8 wavefronts); wavefront w owns slices
-// 64w..64w+63 as two 32-row MFMA blocks against the two 32-node blocks (wavefronts past cnt
-// compute on slice 0 and store nothing, so that every load stays unconditional). Thread t stages
-// rows 2 (t / 64), 2 (t / 64) + 1 of every chunk of spectrum rows for node t % 64, so each
-// spectrum row is read once per block of slices; the contraction runs over the C_pad spectrum rows
-// (zero rows pad 2K up to a multiple of 256). Table parts are [J_pad][C_pad] (spectrum rows
-// contiguous) per slab, loaded at the start of each chunk.
+// Synthesis. Workgroup = 128 nodes x 256 time slices (8 wavefronts); wavefront w owns slices
+// 32w..32w+31 as one 32-row MFMA block against the four 32-node blocks, so each table value it
+// loads serves 128 nodes with only 64 accumulator registers (wavefronts past cnt compute on slice 0
+// and store nothing, so that every load stays unconditional). Thread t stages rows
+// 4 (t / 128) .. + 3 of every chunk of spectrum rows for node t % 128;
+// the contraction runs over the C_pad spectrum rows (zero rows pad 2K up to a multiple of 256).
+// Table parts are [J_pad][C_pad] (spectrum rows contiguous) per slab, loaded at the start of each
+// chunk.
 // ---------------------------------------------------------------------------------------------
-constexpr int kSX = 64;    // nodes per workgroup
+constexpr int kSX = 128;   // nodes per workgroup
 constexpr int kSNB = kSX / 32;
-constexpr int kSJ = 512;   // time slices per workgroup
+constexpr int kSJ = 256;   // time slices per workgroup
+constexpr int kSW = kSJ / 32;  // wavefronts per workgroup
 constexpr int kPFS = 4;    // chunks of spectrum rows loaded ahead
 
+// d0 += A B0 and d1 += A B1 over the six part products, the two chains interleaved
+__device__ __forceinline__ void mfma6x2(f32x16& d0, f32x16& d1, const Split8& a, const Split8& b0, const Split8& b1)
+{
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const Split8& b = q ? b1 : b0;
+            f32x16& d = q ? d1 : d0;
+            switch (t) {
+            case 0: d = mfma(a.l, b.h, d); break;
+            case 1: d = mfma(a.h, b.l, d); break;
+            case 2: d = mfma(a.m, b.m, d); break;
+            case 3: d = mfma(a.m, b.h, d); break;
+            case 4: d = mfma(a.h, b.m, d); break;
+            default: d = mfma(a.h, b.h, d); break;
+            }
+        }
+}
+
 __global__ __launch_bounds__(512) void k_synth_bf16(float* __restrict__ out, const float* __restrict__ a,
-                                                    const unsigned short* __restrict__ th,
-                                                    const unsigned short* __restrict__ tm,
-                                                    const unsigned short* __restrict__ tl, long P, int C_pad,
-                                                    int cnt)
+                                                     const unsigned short* __restrict__ th,
+                                                     const unsigned short* __restrict__ tm,
+                                                     const unsigned short* __restrict__ tl, long P, int C_pad,
+                                                     int cnt)
 {
     __shared__ u32x4 Bs[2 * kSNB * 3 * 64];  // [buf][node block][part][half][col]
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const long x0 = (long)blockIdx.x * kSX;
-    const int j0 = blockIdx.y * kSJ + w * 64;
+    const int j0 = blockIdx.y * kSJ + w * 32;
     const bool active = j0 < cnt;  // wave-uniform
     const int half = lane >> 5, col = lane & 31;
-    const float* sp = a + (x0 + lane < P ? x0 + lane : P - 1);  // a[c][x]; staging: node lane, rows 2w, 2w + 1
+    const int sn = tid & (kSX - 1), sr = tid >> 7;  // staging: node, group of 4 rows
+    const float* sp = a + (x0 + sn < P ? x0 + sn : P - 1);  // a[c][x]
     const int nch = C_pad / kCH;
-    auto fetch = [&](int q, float (&r)[2]) {
+    auto fetch = [&](int q, float (&r)[4]) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i) r[i] = sp[(long)(kCH * q + 2 * w + i) * P];
+        for (int i = 0; i < 4; ++i) r[i] = sp[(long)(kCH * q + 4 * sr + i) * P];
     };
-    // the lane's two slices (slice 0 for the wavefronts past cnt)
-    const long ta = (long)((active ? j0 : 0) + col) * C_pad + 8 * half, tb = ta + 32L * C_pad;
-    auto table = [&](int q, Split8 (&T)[2]) {
-        T[0] = ld_split(th, tm, tl, ta + kCH * q);
-        T[1] = ld_split(th, tm, tl, tb + kCH * q);
-    };
-    float raw[kPFS][2];
+    // the lane's slice (slice 0 for the wavefronts past cnt)
+    const long ta = (long)((active ? j0 : 0) + col) * C_pad + 8 * half;
+    float raw[kPFS][4];
 #pragma unroll
     for (int u = 0; u < kPFS; ++u) fetch(u < nch ? u : nch - 1, raw[u]);
-    f32x16 d[2][kSNB] = {};  // [slice block][node block]
+    f32x16 d[kSNB] = {};  // [node block]
     for (int q0 = 0; q0 < nch; q0 += kPFS) {
 #pragma unroll
         for (int u = 0; u < kPFS; ++u) {
             const int q = q0 + u;
             if (q >= nch) break;  // uniform
-            Split8 T[2];
-            table(q, T);  // L2-resident; in flight across the staging below
+            const Split8 T = ld_split(th, tm, tl, ta + kCH * q);  // L2-resident; in flight across the staging
             __builtin_amdgcn_sched_barrier(0);
-            stage_rows<kSNB, 2>(Bs, q & 1, lane >> 5, w >> 2, lane & 31, 2 * (w & 3), raw[u]);
+            stage_rows<kSNB, 4>(Bs, q & 1, sn >> 5, sr >> 1, sn & 31, 4 * (sr & 1), raw[u]);
             fetch(q + kPFS < nch ? q + kPFS : nch - 1, raw[u]);
             __syncthreads();
 #pragma unroll
-            for (int nb = 0; nb < kSNB; ++nb) mfma6<kSNB>(d, T, read_b<kSNB>(Bs, q & 1, nb, lane), nb);
+            for (int nb = 0; nb < kSNB; nb += 2)
+                mfma6x2(d[nb], d[nb + 1], T, read_b<kSNB>(Bs, q & 1, nb, lane), read_b<kSNB>(Bs, q & 1, nb + 1, lane));
         }
     }
     if (!active) return;
 #pragma unroll
-    for (int jb = 0; jb < 2; ++jb)
+    for (int nb = 0; nb < kSNB; ++nb) {
+        const long x = x0 + 32 * nb + col;
+        if (x >= P) continue;
 #pragma unroll
-        for (int nb = 0; nb < kSNB; ++nb) {
-            const long x = x0 + 32 * nb + col;
-            if (x >= P) continue;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int j = j0 + 32 * jb + d_row(r, lane);
-                if (j < cnt) out[(long)j * P + x] = d[jb][nb][r];
-            }
+        for (int r = 0; r < 16; ++r) {
+            const int j = j0 + d_row(r, lane);
+            if (j < cnt) out[(long)j * P + x] = d[nb][r];
         }
+    }
 }
 
 void launch_synth_bf16(float* out, const float* a, const unsigned short* th, const unsigned short* tm,
@@ -337,7 +354,7 @@ void launch_synth_bf16(float* out, const float* a, const unsigned short* th, con
     if (cnt <= 0) return;
     const unsigned nxb = (unsigned)((P + kSX - 1) / kSX);
     const unsigned njg = (unsigned)((cnt + kSJ - 1) / kSJ);
-    hipLaunchKernelGGL(k_synth_bf16, dim3(nxb, njg), dim3(kSJ), 0, s, out, a, th, tm, tl, (long)P, C_pad, cnt);
+    hipLaunchKernelGGL(k_synth_bf16, dim3(nxb, njg), dim3(kSW * 64), 0, s, out, a, th, tm, tl, (long)P, C_pad, cnt);
 }
 
 
