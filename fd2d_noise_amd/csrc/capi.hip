@@ -1563,6 +1563,11 @@ constexpr int kEngineSlots = 1024;  // solves an engine holds at once
 // takes 14.2 us per solve-step against 13.1 us for the solve's own launch, and stepping the 8 solves
 // in turn with their own launches is slower still.
 constexpr int kSoloTiles = 512;
+// MI355X's last-level (Infinity) cache; large solves are grouped while their step working sets fit
+constexpr double kLastLevelCacheBytes = 256.0 * (1 << 20);
+// fields and coefficients one step reads and writes: 32 B per node without the displacement field
+// (Green runs), 40 B with it
+double step_bytes(const Job& j) { return (double)j.g.P * (j.S->with_u ? 40.0 : 32.0); }
 
 struct Engine {
     hipStream_t st = nullptr;
@@ -1595,19 +1600,34 @@ struct Engine {
         groups.clear();
         singles.clear();
         std::vector<std::pair<std::pair<int, int>, Entry*>> cls;
-        bool solo_taken = false;
+        std::vector<Entry*> big;  // the oldest large solves of one class
+        double big_bytes = 0.0;
         for (auto& e : act) {
             Job& j = *e->job;
             if (j.next > j.nsteps) continue;
-            // A solve that fills the chip on its own runs alone, the oldest first: stepping several
-            // of them in turn (or in one launch) lets their working sets evict each other from the
-            // last-level cache, which a solve of up to ~60 MB per step otherwise keeps
+            // Solves that fill the chip on their own step together, the oldest first, only as many
+            // as the last-level cache holds the per-step working sets of: more of them evict each
+            // other's fields from it; up to that size, one launch over their tiles fills the last
+            // round of workgroups that one solve leaves part-empty.
             if (j.S->n_tiles >= kSoloTiles) {
-                if (!solo_taken) singles.push_back(e.get());
-                solo_taken = true;
+                const double b = step_bytes(j);
+                if (big.empty() || (big_bytes + b <= kLastLevelCacheBytes && big[0]->job->g.R == j.g.R &&
+                                    big[0]->job->S->with_u == j.S->with_u && !j.born && !big[0]->job->born)) {
+                    big.push_back(e.get());
+                    big_bytes += b;
+                }
                 continue;
             }
             cls.push_back({{j.g.R, j.S->with_u ? 1 : 0}, e.get()});
+        }
+        if (big.size() == 1) {
+            singles.push_back(big[0]);
+        } else if (big.size() > 1) {
+            Group gr;
+            gr.R = big[0]->job->g.R;
+            gr.with_u = big[0]->job->S->with_u;
+            gr.members = big;
+            groups.push_back(std::move(gr));
         }
         std::stable_sort(cls.begin(), cls.end(),
                          [](const auto& x, const auto& y) { return x.first < y.first; });
