@@ -871,10 +871,13 @@ struct StepTimer {
     }
     // a launch of `k` steps from step n is timed when one of its steps is a multiple of `every`
     bool on(int n, int k) const { return every > 0 && (n + k - 1) / every > (n - 1) / every; }
+    // Timing-only events: without the system-scope fence of a default event, whose cache
+    // write-back and invalidation would be counted in the timed launch and slow the next one.
+    // collect() runs after the stream has been synchronised, so the times need no fence.
     void record()
     {
         hipEvent_t e;
-        HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         HIPCHK(hipEventRecord(e, st));
         ev.push_back(e);
     }

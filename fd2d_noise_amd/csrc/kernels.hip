@@ -42,7 +42,9 @@ namespace fd2d {
 // tile's x-neighbour, and a tile row later its z-neighbour, read the shared halo lines from the
 // same L2.
 //
-// MODE 0: Green run (no displacement); 1: with displacement (correlation and adjoint runs).
+// MODE 0: Green run (no displacement); 1: with displacement (correlation and adjoint runs);
+// 2: with displacement and the structure-kernel product (the adjoint steps that take it). The
+// product is a mode of its own so that the other steps carry none of its registers and branches.
 __device__ __forceinline__ int node_key(int x, int z) { return ((z + 8) << 16) | (x + 8); }
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* base, unsigned bytes)
@@ -122,7 +124,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     const unsigned cb = a.compact_bytes;
     const bool has_syn = st.syn != nullptr;
     const auto r_syn = rsrc(has_syn ? (const void*)st.syn : (const void*)a.fbase, has_syn ? cb : 0u);
-    const bool kp = HAS_U && st.kfwd != nullptr;  // this step takes a structure-kernel product
+    const bool kp = MODE == 2 && st.kfwd != nullptr;  // this step takes a structure-kernel product
 
     // ---------------- phase 1
     constexpr int NRV = (HV + TW - 1) / TW, NRZ = (HSZ + TW - 1) / TW;
@@ -172,10 +174,17 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         ht[k] = gld(a.axp, xx + kGhostX) * gld(a.azp, zz + kGhost);
         hu[k] = (HAS_U && kp) ? G.field(a.o_u_in, xx, zz) : 0.f;
     }
-    // the forward snapshot of a product step, over the velocity region (used in phase 4; issued
-    // here so its latency hides behind phases 1-3)
+    // the forward snapshot of a product step over the velocity region, and the K_mu values of the
+    // tile's own nodes (used in phase 4; issued here so their latency hides behind phases 1-3)
+    constexpr int NT = TZ / TW;
     float fm[NRV], fh[NHV];
+    double kold[NT];
     if (kp) {
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const unsigned co = G.cofs(x, j0 + w + TW * k);
+            kold[k] = co != kOutside ? gld(a.kacc, (int)(co / 4u)) : 0.0;
+        }
         const auto r_fw = rsrc(st.kfwd, cb);
 #pragma unroll
         for (int k = 0; k < NRV; ++k) fm[k] = bload(r_fw, G.cofs(x, j0 - (R - 1) + w + TW * k));
@@ -281,7 +290,6 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     __syncthreads();
 
     // ---------------- phase 3
-    constexpr int NT = TZ / TW;
     float rm[NT];
 #pragma unroll
     for (int k = 0; k < NT; ++k) rm[k] = G.coef(a.o_dtmu, x, j0 + w + TW * k);
@@ -332,14 +340,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     // region, which covers the (R-1, R) halo of the stencils, so no other tile's output is read;
     // the sums are those of k_kmu, term for term.
     if (kp) {
-        // the K_mu values of the tile's own nodes, loaded before the barrier
         const bool xin = x >= R - 1 && x <= nx - R - 1;
-        double kold[NT];
-#pragma unroll
-        for (int k = 0; k < NT; ++k) {
-            const unsigned co = G.cofs(x, j0 + w + TW * k);
-            kold[k] = co != kOutside ? gld(a.kacc, (int)(co / 4u)) : 0.0;
-        }
         __syncthreads();  // phase 3 has read vv, sx and sz
 #pragma unroll
         for (int k = 0; k < NRV; ++k) {
@@ -878,9 +879,12 @@ void launch_pair(int R, bool with_u, const PairArgs& p, hipStream_t s)
 void launch_group(int R, bool with_u, const GroupArgs& ga, hipStream_t s)
 {
     const dim3 g((unsigned)(kXCD * ga.tiles_per_xcd)), b(kTileThreads);
+    bool product = false;  // some solve of the group takes a structure-kernel product this step
+    for (int j = 0; j < ga.n_jobs; ++j) product = product || ga.step[j].kfwd != nullptr;
 #define FD2D_GROUP(RR)                                                                           \
     case RR:                                                                                     \
-        if (with_u) hipLaunchKernelGGL((k_tile_group<RR, 1>), g, b, 0, s, ga);                   \
+        if (with_u && product) hipLaunchKernelGGL((k_tile_group<RR, 2>), g, b, 0, s, ga);        \
+        else if (with_u) hipLaunchKernelGGL((k_tile_group<RR, 1>), g, b, 0, s, ga);              \
         else hipLaunchKernelGGL((k_tile_group<RR, 0>), g, b, 0, s, ga);                          \
         break;
     switch (R) {
@@ -897,7 +901,8 @@ void launch_step(int R, bool with_u, const StepArgs& a, hipStream_t s)
     const dim3 g((unsigned)(kXCD * a.tiles_per_xcd)), b(kTileThreads);
 #define FD2D_TILE(RR)                                                                            \
     case RR:                                                                                     \
-        if (with_u) hipLaunchKernelGGL((k_tile<RR, 1>), g, b, 0, s, a);                          \
+        if (with_u && a.kfwd) hipLaunchKernelGGL((k_tile<RR, 2>), g, b, 0, s, a);                \
+        else if (with_u) hipLaunchKernelGGL((k_tile<RR, 1>), g, b, 0, s, a);                     \
         else hipLaunchKernelGGL((k_tile<RR, 0>), g, b, 0, s, a);                                 \
         break;
     switch (R) {
