@@ -93,6 +93,23 @@ def test_adjoint_pair(orc, mode, fw_nth):
     _cmp(f"adjoint mode {mode} f {fw_nth}", one, two)
 
 
+@pytest.mark.parametrize("order,nx,nz", [(2, 50, 50), (4, 67, 45), (6, 50, 50)])
+def test_product_steps_each_order(orc, order, nx, nz):
+    # single steps carry the K_mu product in k_tile<R,2>; two-step launches leave it to k_kmu<R>
+    c = Config(order=order, nx=nx, nz=nz, fw_nth=3, Lx=1000.0 * (nx - 1), Lz=1000.0 * (nz - 1))
+    rng = np.random.default_rng(740 + order)
+    mu, rho = hetero_model(rng, c)
+    src = interior_points(rng, c, 1, 0.45)
+    rec = interior_points(rng, c, 2)
+    G, _, _ = orc.run_forward1_green(mu, rho, src, rec, 1, config=c)
+    spectrum = np.exp(-((np.asarray(c.f_sample) - 0.1) ** 2) / 0.05 ** 2)[:, None]
+    corr, C = orc.run_forward2_correlation(mu, rho, G, spectrum, blob_distribution(rng, c), rec, 1, config=c)
+    stf = np.ascontiguousarray(corr[:, ::-1] / np.abs(corr).max()).astype(np.complex128)
+    one, two = both(lambda: fd.run_noise_adjoint(mu, rho, C, stf, rec, rec, spectrum, None, G, 1, config=c))
+    assert np.abs(np.asarray(one[0])).max() > 0, "no structure kernel accumulated"
+    _cmp(f"adjoint products order {order}", one, two)
+
+
 @pytest.mark.parametrize("order", [4, 8])
 def test_tuned_launch_shape_matches_single_steps(orc, order):
     """Mode 0: the first solve of a shape times single and two-step launches on its own early steps
