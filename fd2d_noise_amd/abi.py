@@ -14,11 +14,14 @@ import numpy as np
 
 from .config import Config
 
-FD2D_OK, FD2D_EINVAL, FD2D_ENOMEM, FD2D_EDEVICE, FD2D_EUNSUPPORTED = 0, 1, 2, 3, 4
+FD2D_OK, FD2D_EINVAL, FD2D_ENOMEM, FD2D_EDEVICE, FD2D_EUNSUPPORTED, FD2D_ENONFINITE = 0, 1, 2, 3, 4, 5
 FD2D_GREEN, FD2D_CORRELATION, FD2D_ADJOINT = 1, 2, 3
+# fd2d_solve_state: states of an engine's solve
+(FD2D_SOLVE_RUNNING, FD2D_SOLVE_FINISHING, FD2D_SOLVE_COMPLETE, FD2D_SOLVE_CANCELLED,
+ FD2D_SOLVE_FAILED) = 0, 1, 2, 3, 4
 
 _STATUS = {FD2D_EINVAL: "EINVAL", FD2D_ENOMEM: "ENOMEM", FD2D_EDEVICE: "EDEVICE",
-           FD2D_EUNSUPPORTED: "EUNSUPPORTED"}
+           FD2D_EUNSUPPORTED: "EUNSUPPORTED", FD2D_ENONFINITE: "ENONFINITE"}
 
 
 class Fd2dError(RuntimeError):

@@ -55,6 +55,8 @@ def _bind():
             ("fd2d_engine_submit", [vp, ctypes.POINTER(CSolve), ctypes.POINTER(i64)]),
             ("fd2d_engine_advance", [vp, i32, ctypes.POINTER(i32)]),
             ("fd2d_engine_done", [vp, i64, ctypes.POINTER(i32)]),
+            ("fd2d_engine_progress", [vp, i64, ctypes.POINTER(i32), ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+            ("fd2d_engine_cancel", [vp, i64, ctypes.POINTER(i32)]),
             ("fd2d_engine_destroy", [vp]),
             ("fd2d_run_batch_device", [ctypes.POINTER(CSolve), i32, vp]),
             ("fd2d_solve_device_bytes", [ctypes.POINTER(CSolve), ctypes.POINTER(i64)]),
@@ -246,9 +248,28 @@ class Engine:
         return int(n.value)
 
     def done(self, ticket: int) -> bool:
+        """The solve no longer runs and its queued work has completed (complete, cancelled or failed)."""
         d = ctypes.c_int32(0)
         _check(self._lib.fd2d_engine_done(self._h, int(ticket), ctypes.byref(d)))
         return bool(d.value)
+
+    def progress(self, ticket: int):
+        """(state, steps issued, steps total) of a solve; the step counts are host-side and -1 once
+        the solve is complete. state is one of abi.FD2D_SOLVE_*."""
+        st, a, b = ctypes.c_int32(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        _check(self._lib.fd2d_engine_progress(self._h, int(ticket), ctypes.byref(st), ctypes.byref(a),
+                                              ctypes.byref(b)))
+        return int(st.value), int(a.value), int(b.value)
+
+    def state(self, ticket: int) -> int:
+        return self.progress(ticket)[0]
+
+    def cancel(self, ticket: int) -> bool:
+        """Stops a running solve (its outputs are left undefined); False if it had finished."""
+        c = ctypes.c_int32(0)
+        _check(self._lib.fd2d_engine_cancel(self._h, int(ticket), ctypes.byref(c)))
+        self._live.pop(int(ticket), None)
+        return bool(c.value)
 
     @property
     def active(self) -> int:
@@ -286,9 +307,9 @@ def run_programs(programs: Sequence, rounds: int = 64) -> list:
     results = [None] * len(progs)
     pending = {}
     with Engine() as eng:
-        def resume(q):
+        def resume(q, exc=None):
             try:
-                specs = next(progs[q])
+                specs = progs[q].throw(exc) if exc is not None else next(progs[q])
             except StopIteration as fin:
                 results[q] = fin.value
                 pending.pop(q, None)
@@ -299,6 +320,10 @@ def run_programs(programs: Sequence, rounds: int = 64) -> list:
         while pending:
             eng.advance(rounds)
             for q in [q for q, (_, ts) in pending.items() if all(eng.done(t) for t in ts)]:
-                resume(q)
+                bad = [k for k, t in enumerate(pending[q][1]) if eng.state(t) == abi.FD2D_SOLVE_FAILED]
+                # a failed solve is raised inside its program (which may handle it); the other
+                # programs are not affected
+                resume(q, abi.Fd2dError(abi.FD2D_ENONFINITE, f"solve {bad[0]} of the stage became NaN or Inf")
+                       if bad else None)
     torch.cuda.synchronize()
     return results

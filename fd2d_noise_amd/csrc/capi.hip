@@ -1069,9 +1069,18 @@ void run_job(Job& j)
     for (hipEvent_t e : ev)
         if (e) (void)hipEventDestroy(e);
     j.finish();
+    // the outputs are written either way; a NaN or Inf left in the fields is reported
+    DBuf<int> chk(kCheckBlocks + 1);
+    launch_nonfinite(j.S->fields.p, j.S->fields.n, chk.p, chk.p + kCheckBlocks, j.st);
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, chk.p + kCheckBlocks, sizeof bad, hipMemcpyDeviceToHost, j.st));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(j.st));
     timer.collect();
+    if (bad)
+        raise(FD2D_ENONFINITE, "the wavefield became non-finite (NaN or Inf) within %d steps: dt = %g may exceed the "
+              "stability (CFL) limit of the order-%d stencil at dx = %g, dz = %g", j.nsteps, j.g.dt, 2 * j.g.R,
+              j.g.dx, j.g.dz);
 }
 
 // =============================================================================================
@@ -1503,9 +1512,13 @@ struct Entry {
     int slot = -1;
     StepArgs stat[2]{};             // host copy of the static arguments written to the slot
     hipEvent_t done_ev = nullptr;   // recorded after the solve's last work
+    hipEvent_t chk_ev = nullptr;    // recorded after the copy of its last finiteness check
+    bool chk_pending = false;       // a check is queued and chk_ev not yet seen complete
+    int next_check = 0;             // steps issued at which the next periodic check is queued
     ~Entry()
     {
         if (done_ev) (void)hipEventDestroy(done_ev);
+        if (chk_ev) (void)hipEventDestroy(chk_ev);
     }
     const fd2d_carray* cin(int q) const { return has_in[q] ? &in[q] : nullptr; }
     fd2d_marray* mout(int q) { return has_out[q] ? &out[q] : nullptr; }
@@ -1572,12 +1585,26 @@ constexpr double kLastLevelCacheBytes = 256.0 * (1 << 20);
 // (Green runs), 40 B with it
 double step_bytes(const Job& j) { return (double)j.g.P * (j.S->with_u ? 40.0 : 32.0); }
 
+// Steps between the periodic finiteness checks of an engine's solve (each reads the solve's field
+// pool once: 1/kCheckEvery of a step's traffic).
+constexpr int kCheckEvery = 512;
+
 struct Engine {
     hipStream_t st = nullptr;
     std::vector<std::unique_ptr<Entry>> act;       // running
-    std::vector<std::unique_ptr<Entry>> draining;  // finished, their last work still queued
-    std::vector<int64_t> done_tickets;             // finished and complete
+    std::vector<std::unique_ptr<Entry>> draining;  // finished, cancelled or failed; their last work still queued
+    // Solves that were cancelled or failed, with the steps they had issued. Complete solves keep no
+    // record: a ticket below next_ticket that is neither running, draining nor here is complete.
+    struct Closed {
+        int state;
+        int64_t issued, total;
+    };
+    std::map<int64_t, Closed> closed;
     int64_t next_ticket = 1;
+    // finiteness checks: per-slot flags on the device, copied into pinned host memory before the
+    // check's event; the partials are the scratch of the check kernel (one check at a time per stream)
+    DBuf<int> chk_partial, chk_flag_d;
+    int* chk_flag_h = nullptr;
     bool dirty = true;
     // Static StepArgs of every solve (both parities) at a fixed slot, written once at submission:
     // grouped launches index it with the slot each JobStep carries.
@@ -1595,7 +1622,57 @@ struct Engine {
     {
         StreamScope scope(st);
         slots_d.alloc((size_t)2 * kEngineSlots);
+        chk_partial.alloc(kCheckBlocks);
+        chk_flag_d.alloc(kEngineSlots);
+        HIPCHK(hipHostMalloc((void**)&chk_flag_h, sizeof(int) * kEngineSlots, hipHostMallocDefault));
+        std::fill(chk_flag_h, chk_flag_h + kEngineSlots, 0);
         for (int q = kEngineSlots - 1; q >= 0; --q) free_slots.push_back(q);
+    }
+    // Queue a finiteness check of e's field pool; its flag reaches chk_flag_h[slot] by chk_ev.
+    void queue_check(Entry& e)
+    {
+        const Solver& S = *e.job->S;
+        launch_nonfinite(S.fields.p, S.fields.n, chk_partial.p, chk_flag_d.p + e.slot, st);
+        HIPCHK(hipMemcpyAsync(chk_flag_h + e.slot, chk_flag_d.p + e.slot, sizeof(int), hipMemcpyDeviceToHost, st));
+        if (!e.chk_ev) HIPCHK(hipEventCreateWithFlags(&e.chk_ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(e.chk_ev, st));
+        e.chk_pending = true;
+    }
+    // Take a running solve out (cancelled or failed): no further steps; its queued work drains.
+    void close(std::vector<std::unique_ptr<Entry>>::iterator it, int state)
+    {
+        Entry& e = **it;
+        closed[e.ticket] = Closed{state, (int64_t)e.job->next - 1, (int64_t)e.job->nsteps};
+        if (!e.done_ev) HIPCHK(hipEventCreateWithFlags(&e.done_ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(e.done_ev, st));
+        draining.push_back(std::move(*it));
+        act.erase(it);
+        dirty = true;
+    }
+    // Periodic checks: fail the solves whose last check found a NaN or Inf, and queue the next
+    // check of every solve that has issued kCheckEvery more steps. A failed solve's fields are its
+    // own, so the solves batched with it are not affected.
+    void poll_checks()
+    {
+        for (size_t q = 0; q < act.size();) {
+            Entry& e = *act[q];
+            if (e.chk_pending) {
+                const hipError_t r = hipEventQuery(e.chk_ev);
+                if (r != hipErrorNotReady) {
+                    HIPCHK(r);
+                    e.chk_pending = false;
+                    if (chk_flag_h[e.slot]) {
+                        close(act.begin() + q, FD2D_SOLVE_FAILED);
+                        continue;
+                    }
+                }
+            }
+            if (!e.chk_pending && e.job->next - 1 >= e.next_check) {
+                queue_check(e);
+                e.next_check += kCheckEvery;
+            }
+            ++q;
+        }
     }
     // Re-partition the running solves after the set changed (host work only).
     void rebuild()
@@ -1759,20 +1836,24 @@ struct Engine {
             const hipError_t q = hipEventQuery(e->done_ev);
             if (q == hipErrorNotReady) continue;
             HIPCHK(q);
-            done_tickets.push_back(e->ticket);
+            // the final check of a finished solve was queued before done_ev
+            if (chk_flag_h[e->slot] && !closed.count(e->ticket))
+                closed[e->ticket] = Closed{FD2D_SOLVE_FAILED, (int64_t)e->job->nsteps, (int64_t)e->job->nsteps};
+            chk_flag_h[e->slot] = 0;
             free_slots.push_back(e->slot);
             e.reset();
         }
         draining.erase(std::remove(draining.begin(), draining.end(), nullptr), draining.end());
     }
-    // Finish the solves that have taken their last step: their final work is queued and an event
-    // marks its completion; no host wait.
+    // Finish the solves that have taken their last step: their final work and a last finiteness
+    // check are queued and an event marks their completion; no host wait.
     void retire()
     {
         bool any = false;
         for (auto& e : act)
             if (e->job->next > e->job->nsteps) {
                 e->job->finish();
+                queue_check(*e);
                 HIPCHK(hipEventCreateWithFlags(&e->done_ev, hipEventDisableTiming));
                 HIPCHK(hipEventRecord(e->done_ev, st));
                 draining.push_back(std::move(e));
@@ -1794,6 +1875,8 @@ struct Engine {
         e->stat[1] = e->job->static_args(1);
         HIPCHK(hipMemcpyAsync(slots_d.p + 2 * (size_t)e->slot, e->stat, sizeof e->stat, hipMemcpyHostToDevice, st));
         free_slots.pop_back();
+        chk_flag_h[e->slot] = 0;
+        e->next_check = kCheckEvery;
         e->ticket = next_ticket++;
         act.push_back(std::move(e));
         dirty = true;
@@ -1808,6 +1891,7 @@ struct Engine {
             if (dirty) rebuild();
             for (const Group& gr : groups) step_group(gr);
             for (Entry* e : singles) step_single(*e->job);
+            poll_checks();
         }
         retire();
         HIPCHK(hipGetLastError());
@@ -1818,10 +1902,51 @@ struct Engine {
     }
     // solves submitted and not yet complete
     int pending() const { return (int)(act.size() + draining.size()); }
+    // No longer running and its queued work complete (complete, cancelled or failed).
     bool is_done(int64_t ticket)
     {
         reap();
-        return std::find(done_tickets.begin(), done_tickets.end(), ticket) != done_tickets.end();
+        for (auto& e : act)
+            if (e->ticket == ticket) return false;
+        for (auto& e : draining)
+            if (e->ticket == ticket) return false;
+        return true;
+    }
+    // State of a known ticket and the time steps it has issued (a host-side count: every round
+    // issues one or two steps of each running solve, independent of how far the device has got).
+    int state_of(int64_t ticket, int64_t* issued, int64_t* total)
+    {
+        reap();
+        auto cl = closed.find(ticket);
+        if (cl != closed.end()) {
+            *issued = cl->second.issued;
+            *total = cl->second.total;
+            return cl->second.state;
+        }
+        for (auto& e : act)
+            if (e->ticket == ticket) {
+                *issued = e->job->next - 1;
+                *total = e->job->nsteps;
+                return FD2D_SOLVE_RUNNING;
+            }
+        for (auto& e : draining)
+            if (e->ticket == ticket) {
+                *issued = *total = e->job->nsteps;
+                return FD2D_SOLVE_FINISHING;
+            }
+        *issued = *total = -1;
+        return FD2D_SOLVE_COMPLETE;
+    }
+    // Cancel a running solve: it issues no further steps. Returns false when it is not running.
+    bool cancel(int64_t ticket)
+    {
+        for (auto it = act.begin(); it != act.end(); ++it)
+            if ((*it)->ticket == ticket) {
+                close(it, FD2D_SOLVE_CANCELLED);
+                reap();
+                return true;
+            }
+        return false;
     }
     ~Engine()
     {
@@ -1830,6 +1955,9 @@ struct Engine {
         act.clear();
         draining.clear();
         slots_d.release();
+        chk_partial.release();
+        chk_flag_d.release();
+        if (chk_flag_h) (void)hipHostFree(chk_flag_h);
     }
 };
 
@@ -1918,6 +2046,23 @@ struct Staged {
 };
 
 const fd2d_carray* ptr_or_null(const fd2d_carray& a) { return a.data ? &a : nullptr; }
+
+// A solve that ends non-finite still returns its outputs (the reference returns whatever its arrays
+// hold): `back` copies them to the host before the failure is reported.
+template <class F, class B>
+void solve_then_copy(F&& solve, B&& back)
+{
+    try {
+        solve();
+    } catch (const Failure& e) {
+        if (e.code != FD2D_ENONFINITE) throw;
+        const std::string msg = g_err;
+        back();
+        g_err = msg;
+        throw;
+    }
+    back();
+}
 
 template <class F>
 int guarded(F&& f)
@@ -2216,6 +2361,30 @@ int fd2d_engine_done(const fd2d_engine* engine, int64_t ticket, int32_t* done)
     });
 }
 
+int fd2d_engine_progress(const fd2d_engine* engine, int64_t ticket, int32_t* state, int64_t* steps_issued,
+                         int64_t* steps_total)
+{
+    return guarded([&] {
+        if (!engine || !state) raise(FD2D_EINVAL, "engine and state must not be NULL");
+        if (ticket < 1 || ticket >= engine->e.next_ticket) raise(FD2D_EINVAL, "unknown ticket %lld", (long long)ticket);
+        int64_t a = 0, b = 0;
+        *state = const_cast<fd2d_engine*>(engine)->e.state_of(ticket, &a, &b);
+        if (steps_issued) *steps_issued = a;
+        if (steps_total) *steps_total = b;
+    });
+}
+
+int fd2d_engine_cancel(fd2d_engine* engine, int64_t ticket, int32_t* cancelled)
+{
+    return guarded([&] {
+        if (!engine) raise(FD2D_EINVAL, "engine is NULL");
+        if (ticket < 1 || ticket >= engine->e.next_ticket) raise(FD2D_EINVAL, "unknown ticket %lld", (long long)ticket);
+        StreamScope scope(engine->e.st);
+        const bool c = engine->e.cancel(ticket);
+        if (cancelled) *cancelled = c ? 1 : 0;
+    });
+}
+
 int fd2d_engine_destroy(fd2d_engine* engine)
 {
     return guarded([&] { delete engine; });
@@ -2227,9 +2396,20 @@ int fd2d_run_batch_device(const fd2d_solve* solves, int32_t n, void* stream)
         if (n < 0 || (n > 0 && !solves)) raise(FD2D_EINVAL, "solves is NULL");
         require_device();
         Engine e(as_stream(stream));
-        for (int32_t q = 0; q < n; ++q) e.submit(&solves[q]);
+        std::vector<int64_t> tk((size_t)n);
+        for (int32_t q = 0; q < n; ++q) tk[(size_t)q] = e.submit(&solves[q]);
         while (!e.act.empty()) e.advance(1 << 20);
         HIPCHK(hipStreamSynchronize(e.st));
+        e.reap();
+        std::string bad;
+        for (int32_t q = 0; q < n; ++q) {
+            int64_t a = 0, b = 0;
+            if (e.state_of(tk[(size_t)q], &a, &b) == FD2D_SOLVE_FAILED)
+                bad += (bad.empty() ? "" : ", ") + std::to_string(q);
+        }
+        if (!bad.empty())
+            raise(FD2D_ENONFINITE, "solve(s) %s of the batch became non-finite (NaN or Inf): the time step may "
+                  "exceed the stability (CFL) limit of the FD order; the other solves are complete", bad.c_str());
     });
 }
 
@@ -2307,11 +2487,16 @@ int fd2d_run_forward1_green(const fd2d_config* cfg, const fd2d_carray* mu, const
         fd2d_carray dmu_d = s.in(dmu, 8), mu_d = s.in(mu, 8), rho_d = s.in(rho, 8), src_d = s.in(src, 8),
                     rec_d = s.in(rec, 8), gin_d = s.in(G_in, 4);
         fd2d_marray gf = s.out(G_fft, 16), go = s.out(G_out, 4), sg = s.out(seismograms, 8);
-        green_device(cfg, &mu_d, &rho_d, ptr_or_null(src_d), ptr_or_null(rec_d), mode, ptr_or_null(dmu_d),
-                     ptr_or_null(gin_d), &gf, &go, &sg, nullptr);
-        Staged::back(G_fft, gf, 16);
-        if (mode != 0) Staged::back(G_out, go, 4);
-        Staged::back(seismograms, sg, 8);
+        solve_then_copy(
+            [&] {
+                green_device(cfg, &mu_d, &rho_d, ptr_or_null(src_d), ptr_or_null(rec_d), mode, ptr_or_null(dmu_d),
+                             ptr_or_null(gin_d), &gf, &go, &sg, nullptr);
+            },
+            [&] {
+                Staged::back(G_fft, gf, 16);
+                if (mode != 0) Staged::back(G_out, go, 4);
+                Staged::back(seismograms, sg, 8);
+            });
     });
 }
 
@@ -2327,10 +2512,15 @@ int fd2d_run_forward2_correlation(const fd2d_config* cfg, const fd2d_carray* mu,
         fd2d_carray mu_d = s.in(mu, 8), rho_d = s.in(rho, 8), g_d = s.in(G_fft, 16), sp_d = s.in(spectrum, 8),
                     sd_d = s.in(source_dist, 8), rec_d = s.in(rec, 8), dmu_d = s.in(dmu, 8), cin_d = s.in(C_in, 4);
         fd2d_marray sg = s.out(seismograms, 8), co = s.out(C_out, 4);
-        correlation_device(cfg, &mu_d, &rho_d, &g_d, &sp_d, &sd_d, ptr_or_null(rec_d), mode, ptr_or_null(dmu_d),
-                           ptr_or_null(cin_d), &sg, &co, nullptr);
-        Staged::back(seismograms, sg, 8);
-        if (mode != 0) Staged::back(C_out, co, 4);
+        solve_then_copy(
+            [&] {
+                correlation_device(cfg, &mu_d, &rho_d, &g_d, &sp_d, &sd_d, ptr_or_null(rec_d), mode,
+                                   ptr_or_null(dmu_d), ptr_or_null(cin_d), &sg, &co, nullptr);
+            },
+            [&] {
+                Staged::back(seismograms, sg, 8);
+                if (mode != 0) Staged::back(C_out, co, 4);
+            });
     });
 }
 
@@ -2347,14 +2537,19 @@ int fd2d_run_noise_adjoint(const fd2d_config* cfg, const fd2d_carray* mu, const 
                     src_d = s.in(src, 8), rec_d = s.in(rec, 8), sp_d = s.in(spectrum, 8), sd_d = s.in(source_dist, 8),
                     g_d = s.in(G_fft, 16), dmu_d = s.in(dmu, 8), uin_d = s.in(u_in, 4);
         fd2d_marray kk = s.out(K, 8), sf = s.out(stf_fft, 16), uo = s.out(u_out, 4), sg = s.out(seismograms, 8);
-        adjoint_device(cfg, &mu_d, &rho_d, ptr_or_null(uf_d), ptr_or_null(stf_d), ptr_or_null(src_d),
-                       ptr_or_null(rec_d), ptr_or_null(sp_d), ptr_or_null(sd_d), ptr_or_null(g_d), mode,
-                       ptr_or_null(dmu_d), ptr_or_null(uin_d), &kk, &sf, &uo, &sg, nullptr);
         const int im = (int)mode;
-        Staged::back(K, kk, 8);
-        if (im == 1 || im == 12 || im == 13) Staged::back(stf_fft, sf, 16);
-        if (im == 11 || im == 13) Staged::back(u_out, uo, 4);
-        Staged::back(seismograms, sg, 8);
+        solve_then_copy(
+            [&] {
+                adjoint_device(cfg, &mu_d, &rho_d, ptr_or_null(uf_d), ptr_or_null(stf_d), ptr_or_null(src_d),
+                               ptr_or_null(rec_d), ptr_or_null(sp_d), ptr_or_null(sd_d), ptr_or_null(g_d), mode,
+                               ptr_or_null(dmu_d), ptr_or_null(uin_d), &kk, &sf, &uo, &sg, nullptr);
+            },
+            [&] {
+                Staged::back(K, kk, 8);
+                if (im == 1 || im == 12 || im == 13) Staged::back(stf_fft, sf, 16);
+                if (im == 11 || im == 13) Staged::back(u_out, uo, 4);
+                Staged::back(seismograms, sg, 8);
+            });
     });
 }
 

@@ -231,6 +231,10 @@ void launch_finalize_dft(double2* out, const double2* acc, size_t n, double scal
 void launch_source_kernel(double* Ks, const double2* G, const double2* D, const double* I,
                           const double* spec, size_t P, int K, int nb_cols, double scale,
                           hipStream_t s);
+// *flag = 1 if any of the n floats at x (n a multiple of 4, x 16-byte aligned) is NaN or Inf,
+// else 0; partial holds kCheckBlocks ints of scratch
+constexpr int kCheckBlocks = 256;
+void launch_nonfinite(const float* x, size_t n, int* partial, int* flag, hipStream_t s);
 // point-source amplitudes: amp[n][q] = (float)Re(stf(q, n)) from complex nr x N interleaved
 void launch_stf_real(float* amp, const double2* stf, int64_t d0, int nq, int nsteps, hipStream_t s);
 

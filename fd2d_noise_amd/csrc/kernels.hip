@@ -12,6 +12,8 @@
 // phase of the same launch (k_kmu after the second step of a two-step launch). The frequency-domain
 // work is blocked over time: DFT samples and synthesised source slices are staged as FP32 slabs and
 // contracted with the phase tables on the matrix cores in separate launches (spectral.hip).
+#include <cfloat>
+
 #include "engine.h"
 
 namespace fd2d {
@@ -958,6 +960,36 @@ __global__ void k_scale_to_float(const double* in, float* out, size_t n, double 
 void launch_scale_to_float(const double* in, float* out, size_t n, double scale, hipStream_t s)
 {
     hipLaunchKernelGGL(k_scale_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, out, n, scale);
+}
+
+// Finiteness check of a solve's field pool: partial[b] = 1 when block b saw a NaN or Inf among its
+// elements (grid-stride over float4; n4 = elements / 4), then flag = OR of the partials. Every
+// partial and the flag are written on every call, so no clearing is needed between checks.
+__global__ __launch_bounds__(256) void k_nonfinite_partial(const float4* __restrict__ x, size_t n4,
+                                                           int* __restrict__ partial)
+{
+    int bad = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)kCheckBlocks * 256) {
+        const float4 q = x[i];
+        // a comparison with NaN is false; Inf exceeds the largest finite float
+        bad |= !(fabsf(q.x) <= FLT_MAX) | !(fabsf(q.y) <= FLT_MAX) | !(fabsf(q.z) <= FLT_MAX) |
+               !(fabsf(q.w) <= FLT_MAX);
+    }
+    bad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) partial[blockIdx.x] = bad ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kCheckBlocks) void k_nonfinite_final(const int* __restrict__ partial,
+                                                                  int* __restrict__ flag)
+{
+    const int bad = __syncthreads_or(partial[threadIdx.x]);
+    if (threadIdx.x == 0) *flag = bad ? 1 : 0;
+}
+
+void launch_nonfinite(const float* x, size_t n, int* partial, int* flag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_nonfinite_partial, dim3(kCheckBlocks), dim3(256), 0, s, (const float4*)x, n / 4, partial);
+    hipLaunchKernelGGL(k_nonfinite_final, dim3(1), dim3(kCheckBlocks), 0, s, (const int*)partial, flag);
 }
 
 __global__ void k_stf_real(float* amp, const double2* stf, int64_t d0, int nq, int nsteps)

@@ -209,6 +209,7 @@ class _Live:
     specs: List[Any] = dataclasses.field(default_factory=list)
     t_done: float = 0.0
     result: Any = None
+    error: Optional[BaseException] = None  # set instead of result when the request failed or was cancelled
 
 
 class Server:
@@ -289,30 +290,65 @@ class Server:
             self.live.append(head)
             self._resume(head)
 
+    def _finish(self, live: _Live, result=None, error: Optional[BaseException] = None) -> None:
+        live.result, live.error = result, error
+        live.t_done = time.perf_counter()
+        self.in_use -= live.bytes
+        self.live.remove(live)
+        self.done.append(live)
+        if live.gen is not None and error is not None:
+            live.gen.close()
+        live.gen = None
+
     def _resume(self, live: _Live) -> None:
         """Runs the request program to its next stage and submits that stage's solves."""
         try:
             specs = next(live.gen)
         except StopIteration as fin:
-            live.result = fin.value
-            live.t_done = time.perf_counter()
-            self.in_use -= live.bytes
-            self.live.remove(live)
-            self.done.append(live)
             live.gen = None
+            self._finish(live, fin.value)
+            return
+        except abi.Fd2dError as e:  # the request's own caller work rejected its data
+            self._finish(live, error=e)
             return
         live.specs = specs
         live.tickets = [self.engine.submit(s) for s in specs]
         self.gp_updates += sum(s.config.nx * s.config.nz * s.time_steps() for s in specs)
 
+    def _abandon(self, live: _Live, error: BaseException) -> None:
+        """Stops the request's running solves and ends it with `error`; other requests go on."""
+        for t in live.tickets:
+            self.engine.cancel(t)
+        live.tickets = []
+        self._finish(live, error=error)
+
+    def cancel(self, live: _Live) -> bool:
+        """Cancels a queued or running request (it ends with an error); False if it had finished."""
+        if live in self.queue:
+            self.queue.remove(live)
+            live.error, live.t_done = RuntimeError("cancelled"), time.perf_counter()
+            self.done.append(live)
+            return True
+        if live in self.live:
+            self._abandon(live, RuntimeError("cancelled"))
+            return True
+        return False
+
     def step(self) -> None:
-        """Admits what fits, advances the engine, resumes requests whose stage has completed."""
+        """Admits what fits, advances the engine, resumes requests whose stage has completed and
+        ends those with a failed (non-finite) solve."""
         self._admit()
         if not self.live:
             return
         self.engine.advance(self.rounds)
         for live in list(self.live):
-            if all(self.engine.done(t) for t in live.tickets):
+            states = [self.engine.state(t) for t in live.tickets]
+            if abi.FD2D_SOLVE_FAILED in states:
+                q = states.index(abi.FD2D_SOLVE_FAILED)
+                self._abandon(live, abi.Fd2dError(
+                    abi.FD2D_ENONFINITE, f"solve {q} of the request's stage became NaN or Inf (time step beyond "
+                    "the stability limit of the stencil?)"))
+            elif all(self.engine.done(t) for t in live.tickets):
                 live.tickets, live.specs = [], []
                 self._resume(live)
 

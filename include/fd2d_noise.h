@@ -45,7 +45,10 @@ enum fd2d_status {
     FD2D_EINVAL = 1,        /* inconsistent sizes or arguments MATLAB would reject */
     FD2D_ENOMEM = 2,        /* device or host allocation failed */
     FD2D_EDEVICE = 3,       /* HIP runtime error or no usable gfx950 device */
-    FD2D_EUNSUPPORTED = 4   /* FD order not in {2,4,6,8}, or a mode combination that is rejected */
+    FD2D_EUNSUPPORTED = 4,  /* FD order not in {2,4,6,8}, or a mode combination that is rejected */
+    FD2D_ENONFINITE = 5     /* the wavefield became NaN or Inf (e.g. dt beyond the stability limit of
+                               the stencil); the outputs are still written, as the reference returns
+                               its arrays whatever they hold */
 };
 
 /* Which solver fd2d_query_output_dims() describes. */
@@ -337,11 +340,37 @@ int fd2d_engine_submit(fd2d_engine* engine, const fd2d_solve* solve, int64_t* ti
  * solve is left running it waits for the finished ones, so advancing until *n_active == 0 leaves
  * every output complete. */
 int fd2d_engine_advance(fd2d_engine* engine, int32_t rounds, int32_t* n_active);
-/* *done = 1 when the solve's outputs are complete (it has finished and the stream is synchronised). */
+/* *done = 1 when the solve no longer runs and the work it queued has completed: its outputs are
+ * complete, or it was cancelled or failed (fd2d_engine_progress tells which). */
 int fd2d_engine_done(const fd2d_engine* engine, int64_t ticket, int32_t* done);
+
+/* States of an engine's solve. */
+enum fd2d_solve_state {
+    FD2D_SOLVE_RUNNING = 0,    /* steps left to issue */
+    FD2D_SOLVE_FINISHING = 1,  /* every step issued; its last work is still queued */
+    FD2D_SOLVE_COMPLETE = 2,   /* outputs complete */
+    FD2D_SOLVE_CANCELLED = 3,  /* removed by fd2d_engine_cancel; outputs undefined */
+    FD2D_SOLVE_FAILED = 4      /* its wavefield became NaN or Inf (FD2D_ENONFINITE); the solves batched
+                                  with it are not affected */
+};
+/*
+ * *state of a solve and the time steps it has issued (*steps_issued, *steps_total; either pointer
+ * may be NULL). The count is kept on the host: each fd2d_engine_advance round issues one or two
+ * steps of every running solve, whatever the device has completed, so it is the same on every
+ * machine. A complete solve reports -1 for both (the engine keeps no record of finished solves).
+ * A running solve is checked for NaN / Inf every 512 steps and after its last step; a failed one
+ * stops issuing steps. (No counterpart in the reference, whose solver calls run to the end.)
+ */
+int fd2d_engine_progress(const fd2d_engine* engine, int64_t ticket, int32_t* state, int64_t* steps_issued,
+                         int64_t* steps_total);
+/* Cancels a running solve: it issues no further steps and its outputs are left undefined.
+ * *cancelled (may be NULL) = 1 if it was running, 0 if it had already finished. (No counterpart in
+ * the reference; it serves callers that abandon a request, e.g. a serving loop.) */
+int fd2d_engine_cancel(fd2d_engine* engine, int64_t ticket, int32_t* cancelled);
 /* Waits for the stream and frees the engine; unfinished solves are abandoned. */
 int fd2d_engine_destroy(fd2d_engine* engine);
-/* Runs n solves to completion with one engine. */
+/* Runs n solves to completion with one engine. FD2D_ENONFINITE names the solves that became NaN or
+ * Inf; every other solve of the batch is complete. */
 int fd2d_run_batch_device(const fd2d_solve* solves, int32_t n, void* stream);
 /* Upper bound of the device bytes one solve allocates inside the library (the caller's arrays
  * are not counted): the figure an admission policy charges against an HBM budget. */

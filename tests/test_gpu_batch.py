@@ -168,10 +168,26 @@ def test_engine_continuous_batching(torch_mod, mix):
         eng.advance(37)
         tickets += [eng.submit(s) for s in waves[1]]
         eng.advance(101)
-        assert all(not eng.done(t) for t in tickets[len(waves[0]):])
+        # Which solves are still running is decided on the host: every round issues one or two
+        # steps of each running solve, so after 101 rounds a solve of more than 202 steps is running
+        # with 101..202 steps issued, whatever the device has completed. (Whether a finished solve's
+        # last work has completed on the device, done(), depends on timing and is not asserted.)
+        running = 0
+        for t, sp in zip(tickets[len(waves[0]):], waves[1]):
+            state, issued, total = eng.progress(t)
+            n = sp.time_steps()
+            if n > 2 * 101:
+                assert state == abi.FD2D_SOLVE_RUNNING and total == n and 101 <= issued <= 202, (n, state, issued)
+                running += 1
+            else:
+                assert state in (abi.FD2D_SOLVE_RUNNING, abi.FD2D_SOLVE_FINISHING, abi.FD2D_SOLVE_COMPLETE)
+                if state == abi.FD2D_SOLVE_RUNNING:
+                    assert 101 <= issued < n
+        assert running >= 4  # the third wave joins while these run
         tickets += [eng.submit(s) for s in waves[2]]
         eng.run_until_done()
         assert all(eng.done(t) for t in tickets) and eng.active == 0
+        assert all(eng.progress(t) == (abi.FD2D_SOLVE_COMPLETE, -1, -1) for t in tickets)
     order = list(range(0, len(specs), 3)) + list(range(1, len(specs), 3)) + list(range(2, len(specs), 3))
     torch.cuda.synchronize()
     for q in order:
@@ -224,3 +240,84 @@ def test_calculate_data_matches_station_loop(torch_mod, tmp_path):
     net = F.load_array(str(tmp_path / "array_3_ref.mat"))
     assert np.array_equal(dat["c_data"], ref_data) and dat["t"].shape == (1, c.n_corr)
     assert np.array_equal(net["ref_stat"], ref) and np.array_equal(net["array"], array)
+
+
+def _unstable(c):
+    """The same grid with dt 2.5 times the stability limit of its stencil (CFL): the fields grow by
+    about 20x per step and overflow to Inf, then NaN, within a few dozen steps."""
+    coef = {2: [1.0], 4: [9 / 8, 1 / 24], 6: [1.171875, 0.0651042, 0.0046875],
+            8: [1.1962891, 0.0797526, 0.0095703, 0.0006975]}[c.order]
+    vmax = np.sqrt(4.8e10 * 1.1 / 3000.0)
+    dt_lim = min(c.Lx / (c.nx - 1), c.Lz / (c.nz - 1)) / (np.sqrt(2) * sum(coef) * vmax)
+    return c.replace(dt=2.5 * dt_lim)
+
+
+def test_nonfinite_solve_fails_alone(torch_mod, mix):
+    """A solve whose time step exceeds the stability limit becomes NaN / Inf: the engine reports it
+    FAILED and stops it, the solves batched with it are complete and bit-identical to their single
+    runs, and the single-solve entry point returns FD2D_ENONFINITE with its outputs written."""
+    torch = torch_mod
+    from fd2d_noise_amd import batch as B, device as D
+    cases, ref = mix
+    good = [q for q, (k, c, _, _) in enumerate(cases) if k in ("green", "correlation") and c.nx < 100][:4]
+    kind, c, args, dims = cases[good[0]]
+    cu = _unstable(c).replace(nt=2000)
+    bad_outs = _outs(torch, [((cu.nx, cu.nz, cu.n_sample), torch.complex128), None, ((3, cu.nt), torch.float64)])
+    bad = B.green(*args[:4], 0, None, None, config=cu, G_fft=bad_outs[0], seismograms=bad_outs[2])
+    got = [_outs(torch, cases[q][3]) for q in good]
+    specs = [_spec(cases[q][0], cases[q][1], cases[q][2], o) for q, o in zip(good, got)]
+    check_every = 512  # steps between the engine's periodic finiteness checks
+    with B.Engine() as eng:
+        tb = eng.submit(bad)
+        tg = [eng.submit(s) for s in specs]
+        while eng.progress(tb)[1] < check_every + 2:  # the first periodic check is queued
+            eng.advance(1)
+        torch.cuda.synchronize()  # ... and complete, so the next round's poll sees its flag
+        eng.advance(1)
+        state, issued, total = eng.progress(tb)
+        print(f"unstable solve: state {state}, stopped after {issued} of {total} steps", flush=True)
+        assert state == abi.FD2D_SOLVE_FAILED and total == cu.nt and issued <= check_every + 8
+        eng.run_until_done()
+        assert eng.progress(tb)[:2] == (abi.FD2D_SOLVE_FAILED, issued)
+        assert all(eng.progress(t)[0] == abi.FD2D_SOLVE_COMPLETE for t in tg)
+    torch.cuda.synchronize()
+    for q, o in zip(good, got):
+        _same(torch, f"beside the unstable solve: case {q}", ref[q], o)
+    # the batch call names the failed solve and completes the others
+    got2 = [_outs(torch, cases[q][3]) for q in good]
+    with pytest.raises(abi.Fd2dError) as e:
+        B.run_batch([_spec(cases[q][0], cases[q][1], cases[q][2], o) for q, o in zip(good, got2)] + [bad])
+    assert e.value.code == abi.FD2D_ENONFINITE and f"solve(s) {len(good)} " in str(e.value)
+    torch.cuda.synchronize()
+    for q, o in zip(good, got2):
+        _same(torch, f"run_batch beside the unstable solve: case {q}", ref[q], o)
+    # the single-solve entry point: error code, outputs written
+    seis = D.empty_like_dims((3, cu.nt), torch.float64)
+    with pytest.raises(abi.Fd2dError) as e:
+        D.run_forward1_green(*args[:4], 0, None, None, config=cu, G_fft=bad_outs[0], seismograms=seis)
+    assert e.value.code == abi.FD2D_ENONFINITE and "stability" in str(e.value)
+    assert not torch.isfinite(seis).all()
+
+
+def test_engine_cancel(torch_mod, mix):
+    """A cancelled solve issues no further steps; the others complete bit-identically."""
+    torch = torch_mod
+    from fd2d_noise_amd import batch as B
+    cases, ref = mix
+    sel = [q for q, (k, c, _, _) in enumerate(cases) if k == "correlation" and c.nx < 100][:3]
+    got = [_outs(torch, cases[q][3]) for q in sel]
+    specs = [_spec(cases[q][0], cases[q][1], cases[q][2], o) for q, o in zip(sel, got)]
+    with B.Engine() as eng:
+        ts = [eng.submit(s) for s in specs]
+        eng.advance(10)
+        assert eng.cancel(ts[1])
+        state, issued, total = eng.progress(ts[1])
+        assert state == abi.FD2D_SOLVE_CANCELLED and 10 <= issued <= 20 and total == specs[1].time_steps()
+        eng.advance(5)
+        assert eng.progress(ts[1])[1] == issued
+        eng.run_until_done()
+        assert eng.done(ts[1]) and not eng.cancel(ts[0])
+        assert eng.progress(ts[0])[0] == abi.FD2D_SOLVE_COMPLETE
+    torch.cuda.synchronize()
+    for k in (0, 2):
+        _same(torch, f"beside a cancelled solve: case {sel[k]}", ref[sel[k]], got[k])
