@@ -68,7 +68,7 @@ EXPORTS = (
     "fd2d_engine_create", "fd2d_engine_submit", "fd2d_engine_advance", "fd2d_engine_done", "fd2d_engine_destroy",
     "fd2d_engine_progress", "fd2d_engine_cancel",
     "fd2d_run_batch_device", "fd2d_solve_device_bytes", "fd2d_release_cached_memory",
-    "fd2d_make_adjoint_sources_queued",
+    "fd2d_make_adjoint_sources_queued", "fd2d_make_adjoint_sources_noalloc",
     "fd2d_resident_upload", "fd2d_resident_alloc", "fd2d_resident_view", "fd2d_resident_download",
     "fd2d_resident_free",
 )

@@ -2192,9 +2192,38 @@ std::unique_ptr<fd2d_resident> make_resident(const int64_t dims[3], int32_t eb)
 }  // namespace
 
 namespace {
+// Scratch of fd2d_make_adjoint_sources_noalloc: a static device array, so that entry point allocates
+// nothing. Each device holds its own copy from the library's load until the process ends; host code
+// reaches it only through the per-device address hipGetSymbolAddress returns (the array's name is a
+// host-side stand-in there, not a device address). The kernels write every element of it they read.
+constexpr int64_t kNoAllocScratchDoubles = (int64_t)1 << 21;  // 16 MiB: 3 * nr * nt doubles
+static __device__ __align__(256) double noalloc_scratch[kNoAllocScratchDoubles];
+
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kMaxDevices] = {};
+
+// The current device's address of noalloc_scratch, looked up once per device (on its first call).
+double* noalloc_scratch_ptr()
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) raise(FD2D_EDEVICE, "device %d is beyond the library's %d devices", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+        if (!p) raise(FD2D_EDEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        g_noalloc_ptr[dev] = (double*)p;
+    }
+    return g_noalloc_ptr[dev];
+}
+
+// noalloc: take the kernels' scratch from noalloc_scratch instead of allocating it, and read the
+// window limits straight from t_host (no host copy either)
 int adjoint_sources_entry(const double* u, const double* u0, const double* du, const double* t, const double* t_host,
                           int64_t nr, int64_t nt, int32_t veldis, int32_t measurement, int32_t deriv_order,
-                          double win_left, double win_right, double* misfit, double* adstf, void* stream, bool wait)
+                          double win_left, double win_right, double* misfit, double* adstf, void* stream, bool wait,
+                          bool noalloc = false)
 {
     return guarded([&] {
         require_device();
@@ -2204,10 +2233,16 @@ int adjoint_sources_entry(const double* u, const double* u0, const double* du, c
         if (measurement == FD2D_CC_TIME_SHIFT && deriv_order == 2)
             raise(FD2D_EUNSUPPORTED, "no Hessian vector products for traveltime measurements (make_adjoint_sources.m:121)");
         if (deriv_order == 2 && !du) raise(FD2D_EINVAL, "deriv_order 2 needs du");
+        if (noalloc && 3 * nr * nt > kNoAllocScratchDoubles)
+            raise(FD2D_EINVAL, "nr x nt = %lld x %lld needs %lld doubles of scratch; the no-allocation entry point "
+                  "holds %lld (nr * nt <= %lld)", (long long)nr, (long long)nt, (long long)(3 * nr * nt),
+                  (long long)kNoAllocScratchDoubles, (long long)(kNoAllocScratchDoubles / 3));
         hipStream_t st = as_stream(stream);
         StreamScope scope(st);
         // make_adjoint_sources.m:85-99: window limits against the time axis
-        const std::vector<double> th = t_host ? std::vector<double>(t_host, t_host + nt) : to_host<double>(t, (size_t)nt);
+        std::vector<double> th_copy;
+        if (!t_host) th_copy = to_host<double>(t, (size_t)nt);
+        const double* th = t_host ? t_host : th_copy.data();
         double left = win_left, right = win_right;
         if (measurement != FD2D_WAVEFORM_DIFFERENCE) {
             if (left < 0) {
@@ -2219,9 +2254,10 @@ int adjoint_sources_entry(const double* u, const double* u0, const double* du, c
             if (right > th[(size_t)nt - 1]) right = th[(size_t)nt - 1];
             if (left > th[(size_t)nt - 1]) raise(FD2D_EINVAL, "time series too short? (make_adjoint_sources.m:99)");
         }
-        DBuf<double> scratch((size_t)3 * nr * nt);
+        DBuf<double> work;
+        if (!noalloc) work.alloc((size_t)3 * nr * nt);
         AdjArgs a{u, u0, du, t, nr, nt, th[1] - th[0] >= 0 ? th[1] - th[0] : th[0] - th[1], veldis ? 1 : 0,
-                  measurement, deriv_order, left, right, misfit, adstf, scratch.p};
+                  measurement, deriv_order, left, right, misfit, adstf, noalloc ? noalloc_scratch_ptr() : work.p};
         launch_adjoint_sources(a, st);
         HIPCHK(hipGetLastError());
         if (wait) HIPCHK(hipStreamSynchronize(st));
@@ -2573,6 +2609,19 @@ int fd2d_make_adjoint_sources_queued(const double* u, const double* u0, const do
     }
     return adjoint_sources_entry(u, u0, du, t, t_host, nr, nt, veldis, measurement, deriv_order, win_left, win_right,
                                  misfit, adstf, stream, false);
+}
+
+int fd2d_make_adjoint_sources_noalloc(const double* u, const double* u0, const double* du, const double* t,
+                                      const double* t_host, int64_t nr, int64_t nt, int32_t veldis,
+                                      int32_t measurement, int32_t deriv_order, double win_left, double win_right,
+                                      double* misfit, double* adstf, void* stream)
+{
+    if (!t_host) {
+        g_err = "t_host is required";
+        return FD2D_EINVAL;
+    }
+    return adjoint_sources_entry(u, u0, du, t, t_host, nr, nt, veldis, measurement, deriv_order, win_left, win_right,
+                                 misfit, adstf, stream, false, true);
 }
 
 int fd2d_make_adjoint_sources(const double* u, const double* u0, const double* du, const double* t, int64_t nr,

@@ -236,6 +236,21 @@ int fd2d_make_adjoint_sources_queued(const double* u, const double* u0, const do
                                      int32_t measurement, int32_t deriv_order, double win_left, double win_right,
                                      double* misfit, double* adstf, void* stream);
 
+/*
+ * fd2d_make_adjoint_sources_queued without any device allocation. It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The kernels' scratch (3 * nr * nt doubles)
+ * is a static device array of 2^21 doubles (16 MiB) that each GPU the library runs on holds until the
+ * process ends, so nr * nt <= 699050; larger calls return FD2D_EINVAL. The array's address is looked
+ * up on the first call on each device. Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (on other streams or from other host threads) race. The kernels write every
+ * scratch element before reading it, so nothing is cleared between calls.
+ */
+int fd2d_make_adjoint_sources_noalloc(const double* u, const double* u0, const double* du, const double* t,
+                                      const double* t_host, int64_t nr, int64_t nt, int32_t veldis,
+                                      int32_t measurement, int32_t deriv_order, double win_left, double win_right,
+                                      double* misfit, double* adstf, void* stream);
+
 /* x'*y of two device FP64 vectors (n = 0: 0, pointers may be NULL) with compensated summation (the inner products of
  * inversion/optlib/optlib_lbfgs.m:51,92,102,110,175 and optlib_wolfe.m:59,85,105); result to host. */
 int fd2d_dot_device(const double* x, const double* y, int64_t n, double* result, void* stream);
