@@ -630,7 +630,7 @@ int slab_len(size_t P, size_t n_samples)
     size_t T = std::min<size_t>(kSpecSlab, std::max<size_t>(n_samples, 1));
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
-        while (T > 64 && T * P * sizeof(float) > free_b / 8) T /= 2;
+        while (T > 64 && T * P * sizeof(float) > free_b / 4) T /= 2;
     // several slabs: each starts on a 16-sample boundary (the phase tables are read 8 samples at a time)
     if (T < n_samples) T = std::max<size_t>(16, T / 16 * 16);
     return (int)T;
@@ -750,6 +750,7 @@ std::string table_key(char kind, const fd2d_config* cfg, const std::vector<doubl
 
 struct DftAccumulator {
     double2* acc = nullptr;
+    DBuf<double2> comp;  // Neumaier compensation of acc: the sum is acc + comp
     size_t P = 0;
     int K = 0, C_pad = 0, Jt = 0;
     DBuf<float> slab;
@@ -780,6 +781,8 @@ struct DftAccumulator {
                 }
         });
         HIPCHK(hipMemsetAsync(acc, 0, P * (size_t)K * sizeof(double2), st));
+        comp.alloc(P * (size_t)K);
+        HIPCHK(hipMemsetAsync(comp.p, 0, P * (size_t)K * sizeof(double2), st));
     }
     float* next_slot() { return slab.p + (size_t)cnt * P; }
     void commit()
@@ -789,8 +792,8 @@ struct DftAccumulator {
     void flush()
     {
         if (cnt == 0) return;
-        launch_dft_bf16(acc, slab.p, tab->part[0] + done, tab->part[1] + done, tab->part[2] + done, P, K, C_pad, Jt, cnt,
-                        st);
+        launch_dft_bf16(acc, comp.p, slab.p, tab->part[0] + done, tab->part[1] + done, tab->part[2] + done, P, K, C_pad,
+                        Jt, cnt, st);
         done += cnt;
         cnt = 0;
     }
@@ -1165,7 +1168,7 @@ struct GreenJob : Job {
         if (Gf) {
             dft.flush();
             // G_fft = conj(sum v * (2pi)^-1/2 e^{-iwt} dt)  (:178-180, :196-198)
-            launch_finalize_dft(Gf, Gf, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 1, st);
+            launch_finalize_dft(Gf, Gf, dft.comp.p, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 1, st);
         }
     }
 };
@@ -1457,10 +1460,10 @@ struct AdjointJob : Job {
             // K_s(:,:,b) = sum_n Re(M_b(n) u(n)) = (2pi)^-1/2 dw sum_k I(k,b) spec_k Re(conj(G_k D_k)),
             // D_k = sum_n u(n) e^{-i w_k t_n}
             upload(sp_d, std::vector<double>(spec.begin(), spec.begin() + g.K), st);
-            launch_source_kernel(Kd, (const double2*)G->data, dft.acc, I_d.p, sp_d.p, g.P, g.K, nb_cols,
+            launch_source_kernel(Kd, (const double2*)G->data, dft.acc, dft.comp.p, I_d.p, sp_d.p, g.P, g.K, nb_cols,
                                  kInvSqrt2Pi * g.dw, st);
         }
-        if (sfft) launch_finalize_dft(sfft, sfft, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 0, st);
+        if (sfft) launch_finalize_dft(sfft, sfft, dft.comp.p, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 0, st);
     }
 };
 
@@ -1976,7 +1979,8 @@ int64_t solve_bytes(const fd2d_solve* s)
     b += (int64_t)(g.nx / kTX + 2) * (g.nz / kTileZ + 2) * 8 + 8 * 9 * (n_src + n_rec);  // tile lists
     const int N = 2 * g.nt - 1;
     auto slab = [&](int64_t n) { return std::min<int64_t>(kSpecSlab, std::max<int64_t>(n, 1)); };
-    auto dft_bytes = [&](int64_t n) { return slab(n) * P * 4 + 3 * ((n + 15) / 16 * 16) * C_pad * 2; };
+    // slab, phase-table parts and the accumulators' compensation terms
+    auto dft_bytes = [&](int64_t n) { return slab(n) * P * 4 + 3 * ((n + 15) / 16 * 16) * C_pad * 2 + P * K * 16; };
     auto syn_bytes = [&](int64_t n) {
         const int64_t T = slab(n), J = (T + 63) / 64 * 64, slabs = (std::max<int64_t>(n, 1) + T - 1) / T;
         return T * P * 4 + C_pad * P * 4 + 3 * slabs * C_pad * J * 2;  // slab, spectrum rows, tables

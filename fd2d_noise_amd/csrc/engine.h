@@ -207,12 +207,14 @@ void launch_material(const double* mu, const double* rho, float* bv, float* dtmu
                      int pitch, double dt, hipStream_t s);
 void launch_scale_to_float(const double* in, float* out, size_t n, double scale, hipStream_t s);
 
-// acc[k][x] += sum_j samp[j][x] * e[j][k] over cnt samples: matrix-core products of the three
-// bfloat16 parts of each FP32 operand (FP32-accurate), FP64 accumulators; e{h,m,l}[c][Jt] are the
-// parts of Re e[j][k] (c = 2k) and Im e[j][k] (c = 2k + 1), C_pad (a multiple of 256) rows, offset
-// to the slab's first sample (16-byte aligned).
-void launch_dft_bf16(double2* acc, const float* samp, const unsigned short* eh, const unsigned short* em,
-                     const unsigned short* el, size_t P, int K, int C_pad, int Jt, int cnt, hipStream_t s);
+// acc[k][x] + comp[k][x] += sum_j samp[j][x] * e[j][k] over cnt samples: matrix-core products of
+// the three bfloat16 parts of each FP32 operand (FP32-accurate) summed in FP32 over the slab, then
+// added into the FP64 accumulators with Neumaier compensation (comp holds the compensation terms;
+// the sum is acc + comp); e{h,m,l}[c][Jt] are the parts of Re e[j][k] (c = 2k) and Im e[j][k]
+// (c = 2k + 1), C_pad (a multiple of 256) rows, offset to the slab's first sample (16-byte aligned).
+void launch_dft_bf16(double2* acc, double2* comp, const float* samp, const unsigned short* eh,
+                     const unsigned short* em, const unsigned short* el, size_t P, int K, int C_pad, int Jt,
+                     int cnt, hipStream_t s);
 // out[j][x] = sum_c et[j][c] * a[c][x] for j < cnt, the same products: with a = (Re A, Im A) rows
 // and et = (Re, -Im) of the phase factors (parts t{h,m,l}[J_pad][C_pad], J_pad >= cnt rounded up to
 // 64) this is Re(A[k][x] * coef[j][k]) summed over k.
@@ -224,11 +226,11 @@ void split_bf16(float v, unsigned short& h, unsigned short& m, unsigned short& l
 // (FP64, rounded to FP32), dist(x,k) = sum_b I[k,b] sd[b][x]; zero rows 2K..C_pad-1
 void launch_source_spectrum(float* a32, const double2* Z, int conj_in, const double* sd, int n_sd,
                             const double* I, const double* spec_sum, size_t P, int K, int C_pad, hipStream_t s);
-// out[k][x] = scale * (conj ? conj(acc) : acc)
-void launch_finalize_dft(double2* out, const double2* acc, size_t n, double scale, int conj,
+// out[k][x] = scale * (conj ? conj(acc + comp) : acc + comp)  (out may be acc)
+void launch_finalize_dft(double2* out, const double2* acc, const double2* comp, size_t n, double scale, int conj,
                          hipStream_t s);
-// Ks[b][x] = scale * sum_k I[k,b] spec[k] Re(conj(G[k][x] * D[k][x]))
-void launch_source_kernel(double* Ks, const double2* G, const double2* D, const double* I,
+// Ks[b][x] = scale * sum_k I[k,b] spec[k] Re(conj(G[k][x] * D[k][x])), D = Dacc + Dcomp
+void launch_source_kernel(double* Ks, const double2* G, const double2* Dacc, const double2* Dcomp, const double* I,
                           const double* spec, size_t P, int K, int nb_cols, double scale,
                           hipStream_t s);
 // *flag = 1 if any of the n floats at x (n a multiple of 4, x 16-byte aligned) is NaN or Inf,
@@ -238,6 +240,6 @@ void launch_nonfinite(const float* x, size_t n, int* partial, int* flag, hipStre
 // point-source amplitudes: amp[n][q] = (float)Re(stf(q, n)) from complex nr x N interleaved
 void launch_stf_real(float* amp, const double2* stf, int64_t d0, int nq, int nsteps, hipStream_t s);
 
-constexpr int kSpecSlab = 512;  // time samples per DFT flush / synthesis block (at most)
+constexpr int kSpecSlab = 1024;  // time samples per DFT flush / synthesis block (at most)
 
 }  // namespace fd2d

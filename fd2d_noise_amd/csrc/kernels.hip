@@ -1038,39 +1038,44 @@ void launch_source_spectrum(float* a32, const double2* Z, int conj_in, const dou
                        n_sd, I, spec_sum, P, K, C_pad);
 }
 
-__global__ void k_finalize_dft(double2* out, const double2* acc, size_t n, double scale, int conj)
+__global__ void k_finalize_dft(double2* out, const double2* acc, const double2* comp, size_t n, double scale,
+                               int conj)
 {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    const double2 a = acc[c];
-    out[c] = make_double2(scale * a.x, conj ? -(scale * a.y) : scale * a.y);
+    const double2 a = acc[c], e = comp[c];
+    const double re = a.x + e.x, im = a.y + e.y;
+    out[c] = make_double2(scale * re, conj ? -(scale * im) : scale * im);
 }
 
-void launch_finalize_dft(double2* out, const double2* acc, size_t n, double scale, int conj, hipStream_t s)
+void launch_finalize_dft(double2* out, const double2* acc, const double2* comp, size_t n, double scale, int conj,
+                         hipStream_t s)
 {
-    hipLaunchKernelGGL(k_finalize_dft, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, acc, n, scale, conj);
+    hipLaunchKernelGGL(k_finalize_dft, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out, acc, comp, n, scale,
+                       conj);
 }
 
-__global__ void k_source_kernel(double* Ks, const double2* G, const double2* D, const double* I,
-                                const double* spec, size_t P, int K, int nb_cols, double scale)
+__global__ void k_source_kernel(double* Ks, const double2* G, const double2* Dacc, const double2* Dcomp,
+                                const double* I, const double* spec, size_t P, int K, int nb_cols, double scale)
 {
     const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= P) return;
     for (int b = 0; b < nb_cols; ++b) {
         double s = 0.0;
         for (int k = 0; k < K; ++k) {
-            const double2 g = G[(size_t)k * P + x], d = D[(size_t)k * P + x];
+            const double2 g = G[(size_t)k * P + x], da = Dacc[(size_t)k * P + x], dc = Dcomp[(size_t)k * P + x];
+            const double2 d = make_double2(da.x + dc.x, da.y + dc.y);
             s += I[k + (size_t)K * b] * spec[k] * (g.x * d.x - g.y * d.y);
         }
         Ks[(size_t)b * P + x] = scale * s;
     }
 }
 
-void launch_source_kernel(double* Ks, const double2* G, const double2* D, const double* I, const double* spec,
-                          size_t P, int K, int nb_cols, double scale, hipStream_t s)
+void launch_source_kernel(double* Ks, const double2* G, const double2* Dacc, const double2* Dcomp, const double* I,
+                          const double* spec, size_t P, int K, int nb_cols, double scale, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_source_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, Ks, G, D, I, spec, P,
-                       K, nb_cols, scale);
+    hipLaunchKernelGGL(k_source_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, Ks, G, Dacc, Dcomp, I,
+                       spec, P, K, nb_cols, scale);
 }
 
 }  // namespace fd2d

@@ -13,7 +13,8 @@
 // which leaves out terms below 2^-23 of |a b|, the size of an FP32 product's own rounding. The part
 // products run on v_mfma_f32_32x32x16_bf16 (17x the rate of the FP32 v_mfma_f32_32x32x2_f32, so six
 // of them take a third of its time) with FP32 accumulation over one slab, and every slab is added
-// into FP64 accumulators (flush) or written as the FP32 source slice it becomes anyway (synth). The
+// into FP64 accumulators with Neumaier compensation (flush) or written as the FP32 source slice it
+// becomes anyway (synth). The
 // phase tables are split once on the host; the samples and spectra as they are loaded.
 //
 // MFMA 32x32x16 (one wavefront): lane l supplies A[i = l % 32][k = 8 (l / 32) + v] and
@@ -165,7 +166,18 @@ constexpr int kNB = kFX / 32;
 constexpr int kFC = 256;   // table columns per workgroup
 constexpr int kPFD = 2;    // chunks of samples loaded ahead
 
-__global__ __launch_bounds__(256, 2) void k_dft_bf16(double2* __restrict__ acc, const float* __restrict__ samp,
+// s + c += x with Neumaier's compensation: c collects the rounding error of every addition into s
+// (exactly, by the two-sum of the larger and the smaller operand), so s + c keeps the sum of every x
+// added to within twice the FP64 rounding of the sum itself, whatever the number of additions.
+__device__ __forceinline__ void neumaier(double& s, double& c, double x)
+{
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+__global__ __launch_bounds__(256, 2) void k_dft_bf16(double2* __restrict__ acc, double2* __restrict__ comp,
+                                                  const float* __restrict__ samp,
                                                   const unsigned short* __restrict__ eh,
                                                   const unsigned short* __restrict__ em,
                                                   const unsigned short* __restrict__ el, long P, int K, int Jt,
@@ -219,44 +231,45 @@ __global__ __launch_bounds__(256, 2) void k_dft_bf16(double2* __restrict__ acc, 
             E[1] = En[1];
         }
     }
-    // rows come in (real, imaginary) pairs: column c0 + 32cb + row, row even. The accumulators of
-    // two node blocks are read together (clamped addresses) before any is written back.
+    // rows come in (real, imaginary) pairs: column c0 + 32cb + row, row even. The slab sums are
+    // exact in FP64 and go into the FP64 accumulators with Neumaier compensation (sum + its
+    // compensation term, both read for one node block, clamped addresses, before any is written
+    // back).
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-        for (int n2 = 0; n2 < kNB; n2 += 2) {
-            double2 v[2][8];
-            long o[2][8];
-            bool ok[2][8];
+        for (int nb = 0; nb < kNB; ++nb) {
+            double2 v[8], cv[8];
+            long o[8];
+            bool ok[8];
 #pragma unroll
-            for (int e = 0; e < 2; ++e)
+            for (int r = 0; r < 8; ++r) {
+                const long x = x0 + 32 * nb + col;
+                const int k = (c0 + 32 * cb + d_row(2 * r, lane)) >> 1;
+                ok[r] = x < P && k < K;
+                o[r] = ok[r] ? (long)k * P + x : 0;
+                v[r] = acc[o[r]];
+                cv[r] = comp[o[r]];
+            }
 #pragma unroll
-                for (int r = 0; r < 8; ++r) {
-                    const long x = x0 + 32 * (n2 + e) + col;
-                    const int k = (c0 + 32 * cb + d_row(2 * r, lane)) >> 1;
-                    ok[e][r] = x < P && k < K;
-                    o[e][r] = ok[e][r] ? (long)k * P + x : 0;
-                    v[e][r] = acc[o[e][r]];
+            for (int r = 0; r < 8; ++r)
+                if (ok[r]) {
+                    neumaier(v[r].x, cv[r].x, (double)d[cb][nb][2 * r]);
+                    neumaier(v[r].y, cv[r].y, (double)d[cb][nb][2 * r + 1]);
+                    acc[o[r]] = v[r];
+                    comp[o[r]] = cv[r];
                 }
-#pragma unroll
-            for (int e = 0; e < 2; ++e)
-#pragma unroll
-                for (int r = 0; r < 8; ++r)
-                    if (ok[e][r]) {
-                        v[e][r].x += (double)d[cb][n2 + e][2 * r];
-                        v[e][r].y += (double)d[cb][n2 + e][2 * r + 1];
-                        acc[o[e][r]] = v[e][r];
-                    }
         }
 }
 
-void launch_dft_bf16(double2* acc, const float* samp, const unsigned short* eh, const unsigned short* em,
+void launch_dft_bf16(double2* acc, double2* comp, const float* samp, const unsigned short* eh, const unsigned short* em,
                      const unsigned short* el, size_t P, int K, int C_pad, int Jt, int cnt, hipStream_t s)
 {
     if (cnt <= 0 || K <= 0) return;
     const unsigned nxb = (unsigned)((P + kFX - 1) / kFX);
     const unsigned ncg = (unsigned)(C_pad / kFC);
-    hipLaunchKernelGGL(k_dft_bf16, dim3(nxb, ncg), dim3(256), 0, s, acc, samp, eh, em, el, (long)P, K, Jt, cnt);
+    hipLaunchKernelGGL(k_dft_bf16, dim3(nxb, ncg), dim3(256), 0, s, acc, comp, samp, eh, em, el, (long)P, K, Jt,
+                       cnt);
 }
 
 // ---------------------------------------------------------------------------------------------
