@@ -11,7 +11,11 @@ which is the 'structure' branch of eval_objective_and_gradient.m:52-98.
 
 Prints one JSON line: GP-updates/s over the timed steps, with correlations+kernels/s, per-step
 latency p50/p99, the roofline of the dominant step kernel pair (measured with HIP events on the
-solver stream) and the FP64 CPU oracle timed on a bounded sample of the same chain.
+solver stream), the FP64 CPU oracle timed on a bounded sample of the same chain (on the host's cores
+and on one core), the relative L2 error of the GPU path on that same sample against the oracle (the
+process exits with status 1 when an output exceeds its tolerance), and the speed-up of the batch
+engine over the plain single-solve path on short runs of the batched-stations and serving workloads
+(stations_vs_plain, served_vs_plain).
 """
 from __future__ import annotations
 
@@ -58,8 +62,9 @@ def parse():
     ap.add_argument("--plain", type=int, default=8, help="stations timed on the plain single-solve path")
     ap.add_argument("--rounds", type=int, default=64, help="engine rounds between completion checks")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: up to 16 host cores)")
-    ap.add_argument("--cpu-nt", type=int, default=16, help="nt of the CPU-baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-nt", type=int, default=32, help="nt of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the accuracy check")
+    ap.add_argument("--no-side", action="store_true", help="skip the short stations and serving runs")
     ap.add_argument("--timing-every", type=int, default=31,
                     help="time every n-th launch (31: coprime with fw_nth, so kernel steps enter in proportion; "
                          "each timed launch adds ~6 us of event overhead to the run)")
@@ -190,32 +195,104 @@ def measured_traffic(kernel, c):
     return t.get(kernel)
 
 
-def cpu_baseline(w, nt_sample, threads):
-    """The FP64 oracle on the same chain at the full grid, truncated to nt_sample Green steps."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _near_receivers(w, n=8, rmin=2e3, rmax=6e3, seed=22):
+    """Receivers within reach of the waves of a truncated run (2-6 km from the station)."""
+    rng = np.random.default_rng(seed)
+    ang, r = rng.uniform(0, 2 * np.pi, n), rng.uniform(rmin, rmax, n)
+    c0 = w["src"][0]
+    return np.stack([c0[0] + r * np.cos(ang), c0[1] + r * np.sin(ang)], axis=1)
+
+
+def _chain(solvers, c, w, rec, stf):
+    """Green (mode 1) + correlation (mode 1) + adjoint mode 1 (point sources at the receivers) +
+    adjoint mode 0 (its stf_fft as distributed source): the benchmark's chain."""
+    G, Gout, _ = solvers.run_forward1_green(w["mu"], w["rho"], w["src"], rec, 1, config=c)
+    corr, C = solvers.run_forward2_correlation(w["mu"], w["rho"], G, w["spectrum"], w["source_dist"], rec, 1, config=c)
+    K1, sf, _, _ = solvers.run_noise_adjoint(w["mu"], w["rho"], C, stf, rec, rec, None, None, None, 1, config=c)
+    K0, _, _, _ = solvers.run_noise_adjoint(w["mu"], w["rho"], Gout, sf, rec, rec, w["spectrum"], w["source_dist"],
+                                            None, 0, config=c)
+    return {"G_fft": G, "correlation": corr, "C_out": C, "stf_fft": sf, "K_mu_adjoint1": K1[:, :, -1],
+            "K_mu_adjoint0": K0[:, :, -1]}
+
+
+def _rel(a, b, chunk=1 << 24):
+    a, b = np.asarray(a).reshape(-1, order="F"), np.asarray(b).reshape(-1, order="F")
+    num = den = 0.0
+    for q in range(0, b.size, chunk):
+        d = a[q:q + chunk] - b[q:q + chunk]
+        num += float(np.vdot(d, d).real)
+        den += float(np.vdot(b[q:q + chunk], b[q:q + chunk]).real)
+    return float(np.sqrt(num / den)) if den > 0 else float(np.sqrt(num))
+
+
+TOL = {"correlation": 1e-5, "G_fft": 1e-5, "C_out": 1e-5, "stf_fft": 1e-5, "K_mu_adjoint1": 1e-4,
+       "K_mu_adjoint0": 1e-4}
+
+
+def cpu_baseline_and_accuracy(w, nt_sample, threads, crop=1000):
+    """The FP64 oracle timed on the benchmark chain at the full grid, truncated to nt_sample Green
+    steps (on `threads` host cores), and on a crop x crop window of the same model on one core; the
+    GPU path then runs the full-grid sample with identical inputs and every output is compared."""
+    import fd2d_noise_amd as fd
     import oracle
     c = w["config"].replace(nt=nt_sample)
     c.absorb_width = w["config"].absorb_width
+    rec = np.concatenate([w["rec"], _near_receivers(w)], axis=0)
+    stf = np.random.default_rng(0).standard_normal((rec.shape[0], c.n_corr)).astype(np.complex128)
     orc = oracle.solvers()
     oracle.set_threads(threads)
-    rng = np.random.default_rng(0)
     t0 = time.perf_counter()
-    G, Gout, _ = orc.run_forward1_green(w["mu"], w["rho"], w["src"], w["rec"], 1, config=c)
-    corr, C = orc.run_forward2_correlation(w["mu"], w["rho"], G, w["spectrum"], w["source_dist"], w["rec"], 1,
-                                           config=c)
-    stf = rng.standard_normal(corr.shape).astype(np.complex128)
-    _, sf, _, _ = orc.run_noise_adjoint(w["mu"], w["rho"], C, stf, w["rec"], w["rec"], None, None, None, 1, config=c)
-    orc.run_noise_adjoint(w["mu"], w["rho"], Gout, sf, w["rec"], w["rec"], w["spectrum"], w["source_dist"], None, 0,
-                          config=c)
+    ref = _chain(orc, c, w, rec, stf)
     el = time.perf_counter() - t0
     gp = c.nx * c.nz * (2 * c.nt + 2 * c.n_corr)
-    return {"value": gp / el, "unit": "GP-updates/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, Green+correlation+adjoint(1)+adjoint(0) chain on the full "
-                      f"{c.nx}x{c.nz} order-{c.order} grid truncated to nt={nt_sample} "
-                      f"({2 * nt_sample + 2 * c.n_corr} steps, {gp:.3g} GP-updates, synthetic adjoint "
-                      f"sources), {el:.1f} s"}
+    print(f"# cpu baseline: {el:.1f} s on {threads} threads", file=sys.stderr, flush=True)
+    # one core (the reference's jobs run matlab -singleCompThread): a crop of the same model
+    lo = (c.nx - crop) // 2
+    wc = dict(w)
+    cc = c.replace(nx=crop, nz=crop, Lx=c.dx * (crop - 1), Lz=c.dx * (crop - 1))
+    cc.absorb_width = c.absorb_width
+    for k in ("mu", "rho", "source_dist"):
+        wc[k] = np.asfortranarray(w[k][lo:lo + crop, lo:lo + crop])
+    wc["src"] = w["src"] - lo * c.dx
+    rec_c = _near_receivers(wc)
+    oracle.set_threads(1)
+    t1 = time.perf_counter()
+    _chain(orc, cc, wc, rec_c, stf[:rec_c.shape[0]])
+    el1 = time.perf_counter() - t1
+    gp1 = crop * crop * (2 * c.nt + 2 * c.n_corr)
+    print(f"# cpu baseline, one core: {el1:.1f} s", file=sys.stderr, flush=True)
+    cpu = {"value": gp / el, "unit": "GP-updates/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+           "sample": f"FP64 oracle, Green+correlation+adjoint(1)+adjoint(0) chain on the full {c.nx}x{c.nz} "
+                     f"order-{c.order} grid truncated to nt={nt_sample} ({2 * nt_sample + 2 * c.n_corr} steps, "
+                     f"{gp:.3g} GP-updates, seeded synthetic adjoint sources at {rec.shape[0]} receivers), {el:.1f} s",
+           "single_core": {"value": gp1 / el1, "unit": "GP-updates/s", "cores": 1,
+                           "sample": f"the same chain on the central {crop}x{crop} window of the model "
+                                     f"({gp1:.3g} GP-updates), {el1:.1f} s"}}
+    t2 = time.perf_counter()
+    got = _chain(fd, c, w, rec, stf)
+    errs = {k: _rel(got[k], ref[k]) for k in ref}
+    ok = all(errs[k] <= TOL[k] for k in errs)
+    print(f"# accuracy of the GPU path on the same sample ({time.perf_counter() - t2:.1f} s): "
+          + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), file=sys.stderr, flush=True)
+    acc = {"sample": f"the CPU-baseline chain (full grid, nt={nt_sample}, {rec.shape[0]} receivers of which 8 "
+                     "within 2-6 km of the station, where the truncated run's waves reach)",
+           "rel_l2_vs_fp64_oracle": {k: float(f"{v:.3e}") for k, v in errs.items()},
+           "tolerance": TOL, "within_tolerance": ok}
+    return cpu, acc
 
 
-def stations_bench(args):
+def stations_run(args):
     """Green + correlation per reference station on the shared 1200x1200 grid: the plain path (one
     solve at a time) against the batch engine with `window` stations in flight, each station's
     correlation submitted as soon as its Green run is done. Checks that the two give identical
@@ -299,10 +376,14 @@ def stations_bench(args):
                                "note": "algorithmic step bytes (32 B Green, 40 B correlation per GP-update) "
                                        "over the whole job's wall time, spectral work included"},
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
-def serving_bench(args):
+def stations_bench(args):
+    print(json.dumps(stations_run(args)), flush=True)
+
+
+def serving_run(args):
     """The serving workload at each arrival rate: throughput (requests/s, GP-updates/s) and p50/p99
     latency (arrival to result); then the requests of the highest rate run one solve at a time
     through the single-solve entry points (the plain path), and every result is compared."""
@@ -377,7 +458,11 @@ def serving_bench(args):
         "plain_req_s": len(last) / t_plain, "served_vs_plain": t_plain / top["wall_s"],
         "identical_to_plain": bool(same),
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def serving_bench(args):
+    print(json.dumps(serving_run(args)), flush=True)
 
 
 def inversion_bench(args):
@@ -582,11 +667,39 @@ def main():
             "kernel": f"{kdesc} ({best} run, {steps_per_launch:.2f} time steps per launch)",
             "avg_launch_us": round(launch_ms * 1e3, 2), "steps_per_launch": round(steps_per_launch, 3),
             "alg_bytes_per_gp": alg_bytes(best, c.fw_nth, best == "adjoint")}
-    cpu = None
+    lat_ms = np.array(lat) * 1e3
+    n_rec, gp_step = p.nr, p.gp_updates()
+    # free the pipeline's device memory for the runs below
+    del p
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    _lib.load().fd2d_release_cached_memory()
+    side = {}
+    if not args.no_side:
+        import copy
+        a = copy.copy(args)
+        a.nt, a.stations, a.window, a.plain = None, 64, 8, 4
+        st = stations_run(a)
+        print(f"# stations: {st['batched_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
+        a.rates, a.requests = "200", 24
+        sv = serving_run(a)
+        print(f"# serving: {sv['served_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
+        side = {"stations_vs_plain": round(st["batched_vs_plain"], 3),
+                "served_vs_plain": round(sv["served_vs_plain"], 3),
+                "batched_runs": {
+                    "stations": {"workload": st["config"]["workload"], "correlations_per_s": st["correlations_per_s"],
+                                 "plain_correlations_per_s": st["plain_correlations_per_s"],
+                                 "identical_to_plain": st["identical_to_plain"],
+                                 "latency_ms": st["latency_ms"], "roofline_whole_job": st["roofline_whole_job"]},
+                    "serving": {"workload": sv["config"]["workload"], "rate_req_s": sv["runs"][-1]["rate_req_s"],
+                                "requests": sv["runs"][-1]["requests"],
+                                "throughput_req_s": sv["runs"][-1]["throughput_req_s"],
+                                "plain_req_s": sv["plain_req_s"], "latency_ms": sv["runs"][-1]["latency_ms"],
+                                "identical_to_plain": sv["identical_to_plain"]}}}
+    cpu, acc = None, None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(w, args.cpu_nt, threads)
-    lat_ms = np.array(lat) * 1e3
+        cpu, acc = cpu_baseline_and_accuracy(w, args.cpu_nt, threads)
     out = {
         "metric": "Gpt-updates/s, correlations+kernels/s, p99 latency; FP32 vs plain, L2<=1e-5",
         "value": gp / el, "unit": "GP-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -595,13 +708,17 @@ def main():
         "config": {"workload": "config2: 2400x2400 order-8 nt=8000 Green+correlation+cc-traveltime adjoint "
                                "structure kernel, fw_nth=4, 121 frequencies, 16 receivers",
                    "nx": c.nx, "nz": c.nz, "order": c.order, "nt": c.nt, "fw_nth": c.fw_nth,
-                   "n_sample": c.n_sample, "receivers": p.nr, "gp_updates_per_step": p.gp_updates()},
+                   "n_sample": c.n_sample, "receivers": n_rec, "gp_updates_per_step": gp_step},
         "correlations_kernels_per_s": args.steps / el,
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "accuracy": acc,
     }
+    out.update(side)
     print(json.dumps(out), flush=True)
+    if acc is not None and not acc["within_tolerance"]:
+        sys.exit(1)
 
 
 if __name__ == "__main__":
