@@ -419,11 +419,16 @@ struct Solver {
     Geometry g;
     hipStream_t st;
     bool with_u;
+    // displacement form (correlation / adjoint runs at FD order 6 or 8 without Born terms): the pool
+    // holds v (FP32) and u (FP64) only, sigma is evaluated from u inside the step
+    bool uform;
     PaddedLayout lay;
-    DBuf<float> fields;  // two sets of v, sxy, szy (, u)
+    DBuf<float> fields;  // two sets of v, sxy, szy (, u); displacement form: v0, v1, then u0, u1 in FP64
     DBuf<float> coefs;   // bv, dtmu
     DBuf<float> ax_d, az_d, axp_d, azp_d;
     float *v[2], *sxy[2], *szy[2], *u[2];
+    double* u64[2] = {nullptr, nullptr};  // displacement form: origin pointers of the FP64 u arrays
+    int o_u64[2] = {0, 0};                // ... and their FP32-element offsets in the pool
     int cur = 0;
     int tiles_x = 0, tiles_z = 0, n_tiles = 0;
     DBuf<int> ps_off_d, rec_off_d;
@@ -431,14 +436,16 @@ struct Solver {
     StepArgs a{};
     PairArgs pr{};  // the second step of a two-step launch
 
-    Solver(const fd2d_config* c, const Geometry& geo, hipStream_t s, bool with_u_) : cfg(c), g(geo), st(s), with_u(with_u_)
+    Solver(const fd2d_config* c, const Geometry& geo, hipStream_t s, bool with_u_, bool uform_ = false)
+        : cfg(c), g(geo), st(s), with_u(with_u_), uform(uform_)
     {
         lay = PaddedLayout::make(g.nx, g.nz);
         const size_t E = lay.elems(), o = lay.origin();
         // buffer offsets are 32-bit byte offsets into the field pool (two sets of nf arrays)
         if (E * sizeof(float) * 8 >= ((size_t)1 << 32))
             raise(FD2D_EUNSUPPORTED, "nx x nz = %d x %d: the padded fields exceed 4 GiB", g.nx, g.nz);
-        const int nf = with_u ? 4 : 3;
+        // FP32 arrays per set: v, sxy, szy (, u); displacement form: v and the two halves of FP64 u
+        const int nf = uform ? 3 : with_u ? 4 : 3;
         fields.alloc(E * nf * 2);
         HIPCHK(hipMemsetAsync(fields.p, 0, E * nf * 2 * sizeof(float), st));
         coefs.alloc(E * 2);
@@ -453,6 +460,13 @@ struct Solver {
         upload(axp_d, axp, st);
         upload(azp_d, azp, st);
         for (int b = 0; b < 2; ++b) {
+            if (uform) {
+                v[b] = fields.p + (size_t)b * E + o;
+                sxy[b] = szy[b] = u[b] = nullptr;
+                o_u64[b] = (int)(2 * E + (size_t)b * 2 * E);
+                u64[b] = (double*)(fields.p + o_u64[b]) + o;
+                continue;
+            }
             float* base = fields.p + (size_t)b * nf * E;
             v[b] = base + o;
             sxy[b] = base + E + o;
@@ -488,12 +502,17 @@ struct Solver {
             a.cz[m] = (float)(kCoef[g.R][m] / g.dz);
         }
         a.dt = (float)g.dt;
+        a.dt64 = g.dt;
     }
+    // dt/rho and dt*mu (velocity-stress form) or mu (displacement form)
     void material(const double* mu, const double* rho)
     {
         launch_material(mu, rho, coefs.p + lay.origin(), coefs.p + lay.elems() + lay.origin(), g.nx, g.nz,
-                        lay.pitch, g.dt, st);
+                        lay.pitch, g.dt, uform ? 1.0 : g.dt, st);
     }
+    // the FP32 part of the pool a finiteness check reads (the FP64 halves of u are not FP32 values)
+    const float* check_ptr() const { return fields.p; }
+    size_t check_count() const { return uform ? 2 * lay.elems() : fields.n; }
     // Point sources as per-tile lists over every node whose velocity the tile computes in a one- or
     // two-step launch (the tile plus its (3R-2, 3R-1) halo), in ascending source order so every
     // copy of a halo node sums identically.
@@ -554,6 +573,10 @@ struct Solver {
         a.o_sxy_out = off(a.sxy_out);
         a.o_szy_out = off(a.szy_out);
         a.o_u_out = off(a.u_out);
+        if (uform) {
+            a.o_u_in = o_u64[cur];
+            a.o_u_out = o_u64[1 - cur];
+        }
         a.ps_off = nullptr;
         a.ps_ent = nullptr;
         a.ps_amp = nullptr;
@@ -607,7 +630,8 @@ struct Solver {
     }
     void step()
     {
-        launch_step(g.R, with_u, a, st);
+        if (uform) launch_ustep(g.R, a, st);
+        else launch_step(g.R, with_u, a, st);
         cur ^= 1;
     }
     // steps n and n+1 in one launch (the state buffers swap once, as after one step)
@@ -618,7 +642,11 @@ struct Solver {
         cur ^= 1;
     }
     // structure-kernel product with the displacement the last step produced
-    void kmu(const float* fwd, double* kmu_acc, double scale) { launch_kmu(g.R, a, u[cur], fwd, kmu_acc, scale, st); }
+    void kmu(const float* fwd, double* kmu_acc, double scale)
+    {
+        if (uform) launch_ukmu(g.R, a, u64[cur], fwd, kmu_acc, scale, st);
+        else launch_kmu(g.R, a, u[cur], fwd, kmu_acc, scale, st);
+    }
 };
 
 // Blocked frequency-domain helpers: time samples are staged in FP32 slabs of up to kSpecSlab slices
@@ -939,6 +967,11 @@ int64_t green_n_fw(const Geometry& g)
     return n;
 }
 
+// Correlation and adjoint runs use the displacement form at FD orders 6 and 8 (its 32 B per step
+// against 40 B; at orders 2 and 4 two-step launches of the velocity-stress form move fewer bytes
+// still) and when there are no Born terms (whose stress terms have no displacement counterpart).
+bool displacement_form(const Geometry& g, bool born) { return g.R >= 3 && !born; }
+
 // =============================================================================================
 // Solves as step sequences. A Job holds one solve's set-up (the prologue of the *_device entry
 // points), fills the per-step fields of step n, does the work after a launch and finishes the
@@ -996,6 +1029,9 @@ std::map<std::array<int, 4>, int> g_tuned;  // (nx, nz, R, with_u) -> steps per 
 
 std::array<int, 4> shape_key(const Job& j) { return {j.g.nx, j.g.nz, j.g.R, j.S->with_u ? 1 : 0}; }
 
+// field set of a solve: 0 Green (v, sigma), 1 with displacement (v, sigma, u), 2 displacement form (v, FP64 u)
+int field_kind(const Job& j) { return j.S->uform ? 2 : j.S->with_u ? 1 : 0; }
+
 int tuned_steps(const Job& j)
 {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1013,7 +1049,7 @@ void record_tuned(const Job& j, int steps)
 int launch_steps(const Job& j)
 {
     const int mode = g_steps_per_launch.load();
-    if (j.born) return 1;
+    if (j.born || j.S->uform) return 1;
     if (mode == 1 || mode == 2) return mode;
     const int t = tuned_steps(j);
     if (t) return t;
@@ -1074,7 +1110,7 @@ void run_job(Job& j)
     j.finish();
     // the outputs are written either way; a NaN or Inf left in the fields is reported
     DBuf<int> chk(kCheckBlocks + 1);
-    launch_nonfinite(j.S->fields.p, j.S->fields.n, chk.p, chk.p + kCheckBlocks, j.st);
+    launch_nonfinite(j.S->check_ptr(), j.S->check_count(), chk.p, chk.p + kCheckBlocks, j.st);
     int bad = 0;
     HIPCHK(hipMemcpyAsync(&bad, chk.p + kCheckBlocks, sizeof bad, hipMemcpyDeviceToHost, j.st));
     HIPCHK(hipGetLastError());
@@ -1224,7 +1260,7 @@ struct CorrelationJob : Job {
         records = sg && nr;
         nsteps = N;
 
-        S.reset(new Solver(cfg, g, st, true));
+        S.reset(new Solver(cfg, g, st, true, displacement_form(g, born)));
         S->material((const double*)mu->data, (const double*)rho->data);
         S->set_receivers(rid);
         if (records) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * N, st));
@@ -1288,6 +1324,9 @@ struct AdjointJob : Job {
     DftAccumulator dft;
     DBuf<float> dmu_f;
     const float* kfwd[2] = {nullptr, nullptr};
+    // displacement form: the forward snapshot of the product of the last planned step, which the
+    // next launch computes from its input displacement (or finish(), after the last step)
+    const float* deferred = nullptr;
 
     AdjointJob(const fd2d_config* c, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* u_fwd_,
                const fd2d_carray* stf, const fd2d_carray* src, const fd2d_carray* rec, const fd2d_carray* spectrum,
@@ -1356,7 +1395,7 @@ struct AdjointJob : Job {
         sg = (seis && seis->data) ? (double*)seis->data : nullptr;
         records = sg && nr;
 
-        S.reset(new Solver(cfg, g, st, true));
+        S.reset(new Solver(cfg, g, st, true, displacement_form(g, born)));
         S->material((const double*)mu->data, (const double*)rho->data);
         if (Kd) {
             S->a.kacc = Kd + (size_t)nb_cols * g.P;  // K_mu (:359-367)
@@ -1427,9 +1466,14 @@ struct AdjointJob : Job {
             S->a.born_in1 = (const float*)u_in->data + (size_t)n * g.P;
         }
         kfwd[sub] = nullptr;
+        if (S->uform) S->a.kfwd = deferred;  // the previous step's product, in this launch
         if (kernel_step(n)) {  // :324-331
             kfwd[sub] = (const float*)u_fwd->data + (size_t)(n_ufwd - 1 - i_fw_in) * g.P;
             ++i_fw_in;
+        }
+        if (S->uform) {
+            deferred = kfwd[sub];
+            kfwd[sub] = nullptr;
         }
         if (sampled(n)) *sl.dsamp = dft.next_slot();
         if (records) S->receivers(sub, sg + (size_t)nr * (n - 1));
@@ -1442,6 +1486,7 @@ struct AdjointJob : Job {
     }
     void fuse_products() override
     {
+        if (S->uform) return;  // plan() has set the deferred product
         S->a.kfwd = kfwd[0];
         kfwd[0] = nullptr;
     }
@@ -1454,6 +1499,8 @@ struct AdjointJob : Job {
     }
     void finish() override
     {
+        if (deferred) S->kmu(deferred, Kd + (size_t)nb_cols * g.P, (double)g.f);  // the last step's product
+        deferred = nullptr;
         if (!need_dft) return;
         dft.flush();
         if (src_kernel && Kd) {
@@ -1586,7 +1633,8 @@ constexpr int kSoloTiles = 512;
 constexpr double kLastLevelCacheBytes = 256.0 * (1 << 20);
 // fields and coefficients one step reads and writes: 32 B per node without the displacement field
 // (Green runs), 40 B with it
-double step_bytes(const Job& j) { return (double)j.g.P * (j.S->with_u ? 40.0 : 32.0); }
+// (the displacement form moves 32 B: v and the FP64 u)
+double step_bytes(const Job& j) { return (double)j.g.P * (field_kind(j) == 1 ? 40.0 : 32.0); }
 
 // Steps between the periodic finiteness checks of an engine's solve (each reads the solve's field
 // pool once: 1/kCheckEvery of a step's traffic).
@@ -1615,7 +1663,7 @@ struct Engine {
     std::vector<int> free_slots;
     struct Group {
         int R = 0;
-        bool with_u = false;
+        int kind = 0;  // field_kind of its members
         std::vector<Entry*> members;
     };
     std::vector<Group> groups;
@@ -1635,7 +1683,7 @@ struct Engine {
     void queue_check(Entry& e)
     {
         const Solver& S = *e.job->S;
-        launch_nonfinite(S.fields.p, S.fields.n, chk_partial.p, chk_flag_d.p + e.slot, st);
+        launch_nonfinite(S.check_ptr(), S.check_count(), chk_partial.p, chk_flag_d.p + e.slot, st);
         HIPCHK(hipMemcpyAsync(chk_flag_h + e.slot, chk_flag_d.p + e.slot, sizeof(int), hipMemcpyDeviceToHost, st));
         if (!e.chk_ev) HIPCHK(hipEventCreateWithFlags(&e.chk_ev, hipEventDisableTiming));
         HIPCHK(hipEventRecord(e.chk_ev, st));
@@ -1695,20 +1743,20 @@ struct Engine {
             if (j.S->n_tiles >= kSoloTiles) {
                 const double b = step_bytes(j);
                 if (big.empty() || (big_bytes + b <= kLastLevelCacheBytes && big[0]->job->g.R == j.g.R &&
-                                    big[0]->job->S->with_u == j.S->with_u && !j.born && !big[0]->job->born)) {
+                                    field_kind(*big[0]->job) == field_kind(j) && !j.born && !big[0]->job->born)) {
                     big.push_back(e.get());
                     big_bytes += b;
                 }
                 continue;
             }
-            cls.push_back({{j.g.R, j.S->with_u ? 1 : 0}, e.get()});
+            cls.push_back({{j.g.R, field_kind(j)}, e.get()});
         }
         if (big.size() == 1) {
             singles.push_back(big[0]);
         } else if (big.size() > 1) {
             Group gr;
             gr.R = big[0]->job->g.R;
-            gr.with_u = big[0]->job->S->with_u;
+            gr.kind = field_kind(*big[0]->job);
             gr.members = big;
             groups.push_back(std::move(gr));
         }
@@ -1722,7 +1770,7 @@ struct Engine {
             } else {
                 Group gr;
                 gr.R = cls[q].first.first;
-                gr.with_u = cls[q].first.second != 0;
+                gr.kind = cls[q].first.second;
                 for (size_t k = q; k < r; ++k) gr.members.push_back(cls[k].second);
                 groups.push_back(std::move(gr));
             }
@@ -1753,6 +1801,7 @@ struct Engine {
     int shape_of(const Job& j) const
     {
         const int mode = g_steps_per_launch.load();
+        if (j.S->uform) return 1;  // the displacement form has single-step launches only
         if (mode == 1 || mode == 2) return mode;
         const int t = tuned_steps(j);
         return t ? t : (j.g.R <= 2 ? 2 : 1);
@@ -1794,7 +1843,8 @@ struct Engine {
         ga.tile_start[ga.n_jobs] = total;
         ga.total_tiles = total;
         ga.tiles_per_xcd = (total + kXCD - 1) / kXCD;
-        launch_group(gr.R, gr.with_u, ga, st);
+        if (gr.kind == 2) launch_ugroup(gr.R, ga, st);
+        else launch_group(gr.R, gr.kind == 1, ga, st);
         for (int q = 0; q < ga.n_jobs; ++q) {
             Job& j = *members[q]->job;
             j.S->cur ^= 1;
@@ -1824,7 +1874,7 @@ struct Engine {
         ga.tile_start[ga.n_jobs] = total;
         ga.total_tiles = total;
         ga.tiles_per_xcd = (total + kXCD - 1) / kXCD;
-        launch_pair_group(gr.R, gr.with_u, ga, st);
+        launch_pair_group(gr.R, gr.kind == 1, ga, st);
         for (int q = 0; q < ga.n_jobs; ++q) {
             Job& j = *members[q]->job;
             j.S->cur ^= 1;  // a pair swaps the state buffers once, as one step does
@@ -1974,7 +2024,9 @@ int64_t solve_bytes(const fd2d_solve* s)
     const PaddedLayout lay = PaddedLayout::make(g.nx, g.nz);
     const int64_t P = (int64_t)g.P, E = (int64_t)lay.elems(), K = g.K, C_pad = spec_cols(g.K);
     const bool with_u = s->solver != FD2D_GREEN;
-    int64_t b = E * 4 * (with_u ? 4 : 3) * 2 + E * 4 * 2 + 4 * (int64_t)(lay.pitch + lay.rows + g.nx + g.nz);
+    // field pool: two sets of v, sigma_xy, sigma_zy (, u), or of v and the FP64 u (displacement form)
+    const bool uform = with_u && displacement_form(g, numel(s->dmu) != 0);
+    int64_t b = E * 4 * (uform ? 3 : with_u ? 4 : 3) * 2 + E * 4 * 2 + 4 * (int64_t)(lay.pitch + lay.rows + g.nx + g.nz);
     const int64_t n_src = numel(s->src) ? dim(s->src, 0) : 0, n_rec = numel(s->rec) ? dim(s->rec, 0) : 0;
     b += (int64_t)(g.nx / kTX + 2) * (g.nz / kTileZ + 2) * 8 + 8 * 9 * (n_src + n_rec);  // tile lists
     const int N = 2 * g.nt - 1;

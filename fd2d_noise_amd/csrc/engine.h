@@ -108,6 +108,7 @@ struct StepArgs {
     const float* kfwd;       // forward snapshot of this step (compact), or null
     double* kacc;            // K_mu accumulator (compact, FP64)
     double kscale;           // fw_nth
+    double dt64;             // dt in FP64 (u_n = u_{n-1} + dt v_n of the displacement-form step)
 };
 
 inline int node_key_host(int x, int z) { return ((z + 8) << 16) | (x + 8); }
@@ -202,9 +203,20 @@ void launch_kmu(int R, const StepArgs& a, const float* u, const float* fwd, doub
 void launch_born_green(int R, const StepArgs& a, float* sxy, float* szy, const float* dtdmu, const float* gin,
                        hipStream_t s);
 
-// Material coefficients: bv = dt/rho, dtmu = dt*mu into padded arrays.
+// Material coefficients: bv = dt/rho, dtmu = mu_scale*mu (dt*mu for the velocity-stress step, mu for the
+// displacement-form step) into padded arrays.
 void launch_material(const double* mu, const double* rho, float* bv, float* dtmu, int nx, int nz,
-                     int pitch, double dt, hipStream_t s);
+                     int pitch, double dt, double mu_scale, hipStream_t s);
+
+// Displacement-form steps (correlation and adjoint runs without Born terms, R = 3 or 4): the field
+// pool holds v (FP32) and u (FP64) only, o_u_in / o_u_out are FP32-element offsets of the FP64
+// arrays, o_dtmu holds mu. a.kfwd (or step[j].kfwd) is the forward snapshot of the PREVIOUS step's
+// structure-kernel product, which these launches compute from their staged input displacement.
+void launch_ustep(int R, const StepArgs& a, hipStream_t s);
+void launch_ugroup(int R, const GroupArgs& g, hipStream_t s);
+// the product of the last step of a displacement-form run, from its FP64 displacement u (origin pointer)
+void launch_ukmu(int R, const StepArgs& a, const double* u, const float* fwd, double* kmu, double scale,
+                 hipStream_t s);
 void launch_scale_to_float(const double* in, float* out, size_t n, double scale, hipStream_t s);
 
 // acc[k][x] + comp[k][x] += sum_j samp[j][x] * e[j][k] over cnt samples: matrix-core products of

@@ -757,6 +757,315 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_group(PairGroupArgs G)
 }
 
 // Structure-kernel product of run_noise_adjoint.m:324-331 after a step that stores it:
+// ---------------------------------------------------------------------------------------------
+// Displacement-form step (correlation and adjoint runs without Born terms, FD orders 6 and 8)
+// ---------------------------------------------------------------------------------------------
+// In the correlation and adjoint runs sigma and u both start at zero and are advanced by the same
+// velocity, sigma_n = sigma_{n-1} + dt mu D v_n and u_n = u_{n-1} + dt v_n
+// (code/run_forward2_correlation.m:220-242, code/run_noise_adjoint.m:277-295), so sigma_n = mu D u_n
+// at every step, D being linear. This step keeps v (FP32) and u (FP64) only and evaluates sigma from
+// u where the divergence needs it: 32 B per node and step instead of the 40 B of v, sigma_xy,
+// sigma_zy and u, and sigma carries the rounding of one FP32 evaluation from the FP64 differences of
+// u instead of that of a running FP32 sum. One 512-thread workgroup per 64 x 32 tile:
+//   phase 1 stages u (FP64) over the tile rows with a (2R-1)-column halo and over the tile columns
+//           with a (2R-1)-row halo (the corners are not needed), and loads v, dt/rho, mu, the source
+//           slice and, for a product, the forward snapshot (tile + (R-1, R) halo) and K_mu;
+//   phase 2 sigma_xy = mu dx_v(u) on the tile rows over columns i0-R .. i0+63+R-1 and
+//           sigma_zy = mu dz_v(u) on the tile columns over rows j0-R .. j0+31+R-1 (LDS), with the
+//           zero frames of dx_v / dz_v;
+//   phase 3 the divergence (frames of div_s), point sources and source slice, v_n with the taper,
+//           u_n = u_{n-1} + dt v_n in FP64, the snapshot (u before the update), DFT sample and
+//           receivers (u after it);
+//   phase 4 (MODE 1) the structure-kernel product of the PREVIOUS step, K_mu -= f (D u_{n-1} . D fwd)
+//           (run_noise_adjoint.m:324-331): a product step's displacement is the next step's staged
+//           input, so the product needs no halo work of its own (the last step's product, if any,
+//           runs in k_ukmu).
+__device__ __forceinline__ double bload8(__amdgpu_buffer_rsrc_t r, unsigned off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void bstore8(__amdgpu_buffer_rsrc_t r, unsigned off, double v)
+{
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)off, 0, 0);
+}
+// byte offset of (xx, zz) in a padded FP64 array starting at FP32 element `arr` of the field pool
+__device__ __forceinline__ unsigned pofs8(int arr, int pitch, int xx, int zz)
+{
+    return (unsigned)arr * 4u + (unsigned)((zz + kGhost) * pitch + xx + kGhostX) * 8u;
+}
+
+template <int R, int MODE>
+__device__ __forceinline__ void utile_step(const StepArgs& a, const JobStep& st, int t)
+{
+    constexpr int TZ = kTileZ;
+    constexpr int TT = kTileThreads, TW = TT / 64;
+    constexpr int H = 2 * R - 1;                     // u halo
+    constexpr int WU = kTX + 2 * H, HU = TZ + 2 * H;  // u region: columns i0-H .., rows j0-H ..
+    constexpr int WX = kTX + 2 * R;                  // sigma_xy: columns i0-R .. i0+63+R-1, tile rows
+    constexpr int HZ = TZ + 2 * R;                   // sigma_zy: rows j0-R .. j0+TZ-1+R-1, tile columns
+    constexpr int WF = kTX + 2 * R - 1, HF = TZ + 2 * R - 1;  // forward snapshot: (R-1, R) halo
+    constexpr int HCF = 2 * R - 1;                   // its halo columns
+    __shared__ double uu[HU * WU + 1];               // + 1: the sink of the packed lists' idle lanes
+    __shared__ float sxy[TZ * WX + 1];
+    __shared__ float szy[HZ * kTX];
+    __shared__ float ff[MODE == 1 ? HF * WF + 1 : 1];
+
+    if (t >= a.n_tiles) return;
+    const int i0 = (t % a.tiles_x) * kTX, j0 = (t / a.tiles_x) * TZ;
+    const int nx = a.nx, nz = a.nz, pitch = a.pitch;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int x = i0 + lane;
+    const Grid G{nx, nz, pitch, rsrc(a.fbase, a.fbytes), rsrc(a.cbase, a.cbytes)};
+    const unsigned cb = a.compact_bytes;
+    const bool has_syn = st.syn != nullptr;
+    const auto r_syn = rsrc(has_syn ? (const void*)st.syn : (const void*)a.fbase, has_syn ? cb : 0u);
+    const bool kp = MODE == 1 && st.kfwd != nullptr;
+
+    // ---------------- phase 1
+    constexpr int NRU = (HU + TW - 1) / TW;             // u column block rows per wavefront
+    constexpr int NHU = (TZ * 2 * H + TT - 1) / TT;     // u halo columns on the tile rows (packed)
+    constexpr int NRZ = (HZ + TW - 1) / TW;             // mu over the sigma_zy rows (tile columns)
+    constexpr int NHM = (TZ * 2 * R + TT - 1) / TT;     // mu at the sigma_xy halo columns (packed)
+    constexpr int NT = TZ / TW;
+    double cu[NRU], hu[NHU];
+    float mz[NRZ], hm[NHM];
+    float mv[NT], mb[NT], msy[NT];
+#pragma unroll
+    for (int k = 0; k < NRU; ++k) cu[k] = bload8(G.rf, pofs8(a.o_u_in, pitch, x, j0 - H + w + TW * k));
+#pragma unroll
+    for (int k = 0; k < NHU; ++k) {
+        const int idx = min(tid + TT * k, TZ * 2 * H - 1);
+        const int r = idx / (2 * H), h = idx - r * 2 * H;
+        const int c = h < H ? h : h + kTX;  // u region column
+        hu[k] = bload8(G.rf, pofs8(a.o_u_in, pitch, i0 - H + c, j0 + r));
+    }
+#pragma unroll
+    for (int k = 0; k < NRZ; ++k) mz[k] = G.coef(a.o_dtmu, x, j0 - R + w + TW * k);
+#pragma unroll
+    for (int k = 0; k < NHM; ++k) {
+        const int idx = min(tid + TT * k, TZ * 2 * R - 1);
+        const int r = idx / (2 * R), h = idx - r * 2 * R;
+        hm[k] = G.coef(a.o_dtmu, i0 - R + (h < R ? h : h + kTX), j0 + r);
+    }
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        const int z = j0 + w + TW * k;
+        mv[k] = G.field(a.o_v_in, x, z);
+        mb[k] = G.coef(a.o_bv, x, z);
+        msy[k] = has_syn ? bload(r_syn, G.cofs(x, z)) : 0.f;
+    }
+    constexpr int NRF = (HF + TW - 1) / TW, NHF = (HF * HCF + TT - 1) / TT;
+    float fm[MODE == 1 ? NRF : 1], fh[MODE == 1 ? NHF : 1];
+    double kold[MODE == 1 ? NT : 1];
+    if (kp) {
+        const auto r_fw = rsrc(st.kfwd, cb);
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const unsigned co = G.cofs(x, j0 + w + TW * k);
+            kold[k] = co != kOutside ? gld(a.kacc, (int)(co / 4u)) : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < NRF; ++k) fm[k] = bload(r_fw, G.cofs(x, j0 - (R - 1) + w + TW * k));
+#pragma unroll
+        for (int k = 0; k < NHF; ++k) {
+            const int idx = min(tid + TT * k, HF * HCF - 1);
+            const int r = idx / HCF, h = idx - r * HCF;
+            fh[k] = bload(r_fw, G.cofs(i0 - (R - 1) + (h < R - 1 ? h : h + kTX), j0 - (R - 1) + r));
+        }
+    }
+    // LDS stores
+#pragma unroll
+    for (int k = 0; k < NRU; ++k) {
+        const int r = w + TW * k;
+        if (r < HU) uu[r * WU + lane + H] = cu[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NHU; ++k) {
+        const int idx = tid + TT * k;
+        const int r = idx / (2 * H), h = idx - r * 2 * H;
+        uu[idx < TZ * 2 * H ? (r + H) * WU + (h < H ? h : h + kTX) : HU * WU] = hu[k];
+    }
+    if (kp) {
+#pragma unroll
+        for (int k = 0; k < NRF; ++k) {
+            const int r = w + TW * k;
+            if (r < HF) ff[r * WF + lane + R - 1] = fm[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NHF; ++k) {
+            const int idx = tid + TT * k;
+            const int r = idx / HCF, h = idx - r * HCF;
+            ff[idx < HF * HCF ? r * WF + (h < R - 1 ? h : h + kTX) : HF * WF] = fh[k];
+        }
+    }
+    __syncthreads();
+
+    // ---------------- phase 2: sigma from the FP64 differences of u
+    auto frame = [](int v, int lo, int hi) { return (v >= lo && v <= hi) ? 1.f : 0.f; };
+    // dx_v / dz_v of u at region cell `o` of uu
+    auto dux = [&](int o) {
+        float s = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) s += a.cx[m - 1] * (float)(uu[o + m] - uu[o + 1 - m]);
+        return s;
+    };
+    auto duz = [&](int o) {
+        float s = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) s += a.cz[m - 1] * (float)(uu[o + m * WU] - uu[o + (1 - m) * WU]);
+        return s;
+    };
+    const float fxs = frame(x, R - 1, nx - R - 1);
+#pragma unroll
+    for (int k = 0; k < NRZ; ++k) {
+        const int r = w + TW * k;  // sigma_zy row: z = j0 - R + r
+        if (r >= HZ) break;
+        const int z = j0 - R + r;
+        szy[r * kTX + lane] = mz[k] * (duz((r - R + H) * WU + lane + H) * frame(z, R - 1, nz - R - 1));
+        if (r >= R && r < R + TZ)  // a tile row: sigma_xy at the lane's own column
+            sxy[(r - R) * WX + lane + R] = mz[k] * (dux((r - R + H) * WU + lane + H) * fxs);
+    }
+#pragma unroll
+    for (int k = 0; k < NHM; ++k) {
+        const int idx = min(tid + TT * k, TZ * 2 * R - 1);
+        const int r = idx / (2 * R), h = idx - r * 2 * R;
+        const int c = h < R ? h : h + kTX;  // sigma_xy column: x = i0 - R + c
+        const float v = hm[k] * (dux((r + H) * WU + c - R + H) * frame(i0 - R + c, R - 1, nx - R - 1));
+        sxy[tid + TT * k < TZ * 2 * R ? r * WX + c : TZ * WX] = v;
+    }
+    __syncthreads();
+
+    // ---------------- phase 3: velocity and displacement of the tile's own nodes
+    const int e0 = a.ps_off ? gld(a.ps_off, t) : 0, e1 = a.ps_off ? gld(a.ps_off, t + 1) : 0;
+    const int f0 = a.rec_off ? gld(a.rec_off, t) : 0, f1 = a.rec_off ? gld(a.rec_off, t + 1) : 0;
+    const float fxl = frame(x, R, nx - R - 1);
+    const float axl = gld(a.axp, x + kGhostX);
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {
+        const int lz = w + TW * k, z = j0 + lz;
+        const float* row = sxy + lz * WX + lane + R;
+        float s = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) s += a.cx[m - 1] * (row[m - 1] - row[-m]);
+        const float* col = szy + (lz + R) * kTX + lane;
+        float q = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) q += a.cz[m - 1] * (col[(m - 1) * kTX] - col[-m * kTX]);
+        float ds = s * fxl + q * frame(z, R, nz - R - 1);
+        if (e1 > e0) {
+            const int key = node_key(x, z);
+            for (int e = e0; e < e1; ++e) {
+                const int2 en = gld(a.ps_ent, e);
+                if (en.x == key) ds += st.ps_amp ? gld(st.ps_amp, en.y) : st.ps_const;
+            }
+        }
+        ds += msy[k];
+        const float vn = (mv[k] + mb[k] * ds) * (axl * gld(a.azp, z + kGhost));
+        const double uo = uu[(lz + H) * WU + lane + H];
+        const double un = uo + (double)vn * a.dt64;
+        G.put(a.o_v_out, x, z, vn);
+        bstore8(G.rf, pofs8(a.o_u_out, pitch, x, z), un);
+        const unsigned co = G.cofs(x, z);
+        if (st.snap) bstore(rsrc(st.snap, cb), co, (float)uo);
+        if (st.dsamp) bstore(rsrc(st.dsamp, cb), co, (float)un);
+        if (f1 > f0 && st.seis_col) {
+            const int key = node_key(x, z);
+            for (int e = f0; e < f1; ++e) {
+                const int2 en = gld(a.rec_ent, e);
+                if (en.x == key) gst(st.seis_col, en.y, un);
+            }
+        }
+    }
+
+    // ---------------- phase 4: the previous step's structure-kernel product
+    if (kp) {
+        const bool xin = x >= R - 1 && x <= nx - R - 1;
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const int lz = w + TW * k, z = j0 + lz;
+            const unsigned co = G.cofs(x, z);
+            const bool zin = z >= R - 1 && z <= nz - R - 1;
+            if (co == kOutside || (!xin && !zin)) continue;
+            const int ou = (lz + H) * WU + lane + H;
+            const float* frow = ff + (lz + R - 1) * WF + lane + R - 1;
+            double acc = 0.0;
+            if (xin) {
+                float sf = 0.f;
+#pragma unroll
+                for (int m = 1; m <= R; ++m) sf += a.cx[m - 1] * (frow[m] - frow[1 - m]);
+                acc += (double)dux(ou) * (double)sf;
+            }
+            if (zin) {
+                float sf = 0.f;
+#pragma unroll
+                for (int m = 1; m <= R; ++m) sf += a.cz[m - 1] * (frow[m * WF] - frow[(1 - m) * WF]);
+                acc += (double)duz(ou) * (double)sf;
+            }
+            gst(a.kacc, (int)(co / 4u), kold[k] - acc * a.kscale);
+        }
+    }
+}
+
+template <int R, int MODE>
+__global__ __launch_bounds__(kTileThreads) void k_utile(StepArgs a)
+{
+    const int g = blockIdx.x;
+    const JobStep st{a.ps_amp, a.syn, a.snap, a.dsamp, a.seis_col, nullptr, nullptr, nullptr, a.kfwd,
+                     a.ps_const, 0, 0};
+    utile_step<R, MODE>(a, st, (g % kXCD) * a.tiles_per_xcd + g / kXCD);
+}
+
+template <int R, int MODE>
+__global__ __launch_bounds__(kTileThreads) void k_utile_group(GroupArgs G)
+{
+    const int g = blockIdx.x;
+    const int T = (g % kXCD) * G.tiles_per_xcd + g / kXCD;
+    if (T >= G.total_tiles) return;
+    int j = 0;
+    while (j + 1 < G.n_jobs && G.tile_start[j + 1] <= T) ++j;  // wave-uniform scan
+    j = __builtin_amdgcn_readfirstlane(j);
+    const JobStep st = G.step[j];
+    const StepArgs a = G.jobs[2 * st.slot + st.parity];
+    utile_step<R, MODE>(a, st, T - G.tile_start[j]);
+}
+
+// The structure-kernel product of the last step of a displacement-form run (the others are fused
+// into the next step): K_mu -= scale * (D u . D fwd) with u the FP64 displacement (padded), the sums
+// formed as in utile_step.
+template <int R>
+__global__ __launch_bounds__(256) void k_ukmu(StepArgs a, const double* __restrict__ u, const float* __restrict__ fwd,
+                                              double* __restrict__ kmu, double scale)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long)a.nx * a.nz) return;
+    const int x = (int)(c % a.nx), z = (int)(c / a.nx);
+    const bool xin = x >= R - 1 && x <= a.nx - R - 1, zin = z >= R - 1 && z <= a.nz - R - 1;
+    if (!xin && !zin) return;
+    const long p = (long)z * a.pitch + x, pz = a.pitch, cz = a.nx;
+    double acc = 0.0;
+    if (xin) {
+        float su = 0.f, sf = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) {
+            su += a.cx[m - 1] * (float)(u[p + m] - u[p + 1 - m]);
+            sf += a.cx[m - 1] * (fwd[c + m] - fwd[c + 1 - m]);
+        }
+        acc += (double)su * (double)sf;
+    }
+    if (zin) {
+        float su = 0.f, sf = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) {
+            su += a.cz[m - 1] * (float)(u[p + m * pz] - u[p + (1 - m) * pz]);
+            sf += a.cz[m - 1] * (fwd[c + m * cz] - fwd[c + (1 - m) * cz]);
+        }
+        acc += (double)su * (double)sf;
+    }
+    kmu[c] -= acc * scale;
+}
+
 // K_mu -= fw_nth * (dx_v(u) .* dx_v(u_fwd) + dz_v(u) .* dz_v(u_fwd)) with u the new displacement
 // (padded) and u_fwd the forward snapshot (compact); the products are formed in FP64.
 // PF / PD: float / double pointer types (global-address-space pointers in the grouped launch)
@@ -916,6 +1225,41 @@ void launch_step(int R, bool with_u, const StepArgs& a, hipStream_t s)
 #undef FD2D_TILE
 }
 
+void launch_ustep(int R, const StepArgs& a, hipStream_t s)
+{
+    const dim3 g((unsigned)(kXCD * a.tiles_per_xcd)), b(kTileThreads);
+    if (R == 3) {
+        if (a.kfwd) hipLaunchKernelGGL((k_utile<3, 1>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_utile<3, 0>), g, b, 0, s, a);
+    } else {
+        if (a.kfwd) hipLaunchKernelGGL((k_utile<4, 1>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_utile<4, 0>), g, b, 0, s, a);
+    }
+}
+
+void launch_ugroup(int R, const GroupArgs& ga, hipStream_t s)
+{
+    const dim3 g((unsigned)(kXCD * ga.tiles_per_xcd)), b(kTileThreads);
+    bool product = false;
+    for (int j = 0; j < ga.n_jobs; ++j) product = product || ga.step[j].kfwd != nullptr;
+    if (R == 3) {
+        if (product) hipLaunchKernelGGL((k_utile_group<3, 1>), g, b, 0, s, ga);
+        else hipLaunchKernelGGL((k_utile_group<3, 0>), g, b, 0, s, ga);
+    } else {
+        if (product) hipLaunchKernelGGL((k_utile_group<4, 1>), g, b, 0, s, ga);
+        else hipLaunchKernelGGL((k_utile_group<4, 0>), g, b, 0, s, ga);
+    }
+}
+
+void launch_ukmu(int R, const StepArgs& a, const double* u, const float* fwd, double* kmu, double scale,
+                 hipStream_t s)
+{
+    const long n = (long)a.nx * a.nz;
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+    if (R == 3) hipLaunchKernelGGL(k_ukmu<3>, g, b, 0, s, a, u, fwd, kmu, scale);
+    else hipLaunchKernelGGL(k_ukmu<4>, g, b, 0, s, a, u, fwd, kmu, scale);
+}
+
 void launch_born_green(int R, const StepArgs& a, float* sxy, float* szy, const float* dtdmu, const float* gin,
                        hipStream_t s)
 {
@@ -933,22 +1277,22 @@ void launch_born_green(int R, const StepArgs& a, float* sxy, float* szy, const f
 // Set-up and element-wise kernels
 // ---------------------------------------------------------------------------------------------
 __global__ void k_material(const double* mu, const double* rho, float* bv, float* dtmu, int nx, int nz,
-                           int pitch, double dt)
+                           int pitch, double dt, double mu_scale)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (long)nx * nz) return;
     const int i = (int)(c % nx), j = (int)(c / nx);
     const long p = (long)j * pitch + i;
     bv[p] = (float)(dt / rho[c]);
-    dtmu[p] = (float)(dt * mu[c]);
+    dtmu[p] = (float)(mu_scale * mu[c]);
 }
 
 void launch_material(const double* mu, const double* rho, float* bv, float* dtmu, int nx, int nz,
-                     int pitch, double dt, hipStream_t s)
+                     int pitch, double dt, double mu_scale, hipStream_t s)
 {
     const long n = (long)nx * nz;
     hipLaunchKernelGGL(k_material, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, mu, rho, bv, dtmu,
-                       nx, nz, pitch, dt);
+                       nx, nz, pitch, dt, mu_scale);
 }
 
 __global__ void k_scale_to_float(const double* in, float* out, size_t n, double scale)

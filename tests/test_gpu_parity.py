@@ -252,3 +252,44 @@ def test_edge_positions(orc, order):
           f"adjoint {e[3]:.2e} K_s {e[4]:.2e}")
     assert e[0] < TOL_FIELD and e[1] < TOL_FIELD and e[2] < TOL_CORR and e[3] < TOL_CORR and e[4] < TOL_KERNEL
     assert np.array_equal(s[2], s[4]) and np.array_equal(corr[2], corr[4])  # the same node twice
+
+
+@pytest.mark.parametrize("order", [6, 8])
+@pytest.mark.parametrize("mode,fw_nth,nt", [(1, 1, 150), (1, 3, 151), (0, 2, 150), (13, 1, 150), (11, 2, 150)])
+def test_displacement_form(orc, order, mode, fw_nth, nt):
+    """Correlation and adjoint runs at FD orders 6 and 8 step v and an FP64 displacement, with sigma
+    evaluated from it (no sigma arrays) and each structure-kernel product computed in the next
+    step's launch (the last step's in a launch of its own: nt = 151 gives 301 steps, a multiple of
+    fw_nth = 3; the half-length mode-0 run ends on a product step at fw_nth = 2). Correlation traces
+    and snapshots, adjoint traces, both kernels, stf_fft and u_out against the oracle."""
+    d0 = tiny().Lx / (tiny().nx - 1)
+    c = tiny(order=order, fw_nth=fw_nth, nt=nt, nx=70, nz=58, Lx=69 * d0, Lz=57 * d0)
+    rng = np.random.default_rng(800 + 10 * order + mode)
+    mu, rho = hetero_model(rng, c)
+    src = interior_points(rng, c, 1, 0.45)
+    rec = interior_points(rng, c, 3)
+    G, Gout, _ = orc.run_forward1_green(mu, rho, src, rec, 1, config=c)
+    spectrum = np.exp(-((np.asarray(c.f_sample) - 0.1) ** 2) / 0.05 ** 2)[:, None]
+    sd = blob_distribution(rng, c)
+    s, C = fd.run_forward2_correlation(mu, rho, G, spectrum, sd, rec, 1, config=c)
+    s0, C0 = orc.run_forward2_correlation(mu, rho, G, spectrum, sd, rec, 1, config=c)
+    stf = np.ascontiguousarray(s0[:, ::-1] / np.abs(s0).max()).astype(np.complex128)
+    if mode == 0:
+        _, a1, _, _ = orc.run_noise_adjoint(mu, rho, C0, stf, rec, rec, None, None, None, 1, config=c)
+        args = (mu, rho, Gout, a1, rec, rec, spectrum, sd, None, 0)
+    else:
+        args = (mu, rho, C0, stf, rec, rec, spectrum, None, G, mode)
+    K, a, U, sa = fd.run_noise_adjoint(*args, config=c)
+    K0, a0, U0, sa0 = orc.run_noise_adjoint(*args, config=c)
+    e = {"correlation": rel_l2(s, s0), "C_out": rel_l2(C, C0), "adjoint seis": rel_l2(sa, sa0),
+         "K_mu": rel_l2(K[:, :, -1], K0[:, :, -1]), "K_s": rel_l2(K[:, :, 0], K0[:, :, 0])}
+    if mode in (1, 13):
+        e["stf_fft"] = rel_l2(a, a0)
+    if mode in (11, 13):
+        e["u_out"] = rel_l2(U, U0)
+    print(f"displacement form order={order} mode={mode} f={fw_nth} nt={nt}: "
+          + " ".join(f"{k} {v:.2e}" for k, v in e.items()))
+    assert np.abs(K0[:, :, -1]).max() > 0
+    assert e["correlation"] < TOL_CORR and e["adjoint seis"] < TOL_CORR
+    assert e["C_out"] < TOL_FIELD and e.get("stf_fft", 0) < TOL_FIELD and e.get("u_out", 0) < TOL_FIELD
+    assert e["K_mu"] < TOL_KERNEL and e["K_s"] < TOL_KERNEL
