@@ -32,13 +32,22 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes per GP-update of a fused FP32 step (v, sigma_xy, sigma_zy, dt/rho, dt*mu
-# read, v, sigma_xy, sigma_zy written = 32 B; +8 B for u read/write; +4/fw_nth B per stored or
+# Algorithmic HBM bytes per GP-update of the engine's fused FP32 steps: the velocity-stress step reads
+# v, sigma_xy, sigma_zy, dt/rho, dt*mu and writes v, sigma_xy, sigma_zy (32 B), +8 B for the
+# displacement u of correlation and adjoint runs; the displacement-form step of those runs at FD
+# orders 6 / 8 reads v, u (FP64), dt/rho, mu and writes v, u (32 B). +4/fw_nth B per stored or
 # replayed FP32 snapshot; adjoint runs with a structure kernel also read and write the FP64 K_mu
-# accumulator at every fw_nth-th step, +16/fw_nth B).
-def alg_bytes(solver: str, fw_nth: int, kernel: bool = False) -> float:
-    b = {"green": 32.0, "correlation": 40.0, "adjoint": 40.0}[solver] + 4.0 / fw_nth
+# accumulator at every fw_nth-th step, +16/fw_nth B.
+def alg_bytes(solver: str, fw_nth: int, kernel: bool = False, displacement_form: bool = False) -> float:
+    b = (32.0 if displacement_form or solver == "green" else 40.0) + 4.0 / fw_nth
     return b + (16.0 / fw_nth if kernel else 0.0)
+
+
+# The algorithmic bytes of a fused single pass of the reference's velocity-stress step
+# (code/run_forward1_green.m:130-192, run_forward2_correlation.m:178-251, run_noise_adjoint.m:233-355):
+# Green 32 B, correlation and adjoint 40 B, +4/fw_nth per stored or replayed snapshot.
+def s8d_bytes(solver: str, fw_nth: int) -> float:
+    return (32.0 if solver == "green" else 40.0) + 4.0 / fw_nth
 
 
 def parse():
@@ -125,11 +134,20 @@ class Pipeline:
         self.obs = self.corr.clone()
 
     def adjoint_sources(self):
-        """cc_time_shift misfits and adjoint sources on the device, into the complex stf buffer."""
+        """cc_time_shift misfits and adjoint sources on the device, into the complex stf buffer. A run
+        truncated in time (profiling) whose traces are still zero at the receivers would give 0/0
+        adjoint sources (cc_time_shift.m:46-52 divides by the energy of the trace); it takes seeded
+        synthetic ones instead."""
         from fd2d_noise_amd.measurement import make_adjoint_sources_device
-        make_adjoint_sources_device(self.corr, self.obs, None, self.t_d, "dis", "cc_time_shift", window=self.window,
-                                    misfit=self.misfit, adstf=self.adstf)
-        self.torch.view_as_real(self.stf)[..., 0].copy_(self.adstf)  # complex(adstf)
+        torch = self.torch
+        if self.t[-1] <= 220.0 and not bool(torch.any(self.corr != 0)):
+            g = torch.Generator(device="cuda").manual_seed(0)
+            self.adstf.copy_(torch.randn(self.adstf.shape, dtype=torch.float64, device="cuda", generator=g))
+            self.misfit.zero_()
+        else:
+            make_adjoint_sources_device(self.corr, self.obs, None, self.t_d, "dis", "cc_time_shift",
+                                        window=self.window, misfit=self.misfit, adstf=self.adstf)
+        torch.view_as_real(self.stf)[..., 0].copy_(self.adstf)  # complex(adstf)
 
     def step(self):
         D, c = self.D, self.c
@@ -609,7 +627,7 @@ def main():
         name = {"green": "green", "correlation": "correlation"}.get(args.only, "adjoint")
         d = tim[name]
         per = d["ms"] / max(d["steps"], 1)
-        ab = alg_bytes(name, c.fw_nth, name == "adjoint")
+        ab = alg_bytes(name, c.fw_nth, name == "adjoint", name != "green" and c.order >= 6)
         print(json.dumps({"only": args.only, "steps_per_launch": args.steps_per_launch,
                           "GP_updates_per_s": gp / el, "avg_step_us": per * 1e3,
                           "avg_launch_us": d["ms"] / max(d["launches"], 1) * 1e3,
@@ -651,12 +669,19 @@ def main():
     d = tim[best]
     launch_ms = d["ms"] / d["launches"]
     steps_per_launch = d["steps"] / d["launches"]
-    bytes_launch = alg_bytes(best, c.fw_nth, best == "adjoint") * c.nx * c.nz * steps_per_launch
+    r = c.order // 2
+    uform = best != "green" and r >= 3  # correlation and adjoint runs at orders 6 / 8: displacement form
+    ab = alg_bytes(best, c.fw_nth, best == "adjoint", uform)
+    bytes_launch = ab * c.nx * c.nz * steps_per_launch
     achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
     kname = "k_pair" if steps_per_launch > 1.5 else "k_tile"
-    r = c.order // 2
     if best == "green":
         kkey = kdesc = f"{kname}<{r},0>"
+    elif uform and best == "adjoint":
+        # plain steps on k_utile<R,0>, the steps that also take the previous step's product on k_utile<R,1>
+        kkey, kdesc = f"k_utile<{r},0|1> adjoint run", f"k_utile<{r},0> + k_utile<{r},1>"
+    elif uform:
+        kkey = kdesc = f"k_utile<{r},0>"
     elif best == "adjoint" and kname == "k_tile":
         # plain steps on k_tile<R,1>, the steps with the structure-kernel product on k_tile<R,2>
         kkey, kdesc = f"k_tile<{r},1|2> adjoint run", f"k_tile<{r},1> + k_tile<{r},2>"
@@ -666,7 +691,14 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kkey, c),
             "kernel": f"{kdesc} ({best} run, {steps_per_launch:.2f} time steps per launch)",
             "avg_launch_us": round(launch_ms * 1e3, 2), "steps_per_launch": round(steps_per_launch, 3),
-            "alg_bytes_per_gp": alg_bytes(best, c.fw_nth, best == "adjoint")}
+            "alg_bytes_per_gp": ab}
+    # the whole pipeline against the reference's velocity-stress bytes per GP-update
+    s8d = (c.nx * c.nz * (c.nt * s8d_bytes("green", c.fw_nth) + 2 * c.n_corr * s8d_bytes("correlation", c.fw_nth)
+                          + c.nt * s8d_bytes("adjoint", c.fw_nth))) * args.steps / el / 1e9
+    pipeline_s8d = {"achieved": round(s8d, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(s8d / HBM_PEAK_GBS, 4),
+                    "note": "sum over the pipeline's solves of GP-updates x the velocity-stress step's bytes "
+                            "(Green 32, correlation and adjoint 40, +4/fw_nth per snapshot) over the wall time"}
     lat_ms = np.array(lat) * 1e3
     n_rec, gp_step = p.nr, p.gp_updates()
     # free the pipeline's device memory for the runs below
@@ -712,6 +744,7 @@ def main():
         "correlations_kernels_per_s": args.steps / el,
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
         "roofline": roof,
+        "pipeline_hbm_equivalent": pipeline_s8d,
         "cpu_baseline": cpu,
         "accuracy": acc,
     }

@@ -807,9 +807,13 @@ __device__ __forceinline__ void utile_step(const StepArgs& a, const JobStep& st,
     constexpr int WF = kTX + 2 * R - 1, HF = TZ + 2 * R - 1;  // forward snapshot: (R-1, R) halo
     constexpr int HCF = 2 * R - 1;                   // its halo columns
     __shared__ double uu[HU * WU + 1];               // + 1: the sink of the packed lists' idle lanes
-    __shared__ float sxy[TZ * WX + 1];
-    __shared__ float szy[HZ * kTX];
-    __shared__ float ff[MODE == 1 ? HF * WF + 1 : 1];
+    // sigma_xy and sigma_zy, then (phase 4) the forward snapshot in the same space, so a product
+    // step holds no more LDS than a plain one (three workgroups per CU)
+    __shared__ float sg[TZ * WX + 1 + HZ * kTX];
+    float* const sxy = sg;
+    float* const szy = sg + TZ * WX + 1;
+    float* const ff = sg;
+    static_assert(HF * WF + 1 <= TZ * WX + 1 + HZ * kTX, "the forward snapshot must fit the sigma space");
 
     if (t >= a.n_tiles) return;
     const int i0 = (t % a.tiles_x) * kTX, j0 = (t / a.tiles_x) * TZ;
@@ -886,19 +890,6 @@ __device__ __forceinline__ void utile_step(const StepArgs& a, const JobStep& st,
         const int idx = tid + TT * k;
         const int r = idx / (2 * H), h = idx - r * 2 * H;
         uu[idx < TZ * 2 * H ? (r + H) * WU + (h < H ? h : h + kTX) : HU * WU] = hu[k];
-    }
-    if (kp) {
-#pragma unroll
-        for (int k = 0; k < NRF; ++k) {
-            const int r = w + TW * k;
-            if (r < HF) ff[r * WF + lane + R - 1] = fm[k];
-        }
-#pragma unroll
-        for (int k = 0; k < NHF; ++k) {
-            const int idx = tid + TT * k;
-            const int r = idx / HCF, h = idx - r * HCF;
-            ff[idx < HF * HCF ? r * WF + (h < R - 1 ? h : h + kTX) : HF * WF] = fh[k];
-        }
     }
     __syncthreads();
 
@@ -981,6 +972,19 @@ __device__ __forceinline__ void utile_step(const StepArgs& a, const JobStep& st,
 
     // ---------------- phase 4: the previous step's structure-kernel product
     if (kp) {
+        __syncthreads();  // phase 3 has read sigma: the snapshot takes its space
+#pragma unroll
+        for (int k = 0; k < NRF; ++k) {
+            const int r = w + TW * k;
+            if (r < HF) ff[r * WF + lane + R - 1] = fm[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NHF; ++k) {
+            const int idx = tid + TT * k;
+            const int r = idx / HCF, h = idx - r * HCF;
+            ff[idx < HF * HCF ? r * WF + (h < R - 1 ? h : h + kTX) : HF * WF] = fh[k];
+        }
+        __syncthreads();
         const bool xin = x >= R - 1 && x <= nx - R - 1;
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
