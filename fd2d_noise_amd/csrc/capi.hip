@@ -1563,6 +1563,7 @@ struct Entry {
     StepArgs stat[2]{};             // host copy of the static arguments written to the slot
     hipEvent_t done_ev = nullptr;   // recorded after the solve's last work
     hipEvent_t chk_ev = nullptr;    // recorded after the copy of its last finiteness check
+    int lane = 0;                   // the engine stream its work runs on
     bool chk_pending = false;       // a check is queued and chk_ev not yet seen complete
     int next_check = 0;             // steps issued at which the next periodic check is queued
     ~Entry()
@@ -1640,8 +1641,20 @@ double step_bytes(const Job& j) { return (double)j.g.P * (field_kind(j) == 1 ? 4
 // pool once: 1/kCheckEvery of a step's traffic).
 constexpr int kCheckEvery = 512;
 
+// Streams ("lanes") of an engine: each solve runs on the lane of its FD order and field set, so the
+// launches of different classes -- which cannot share a launch -- run side by side on the GPU
+// (small grids fill a fraction of the CUs each); the solves of one grouped launch share a lane.
+constexpr int kLanes = 4;
+int lane_of(int order, int solver)
+{
+    const int R = order / 2;
+    return (solver == FD2D_GREEN ? 0 : 1) + 2 * ((R >> 1) & 1);  // orders 4 and 8 (and 2 and 6) apart
+}
+
 struct Engine {
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;          // the caller's stream: inputs are ready on it at submission
+    hipStream_t lanes[kLanes] = {};
+    hipEvent_t submit_ev = nullptr;    // recorded on st at each submission, waited for by the lane
     std::vector<std::unique_ptr<Entry>> act;       // running
     std::vector<std::unique_ptr<Entry>> draining;  // finished, cancelled or failed; their last work still queued
     // Solves that were cancelled or failed, with the steps they had issued. Complete solves keep no
@@ -1653,8 +1666,9 @@ struct Engine {
     std::map<int64_t, Closed> closed;
     int64_t next_ticket = 1;
     // finiteness checks: per-slot flags on the device, copied into pinned host memory before the
-    // check's event; the partials are the scratch of the check kernel (one check at a time per stream)
-    DBuf<int> chk_partial, chk_flag_d;
+    // check's event; the partials are the scratch of the check kernel (one per lane: one check at a
+    // time per stream)
+    DBuf<int> chk_partial[kLanes], chk_flag_d;
     int* chk_flag_h = nullptr;
     bool dirty = true;
     // Static StepArgs of every solve (both parities) at a fixed slot, written once at submission:
@@ -1672,8 +1686,10 @@ struct Engine {
     explicit Engine(hipStream_t s) : st(s)
     {
         StreamScope scope(st);
+        for (int q = 0; q < kLanes; ++q) HIPCHK(hipStreamCreateWithFlags(&lanes[q], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&submit_ev, hipEventDisableTiming));
         slots_d.alloc((size_t)2 * kEngineSlots);
-        chk_partial.alloc(kCheckBlocks);
+        for (int q = 0; q < kLanes; ++q) chk_partial[q].alloc(kCheckBlocks);
         chk_flag_d.alloc(kEngineSlots);
         HIPCHK(hipHostMalloc((void**)&chk_flag_h, sizeof(int) * kEngineSlots, hipHostMallocDefault));
         std::fill(chk_flag_h, chk_flag_h + kEngineSlots, 0);
@@ -1683,10 +1699,11 @@ struct Engine {
     void queue_check(Entry& e)
     {
         const Solver& S = *e.job->S;
-        launch_nonfinite(S.check_ptr(), S.check_count(), chk_partial.p, chk_flag_d.p + e.slot, st);
-        HIPCHK(hipMemcpyAsync(chk_flag_h + e.slot, chk_flag_d.p + e.slot, sizeof(int), hipMemcpyDeviceToHost, st));
+        const hipStream_t ls = e.job->st;
+        launch_nonfinite(S.check_ptr(), S.check_count(), chk_partial[e.lane].p, chk_flag_d.p + e.slot, ls);
+        HIPCHK(hipMemcpyAsync(chk_flag_h + e.slot, chk_flag_d.p + e.slot, sizeof(int), hipMemcpyDeviceToHost, ls));
         if (!e.chk_ev) HIPCHK(hipEventCreateWithFlags(&e.chk_ev, hipEventDisableTiming));
-        HIPCHK(hipEventRecord(e.chk_ev, st));
+        HIPCHK(hipEventRecord(e.chk_ev, ls));
         e.chk_pending = true;
     }
     // Take a running solve out (cancelled or failed): no further steps; its queued work drains.
@@ -1695,7 +1712,7 @@ struct Engine {
         Entry& e = **it;
         closed[e.ticket] = Closed{state, (int64_t)e.job->next - 1, (int64_t)e.job->nsteps};
         if (!e.done_ev) HIPCHK(hipEventCreateWithFlags(&e.done_ev, hipEventDisableTiming));
-        HIPCHK(hipEventRecord(e.done_ev, st));
+        HIPCHK(hipEventRecord(e.done_ev, e.job->st));
         draining.push_back(std::move(*it));
         act.erase(it);
         dirty = true;
@@ -1782,6 +1799,7 @@ struct Engine {
     // and the pair is allowed.
     void step_single(Job& j)
     {
+        StreamScope scope(j.st);
         const int k = (shape_of(j) == 2 && !j.born && j.next + 1 <= j.nsteps && j.pairable(j.next)) ? 2 : 1;
         j.S->clear_step();
         for (int q = 0; q < k; ++q) j.plan(j.next + q, q);
@@ -1816,6 +1834,7 @@ struct Engine {
     }
     void step_group(const Group& gr)
     {
+        StreamScope scope(gr.members[0]->job->st);
         std::vector<Entry*> one, two;
         for (Entry* e : gr.members) (pairs_now(*e->job) ? two : one).push_back(e);
         if (!one.empty()) launch_singles(gr, one);
@@ -1843,8 +1862,9 @@ struct Engine {
         ga.tile_start[ga.n_jobs] = total;
         ga.total_tiles = total;
         ga.tiles_per_xcd = (total + kXCD - 1) / kXCD;
-        if (gr.kind == 2) launch_ugroup(gr.R, ga, st);
-        else launch_group(gr.R, gr.kind == 1, ga, st);
+        const hipStream_t ls = members[0]->job->st;  // the group's lane
+        if (gr.kind == 2) launch_ugroup(gr.R, ga, ls);
+        else launch_group(gr.R, gr.kind == 1, ga, ls);
         for (int q = 0; q < ga.n_jobs; ++q) {
             Job& j = *members[q]->job;
             j.S->cur ^= 1;
@@ -1874,7 +1894,7 @@ struct Engine {
         ga.tile_start[ga.n_jobs] = total;
         ga.total_tiles = total;
         ga.tiles_per_xcd = (total + kXCD - 1) / kXCD;
-        launch_pair_group(gr.R, gr.kind == 1, ga, st);
+        launch_pair_group(gr.R, gr.kind == 1, ga, members[0]->job->st);
         for (int q = 0; q < ga.n_jobs; ++q) {
             Job& j = *members[q]->job;
             j.S->cur ^= 1;  // a pair swaps the state buffers once, as one step does
@@ -1905,10 +1925,13 @@ struct Engine {
         bool any = false;
         for (auto& e : act)
             if (e->job->next > e->job->nsteps) {
-                e->job->finish();
+                {
+                    StreamScope scope(e->job->st);
+                    e->job->finish();
+                }
                 queue_check(*e);
                 HIPCHK(hipEventCreateWithFlags(&e->done_ev, hipEventDisableTiming));
-                HIPCHK(hipEventRecord(e->done_ev, st));
+                HIPCHK(hipEventRecord(e->done_ev, e->job->st));
                 draining.push_back(std::move(e));
                 any = true;
             }
@@ -1920,13 +1943,20 @@ struct Engine {
     }
     int64_t submit(const fd2d_solve* s)
     {
-        StreamScope scope(st);
+        check_solve(s);
         if (free_slots.empty()) raise(FD2D_ENOMEM, "the engine holds %d solves already", kEngineSlots);
-        std::unique_ptr<Entry> e = make_entry(s, st);
+        // the solve's set-up and steps run on its lane, after the work queued on the caller's stream
+        const int lane = lane_of(s->cfg->order, s->solver);
+        const hipStream_t ls = lanes[lane];
+        HIPCHK(hipEventRecord(submit_ev, st));
+        HIPCHK(hipStreamWaitEvent(ls, submit_ev, 0));
+        StreamScope scope(ls);
+        std::unique_ptr<Entry> e = make_entry(s, ls);
+        e->lane = lane;
         e->slot = free_slots.back();
         e->stat[0] = e->job->static_args(0);
         e->stat[1] = e->job->static_args(1);
-        HIPCHK(hipMemcpyAsync(slots_d.p + 2 * (size_t)e->slot, e->stat, sizeof e->stat, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(slots_d.p + 2 * (size_t)e->slot, e->stat, sizeof e->stat, hipMemcpyHostToDevice, ls));
         free_slots.pop_back();
         chk_flag_h[e->slot] = 0;
         e->next_check = kCheckEvery;
@@ -1949,8 +1979,17 @@ struct Engine {
         retire();
         HIPCHK(hipGetLastError());
         if (act.empty() && !draining.empty()) {  // nothing left to overlap with: wait for the rest
-            HIPCHK(hipStreamSynchronize(st));
+            sync_lanes();
             reap();
+        }
+    }
+    // Wait for every lane; the caller's stream then also waits for them (ordering later work on it).
+    void sync_lanes()
+    {
+        for (int q = 0; q < kLanes; ++q) {
+            HIPCHK(hipEventRecord(submit_ev, lanes[q]));
+            HIPCHK(hipStreamWaitEvent(st, submit_ev, 0));
+            HIPCHK(hipStreamSynchronize(lanes[q]));
         }
     }
     // solves submitted and not yet complete
@@ -2003,14 +2042,19 @@ struct Engine {
     }
     ~Engine()
     {
+        for (int q = 0; q < kLanes; ++q)
+            if (lanes[q]) (void)hipStreamSynchronize(lanes[q]);
         (void)hipStreamSynchronize(st);
         StreamScope scope(st);
         act.clear();
         draining.clear();
         slots_d.release();
-        chk_partial.release();
+        for (int q = 0; q < kLanes; ++q) chk_partial[q].release();
         chk_flag_d.release();
         if (chk_flag_h) (void)hipHostFree(chk_flag_h);
+        if (submit_ev) (void)hipEventDestroy(submit_ev);
+        for (int q = 0; q < kLanes; ++q)
+            if (lanes[q]) (void)hipStreamDestroy(lanes[q]);
     }
 };
 
@@ -2491,6 +2535,7 @@ int fd2d_run_batch_device(const fd2d_solve* solves, int32_t n, void* stream)
         std::vector<int64_t> tk((size_t)n);
         for (int32_t q = 0; q < n; ++q) tk[(size_t)q] = e.submit(&solves[q]);
         while (!e.act.empty()) e.advance(1 << 20);
+        e.sync_lanes();
         HIPCHK(hipStreamSynchronize(e.st));
         e.reap();
         std::string bad;
