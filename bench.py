@@ -64,6 +64,8 @@ def parse():
                          "stream of mixed correlation / kernel / Hessian-vector requests through the serving "
                          "loop against the same requests run one solve at a time")
     ap.add_argument("--iterations", type=int, default=10, help="L-BFGS iterations of the inversion workload")
+    ap.add_argument("--hvp-probes", type=int, default=8,
+                    help="Rademacher Hessian-vector products per L-BFGS iteration of the inversion workload")
     ap.add_argument("--rates", type=str, default="5,20,200", help="arrival rates [req/s] of the serving workload")
     ap.add_argument("--requests", type=int, default=48, help="requests per rate of the serving workload")
     ap.add_argument("--stations", type=int, default=64, help="stations of the stations workload")
@@ -487,7 +489,9 @@ def inversion_bench(args):
     """Structure inversion of the 20-station ring: observed correlations from the true model
     (calculate_data, stations batched), one objective + gradient evaluation of the initial model on
     the plain path (stations one after another) and batched (compared bit for bit), then L-BFGS
-    iterations with the batched problem."""
+    iterations with the batched problem and, at every iterate, --hvp-probes Rademacher
+    Hessian-vector products batched over stations and probes (one probe also on the plain path,
+    compared bit for bit)."""
     import torch
     from fd2d_noise_amd import formats as F, inversion as V
     from fd2d_noise_amd.workloads import config4_workload
@@ -533,18 +537,68 @@ def inversion_bench(args):
             self.n += 1
             return prob.eval_grad_objective(m, name)
     cp = Counting()
+    # Hessian-vector products at every L-BFGS iterate: Rademacher probes z (+-1 per node, times 1% of
+    # mu_0 on the structure map), giving the Hutchinson estimate mean(z .* Hz) of diag(H)
+    # (eval_hessian_vector_product_2.m per probe)
+    Q = args.hvp_probes
+    rng_z = np.random.default_rng(44)
+    dmu_scale = 0.01 * w["mu_0"]
+    n_hvp = n_ref * (c.nt + c.n_corr)  # steps per station of the model's Green and correlation runs
+    n_probe = n_ref * (c.nt + 5 * c.n_corr)  # per probe: Born Green, Born correlation, 4 adjoint runs
+    hvp_log = []
+
+    def probes_at(it, model):
+        if Q <= 0 or it == 0:
+            return
+        Z = [rng_z.choice([-1.0, 1.0], P) for _ in range(Q)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Hz = prob.eval_hessian_vector_products(model["m"], [dmu_scale * z for z in Z])
+        torch.cuda.synchronize()
+        dt_ = time.perf_counter() - t0
+        plan = prob.last_hvp_plan
+        groups = plan["groups_per_station"]
+        steps = groups * n_hvp + Q * n_probe
+        diag = np.mean([z * h[P:].cpu().numpy() for z, h in zip(Z, Hz)], axis=0)
+        hvp_log.append({"iteration": it, "s": dt_, "gp_updates": steps * P, "groups_per_station": groups,
+                        "diag_estimate_mean": float(diag.mean()), "diag_estimate_std": float(diag.std())})
+        print(f"# iteration {it}: {Q} HVP probes {dt_:.1f} s ({steps * P / dt_ / 1e9:.1f} G GP/s, "
+              f"{groups} group(s) per station)", file=sys.stderr, flush=True)
     # first step: a 1% relative perturbation of the structure map at most (start_inversion.m:118-122
     # sets init_step_length per inversion type by hand; optlib_lbfgs.m:119-121 leaves this guess open)
     sigma0 = 0.01 / float(g2.abs().max())
     t0 = time.perf_counter()
     flag, m, usr = V.optlib_lbfgs(m0, V.Options(max_iterations=args.iterations, successive_change=0.0,
                                                 init_step_length=sigma0), usr,
-                                  problem=cp, rng=np.random.default_rng(0))
+                                  problem=cp, rng=np.random.default_rng(0), on_iteration=probes_at)
     torch.cuda.synchronize()
     t_inv = time.perf_counter() - t0
     objs = [float(mod["objective"]) for mod in usr.model]
-    print(f"# L-BFGS: {len(objs) - 1} iterations, {cp.n} evaluations, {t_inv:.1f} s, objective {objs[0]:.4g} -> "
-          f"{objs[-1]:.4g}", file=sys.stderr, flush=True)
+    t_hvp = sum(h["s"] for h in hvp_log)
+    print(f"# L-BFGS: {len(objs) - 1} iterations, {cp.n} evaluations, {t_inv:.1f} s ({t_hvp:.1f} s in HVPs), "
+          f"objective {objs[0]:.4g} -> {objs[-1]:.4g}", file=sys.stderr, flush=True)
+    hvp = None
+    if Q > 0:
+        # one probe at the final model on the plain path (stations one after another) against the
+        # batched result, bit for bit
+        z = rng_z.choice([-1.0, 1.0], P) * dmu_scale
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h_plain = prob.eval_hessian_vector_product(m, z)
+        torch.cuda.synchronize()
+        t_plain1 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        h_batch = prob.eval_hessian_vector_products(m, [z])[0]
+        torch.cuda.synchronize()
+        t_batch1 = time.perf_counter() - t0
+        hvp = {"probes_per_iteration": Q, "probe": "Rademacher +-1 x 1% mu_0 on the structure map",
+               "per_iteration": hvp_log, "total_s": t_hvp,
+               "gp_updates_per_s": sum(h["gp_updates"] for h in hvp_log) / max(t_hvp, 1e-9),
+               "plan": prob.last_hvp_plan,
+               "one_probe_s": {"plain": t_plain1, "batched": t_batch1},
+               "one_probe_identical": bool(torch.equal(h_plain, h_batch))}
+        print(f"# one probe: plain {t_plain1:.1f} s, batched {t_batch1:.1f} s, identical "
+              f"{hvp['one_probe_identical']}", file=sys.stderr, flush=True)
     out = {
         "metric": "Gpt-updates/s, correlations+kernels/s, p99 latency; FP32 vs plain, L2<=1e-5",
         "value": steps_eval * P / t_batch, "unit": "GP-updates/s", "n_gpus": 1, "steps": len(objs) - 1,
@@ -558,6 +612,7 @@ def inversion_bench(args):
         "lbfgs": {"flag": int(flag), "iterations": len(objs) - 1, "evaluations": cp.n, "wall_s": t_inv,
                   "objectives": objs},
         "observed_data_s": t_data,
+        "hvp": hvp,
     }
     print(json.dumps(out), flush=True)
 

@@ -298,14 +298,27 @@ class Engine:
             pass
 
 
-def run_programs(programs: Sequence, rounds: int = 64) -> list:
+def available_bytes() -> int:
+    """Device bytes a batch can plan with: the free device memory after the library has returned
+    its cached blocks, plus what torch's caching allocator holds but does not use."""
+    _check(_bind().fd2d_release_cached_memory())
+    free, _ = torch.cuda.mem_get_info()
+    return int(free + torch.cuda.memory_reserved() - torch.cuda.memory_allocated())
+
+
+def run_programs(programs: Sequence, rounds: int = 64, costs: Optional[Sequence[int]] = None,
+                 budget: Optional[int] = None) -> list:
     """Runs solve programs together through one engine and returns their results in order. A
     program is a generator that yields the solves of its next stage (a list of SolveSpec), is
     resumed once all of them are complete, and returns its result; the stages of all programs share
-    the engine's grouped launches."""
+    the engine's grouped launches. With costs (peak device bytes of each program) and budget, a
+    program starts only when its cost fits beside the programs running (the first always starts),
+    in order, as earlier ones finish; a generator allocates nothing before it starts."""
     progs = list(programs)
     results = [None] * len(progs)
     pending = {}
+    running = {}  # started programs -> cost
+    nxt = 0
     with Engine() as eng:
         def resume(q, exc=None):
             try:
@@ -313,17 +326,32 @@ def run_programs(programs: Sequence, rounds: int = 64) -> list:
             except StopIteration as fin:
                 results[q] = fin.value
                 pending.pop(q, None)
+                running.pop(q, None)
+                progs[q] = None
                 return
             pending[q] = (specs, [eng.submit(s) for s in specs])
-        for q in range(len(progs)):
-            resume(q)
+
+        def admit():
+            nonlocal nxt
+            while nxt < len(progs):
+                cost = 0 if costs is None else int(costs[nxt])
+                if budget is not None and running and sum(running.values()) + cost > budget:
+                    return
+                running[nxt] = cost
+                nxt += 1
+                resume(nxt - 1)
+        admit()
         while pending:
             eng.advance(rounds)
             for q in [q for q, (_, ts) in pending.items() if all(eng.done(t) for t in ts)]:
-                bad = [k for k, t in enumerate(pending[q][1]) if eng.state(t) == abi.FD2D_SOLVE_FAILED]
+                # the finished stage's specs are released before the program runs on, so that the
+                # arrays it drops are free for its next stage
+                _, ts = pending.pop(q)
+                bad = [k for k, t in enumerate(ts) if eng.state(t) == abi.FD2D_SOLVE_FAILED]
                 # a failed solve is raised inside its program (which may handle it); the other
                 # programs are not affected
                 resume(q, abi.Fd2dError(abi.FD2D_ENONFINITE, f"solve {bad[0]} of the stage became NaN or Inf")
                        if bad else None)
+            admit()
     torch.cuda.synchronize()
     return results

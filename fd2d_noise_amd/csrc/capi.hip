@@ -66,11 +66,14 @@ struct StreamScope {
 // released blocks are kept and handed out again for requests of at least 2/3 of their size. A
 // released block carries an event recorded on the stream that used it; the next user waits on the
 // event, so a block still read by queued work is never handed out early. At most `limit` bytes are
-// kept (a quarter of the device memory); fd2d_release_cached_memory() frees them all.
+// kept (a quarter of the device memory); fd2d_release_cached_memory() frees them all. A stream
+// that is about to be destroyed is detached first: the events recorded on it are waited for and
+// destroyed, so that no cached event refers to a stream that no longer exists.
 struct BlockCache {
     struct Block {
         void* p;
         hipEvent_t ev;
+        hipStream_t st;  // the stream ev was recorded on
     };
     std::mutex mu;
     std::multimap<size_t, Block> blocks;
@@ -78,7 +81,7 @@ struct BlockCache {
     bool limit_set = false;
     void* take(size_t bytes, size_t* got)
     {
-        Block blk{nullptr, nullptr};
+        Block blk{nullptr, nullptr, nullptr};
         {
             std::lock_guard<std::mutex> lk(mu);
             auto it = blocks.lower_bound(bytes);
@@ -89,7 +92,7 @@ struct BlockCache {
             blocks.erase(it);
         }
         if (blk.ev) {
-            (void)hipEventSynchronize(blk.ev);
+            if (hipEventSynchronize(blk.ev) != hipSuccess) (void)hipGetLastError();
             (void)hipEventDestroy(blk.ev);
         }
         return blk.p;
@@ -117,15 +120,28 @@ struct BlockCache {
             (void)hipFree(p);
             return;
         }
-        blocks.emplace(bytes, Block{p, ev});
+        blocks.emplace(bytes, Block{p, ev, st});
         held += bytes;
+    }
+    void detach(hipStream_t s)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& kv : blocks) {
+            Block& b = kv.second;
+            if (b.ev && b.st == s) {
+                if (hipEventSynchronize(b.ev) != hipSuccess) (void)hipGetLastError();
+                (void)hipEventDestroy(b.ev);
+                b.ev = nullptr;
+                b.st = nullptr;
+            }
+        }
     }
     void clear()
     {
         std::lock_guard<std::mutex> lk(mu);
         for (auto& kv : blocks) {
             if (kv.second.ev) {
-                (void)hipEventSynchronize(kv.second.ev);
+                if (hipEventSynchronize(kv.second.ev) != hipSuccess) (void)hipGetLastError();
                 (void)hipEventDestroy(kv.second.ev);
             }
             (void)hipFree(kv.second.p);
@@ -689,7 +705,8 @@ struct DeviceTable {
     unsigned short* part[3] = {nullptr, nullptr, nullptr};
     std::vector<unsigned short> host[3];  // staging of the asynchronous upload
     size_t n = 0;                         // elements per part
-    hipEvent_t ready = nullptr;
+    hipEvent_t ready = nullptr;     // the upload is complete; null once it is known to be
+    hipStream_t ready_st = nullptr;  // the stream of the upload
     ~DeviceTable()
     {
         for (unsigned short* q : part)
@@ -726,7 +743,7 @@ std::shared_ptr<const DeviceTable> cached_table(const std::string& key, hipStrea
             std::shared_ptr<DeviceTable> t = c.lru[q].second;
             c.lru.erase(c.lru.begin() + (long)q);
             c.lru.emplace_back(key, t);
-            HIPCHK(hipStreamWaitEvent(st, t->ready, 0));
+            if (t->ready) HIPCHK(hipStreamWaitEvent(st, t->ready, 0));
             return t;
         }
     SplitTable e;
@@ -748,6 +765,7 @@ std::shared_ptr<const DeviceTable> cached_table(const std::string& key, hipStrea
     }
     HIPCHK(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming));
     HIPCHK(hipEventRecord(t->ready, st));
+    t->ready_st = st;
     c.lru.emplace_back(key, t);
     c.bytes += 3 * t->n * sizeof(unsigned short);
     for (size_t q = 0; c.bytes > kTableCacheBytes && q + 1 < c.lru.size();) {
@@ -759,6 +777,24 @@ std::shared_ptr<const DeviceTable> cached_table(const std::string& key, hipStrea
         }
     }
     return t;
+}
+
+// Before the stream s is destroyed: the cached device blocks and tables whose events were
+// recorded on s lose those events (the work they mark is waited for first).
+void detach_stream(hipStream_t s)
+{
+    for (BlockCache& c : g_cache) c.detach(s);
+    std::lock_guard<std::mutex> lk(g_table_mu);
+    for (int d = 0; d < 64; ++d)
+        for (auto& kv : g_tables[d].lru) {
+            DeviceTable& t = *kv.second;
+            if (t.ready && t.ready_st == s) {
+                if (hipEventSynchronize(t.ready) != hipSuccess) (void)hipGetLastError();
+                (void)hipEventDestroy(t.ready);
+                t.ready = nullptr;
+                t.ready_st = nullptr;
+            }
+        }
 }
 
 // Key of the phase table e^{sign i w_k t_j} * scale over the times ts, in layout `kind` with slab
@@ -2054,7 +2090,10 @@ struct Engine {
         if (chk_flag_h) (void)hipHostFree(chk_flag_h);
         if (submit_ev) (void)hipEventDestroy(submit_ev);
         for (int q = 0; q < kLanes; ++q)
-            if (lanes[q]) (void)hipStreamDestroy(lanes[q]);
+            if (lanes[q]) {
+                detach_stream(lanes[q]);
+                (void)hipStreamDestroy(lanes[q]);
+            }
     }
 };
 

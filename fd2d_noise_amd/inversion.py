@@ -333,16 +333,17 @@ class NoiseProblem:
         nb_k = max(c.n_basis_fct, 1) + 1
         source_dist, mu = Pm[:n_maps - 1], Pm[n_maps - 1]
         n_ref, nr, N = self.ref.shape[0], self.n_rec, c.n_corr
-        # stations per batch: what fits in 70% of the free device memory (the arrays a station holds
-        # at its peak: spectra, forward snapshots, the library's slabs and spectra)
+        # a station starts when its peak fits in 70% of the available device memory beside the
+        # stations running: G_fft and the transformed adjoint state, the forward snapshots G and
+        # C, two kernels, traces and adjoint sources, and the library buffers of its largest solve
         P, K = c.nx * c.nz, c.n_sample
-        per = 16 * P * K * 3 + 4 * P * (c.n_fw_green() + c.n_fw_corr()) + 4 * P * 512 * 2 + 8 * P * 8
-        free, _ = torch.cuda.mem_get_info()
-        chunk = max(1, min(n_ref, int(0.7 * free) // per))
-        out = []
-        for i0 in range(0, n_ref, chunk):
-            out += B.run_programs([self._station_program(i, mu, self.rho, self.spectrum, source_dist, only_objective)
-                                   for i in range(i0, min(n_ref, i0 + chunk))])
+        lib = max(B.solve_bytes(abi.FD2D_GREEN, c, 1, 1, 0),
+                  B.solve_bytes(abi.FD2D_CORRELATION, c, 1, 0, nr, True, 1, True),
+                  B.solve_bytes(abi.FD2D_ADJOINT, c, 1, nr, nr, True, 1, False, (N, nr, 1), fft_out=True),
+                  B.solve_bytes(abi.FD2D_ADJOINT, c, 0, nr, nr, False, 1, True, (c.nx, c.nz, K)))
+        per = 2 * 16 * P * K + 4 * P * (c.n_fw_green() + c.n_fw_corr()) + 2 * 8 * P * nb_k + 4 * 8 * nr * N + lib
+        out = B.run_programs([self._station_program(i, mu, self.rho, self.spectrum, source_dist, only_objective)
+                              for i in range(n_ref)], costs=[per] * n_ref, budget=int(0.7 * B.available_bytes()))
         c_all = torch.zeros((N, n_ref * nr), dtype=torch.float64, device="cuda")
         grad = torch.zeros((nb_k, c.nz, c.nx), dtype=torch.float64, device="cuda")
         w = float(usr.kernel_weighting)
@@ -452,26 +453,9 @@ class NoiseProblem:
         saved wavefields of every step, so the configuration needs fw_nth = 1."""
         from .measurement import make_adjoint_sources_device
         torch, D, c, usr = self.torch, self.D, self.c, self.usr
-        if c.fw_nth != 1:
-            raise ValueError("the Hessian-vector product needs fw_nth = 1 (eval_hessian_vector_product_2.m:75,154)")
-        Pm = map_m_to_parameters(m, usr)
-        n_maps = Pm.shape[0]
-        mu, sd_ref = Pm[n_maps - 1], Pm[:n_maps - 1]
+        Pm, mu, sd_ref = self._hvp_model(m)
         rho, spec = self.rho, self.spectrum
-        dmp = dm_parameters if torch.is_tensor(dm_parameters) else torch.as_tensor(
-            np.ascontiguousarray(np.asarray(dm_parameters, dtype=np.float64).reshape(-1, order="F")), device="cuda")
-        if dmp.numel() != Pm.numel():  # :19-31
-            dm = torch.zeros_like(Pm)
-            if usr.type == "source":
-                dm[:n_maps - 1].view(-1).copy_(dmp.reshape(-1))
-            elif usr.type == "structure":
-                dm[n_maps - 1].view(-1).copy_(dmp.reshape(-1))
-        else:
-            dm = dmp.reshape(Pm.shape).contiguous()
-        sd_pert = dm[:n_maps - 1]
-        dmu = dm[n_maps - 1]
-        has_dmu = bool(torch.count_nonzero(dmu).item())
-        has_dsd = bool(torch.count_nonzero(sd_pert).item())
+        sd_pert, dmu, has_dmu, has_dsd = self._hvp_perturbation(Pm, dm_parameters)
         f64, f32, cd = torch.float64, torch.float32, torch.complex128
         nr, N = self.n_rec, c.n_corr
         w = float(usr.kernel_weighting)
@@ -551,12 +535,213 @@ class NoiseProblem:
             if usr.type != "source":  # RED :168-170
                 adjoint(G, kT, sd_ref, None, 10, dmu if p is not None else None, p)
                 add()
-        Hm = map_gradparameters_to_gradm(H, usr)  # map_Hparameters_to_Hm.m:19-38
+        return self._hvp_finish(H)
+
+    def _hvp_model(self, m):
+        if self.c.fw_nth != 1:
+            raise ValueError("the Hessian-vector product needs fw_nth = 1 (eval_hessian_vector_product_2.m:75,154)")
+        Pm = map_m_to_parameters(m, self.usr)
+        n_maps = Pm.shape[0]
+        return Pm, Pm[n_maps - 1], Pm[:n_maps - 1]
+
+    def _hvp_perturbation(self, Pm, dm_parameters):
+        """dm_parameters as (source maps, structure map, has_dmu, has_dsd): the full parameter
+        array, or only the source / structure maps for usr.type 'source' / 'structure'
+        (eval_hessian_vector_product_2.m:19-31)."""
+        torch, usr = self.torch, self.usr
+        n_maps = Pm.shape[0]
+        dmp = dm_parameters if torch.is_tensor(dm_parameters) else torch.as_tensor(
+            np.ascontiguousarray(np.asarray(dm_parameters, dtype=np.float64).reshape(-1, order="F")), device="cuda")
+        if dmp.numel() != Pm.numel():
+            dm = torch.zeros_like(Pm)
+            if usr.type == "source":
+                dm[:n_maps - 1].view(-1).copy_(dmp.reshape(-1))
+            elif usr.type == "structure":
+                dm[n_maps - 1].view(-1).copy_(dmp.reshape(-1))
+        else:
+            dm = dmp.reshape(Pm.shape).contiguous()
+        sd_pert, dmu = dm[:n_maps - 1], dm[n_maps - 1]
+        return sd_pert, dmu, bool(torch.count_nonzero(dmu).item()), bool(torch.count_nonzero(sd_pert).item())
+
+    def _hvp_finish(self, H):
+        Hm = map_gradparameters_to_gradm(H, self.usr)  # map_Hparameters_to_Hm.m:19-38
         # :191-196: alpha / beta * weighting on the interior (the reference adds them without dm)
-        if usr.reg_alpha or usr.reg_beta:
+        if self.usr.reg_alpha or self.usr.reg_beta:
             add_v = self._hvp_reg_term(Hm.numel())
             axpby(1.0, Hm, 1.0, add_v, out=Hm)
         return Hm
+
+    def _hvp_station_program(self, i, mu, sd_ref, probes):
+        """The solves of reference station i for the Hessian-vector products of several
+        perturbations (eval_hessian_vector_product_2.m:57-170) as a program for batch.run_programs.
+        The Green and correlation runs of the model (:57, :71) do not depend on the perturbation
+        and run once for all of them; every perturbation then runs its own Born Green (:61) and
+        Born correlation (:75-91), its adjoint sources (:99-105) and the adjoint runs YELLOW (:113),
+        GRAY (:129, :141), BLACK (:154) and RED (:168), each stage's solves of all perturbations
+        in one engine stage. Arrays are dropped as soon as the last run that reads them is done.
+        Returns, per perturbation, the kernels in the order the sequential loop adds them, each as
+        (K, part) with part the parameter class whose maps are zeroed before the sum."""
+        from . import batch as B
+        from .measurement import make_adjoint_sources_device
+        torch, D, c, usr = self.torch, self.D, self.c, self.usr
+        f64, f32, cd = torch.float64, torch.float32, torch.complex128
+        src = np.asfortranarray(self.ref[i:i + 1])
+        rec = np.asfortranarray(self.recs[i])
+        nr, N = self.n_rec, c.n_corr
+        rho = self.rho
+        spec_h = np.asfortranarray(D.to_host(self.spectrum))
+        w = float(usr.kernel_weighting)
+        nbk = max(c.n_basis_fct, 1) + 1
+        Q = len(probes)
+
+        def spectra():
+            return D.empty_like_dims((c.nx, c.nz, c.n_sample), cd)
+
+        def snaps(n):
+            return D.empty_like_dims((c.nx, c.nz, n), f32)
+
+        def kern():
+            return D.empty_like_dims((c.nx, c.nz, nbk), f64)
+
+        G_fft, G = spectra(), snaps(c.n_fw_green())
+        yield [B.green(mu, rho, src, None, 1, None, None, config=c, G_fft=G_fft, G_out=G)]  # :57
+        c_ref, C = D.empty_like_dims((nr, N), f64), snaps(c.n_fw_corr())
+        stage = [B.correlation(mu, rho, G_fft, spec_h, sd_ref, rec, 1, None, None, config=c, seismograms=c_ref,
+                               C_out=C)]  # :71
+        dG_fft, dG = [None] * Q, [None] * Q
+        for q, (_, dmu, has_dmu, _) in enumerate(probes):
+            if has_dmu:  # :61
+                dG_fft[q], dG[q] = spectra(), snaps(c.n_fw_green())
+                stage.append(B.green(mu, rho, src, None, 5, dmu, G, config=c, G_fft=dG_fft[q], G_out=dG[q]))
+        yield stage
+        c1, dC1, c2, dC2 = [None] * Q, [None] * Q, [None] * Q, [None] * Q
+        stage = []
+        for q, (sd_pert, dmu, has_dmu, has_dsd) in enumerate(probes):
+            if has_dmu:  # :75
+                c1[q], dC1[q] = D.empty_like_dims((nr, N), f64), snaps(c.n_fw_corr())
+                stage.append(B.correlation(mu, rho, dG_fft[q], spec_h, sd_ref, rec, 1, dmu, C, config=c,
+                                           seismograms=c1[q], C_out=dC1[q]))
+            if has_dsd:  # :84
+                c2[q], dC2[q] = D.empty_like_dims((nr, N), f64), snaps(c.n_fw_corr())
+                stage.append(B.correlation(mu, rho, G_fft, spec_h, sd_pert, rec, 1, None, None, config=c,
+                                           seismograms=c2[q], C_out=dC2[q]))
+        if stage:
+            yield stage
+        dC, stf1, stf2 = [None] * Q, [None] * Q, [None] * Q
+        for q in range(Q):
+            c_pert = torch.zeros_like(c_ref)
+            if c1[q] is not None:
+                axpby(1.0, c_pert.view(-1), 1.0, c1[q].view(-1), out=c_pert.view(-1))
+            if c2[q] is not None:
+                axpby(1.0, c_pert.view(-1), 1.0, c2[q].view(-1), out=c_pert.view(-1))
+            dC[q] = dC1[q] if dC2[q] is None else (dC2[q] if dC1[q] is None else axpby_f32(1.0, dC1[q], 1.0, dC2[q]))
+            for order in ("1st", "2nd"):  # :99-105
+                _, a_s = make_adjoint_sources_device(c_ref, self.c_data[i], c_pert, self.t, usr.veldis,
+                                                     usr.measurement_source, order, usr.window)
+                _, a_t = make_adjoint_sources_device(c_ref, self.c_data[i], c_pert, self.t, usr.veldis,
+                                                     usr.measurement_structure, order, usr.window)
+                st = torch.zeros((N, nr), dtype=cd, device="cuda")
+                torch.view_as_real(st)[..., 0].copy_(axpby(1.0 - w, a_s, w, a_t))
+                (stf1 if order == "1st" else stf2)[q] = st
+        del c1, dC1, c2, dC2
+        out = [[] for _ in range(Q)]
+        uT, u_dag = [None] * Q, [None] * Q
+        if usr.type != "source":  # YELLOW :113-121
+            stage = []
+            for q in range(Q):
+                uT[q], u_dag[q], KY = spectra(), snaps(c.n_fw_corr()), kern()
+                stage.append(B.adjoint(mu, rho, dC[q], stf1[q], rec, None, spec_h, None, dG_fft[q], 13, None, None,
+                                       config=c, K=KY, stf_fft=uT[q], u_out=u_dag[q]))
+                out[q].append((KY, "structure" if usr.type == "structure" else None))
+            yield stage
+        del dC, stf1, dG_fft
+        p, kT = [None] * Q, [None] * Q
+        stage = []
+        for q, (sd_pert, dmu, has_dmu, has_dsd) in enumerate(probes):
+            if usr.type != "source" and has_dmu:  # GRAY :129-133
+                p[q], KG = snaps(c.n_fw_corr()), kern()
+                stage.append(B.adjoint(mu, rho, dG[q], uT[q], rec, None, spec_h, sd_ref, None, 11, None, None,
+                                       config=c, K=KG, u_out=p[q]))
+                out[q].append((KG, None))
+            if usr.type == "joint" and has_dsd:  # :141-146
+                pa, KA = snaps(c.n_fw_corr()), kern()
+                stage.append(B.adjoint(mu, rho, G, uT[q], rec, None, spec_h, sd_pert, None, 11, None, None,
+                                       config=c, K=KA, u_out=pa))
+                out[q].append((KA, None))
+                p[q] = pa if p[q] is None else (p[q], pa)
+            # BLACK :154-160
+            kT[q], KB = spectra(), kern()
+            stage.append(B.adjoint(mu, rho, C, stf2[q], rec, None, spec_h, None, G_fft, 12,
+                                   dmu if u_dag[q] is not None else None, u_dag[q], config=c, K=KB, stf_fft=kT[q]))
+            out[q].append((KB, "source" if usr.type == "source" else ("structure" if usr.type == "structure" else None)))
+        yield stage
+        del dG, uT, u_dag, stf2, C, G_fft
+        if usr.type != "source":  # RED :168-170
+            stage = []
+            for q, (_, dmu, _, _) in enumerate(probes):
+                if isinstance(p[q], tuple):
+                    p[q] = axpby_f32(1.0, p[q][0], 1.0, p[q][1])
+                KR = kern()
+                stage.append(B.adjoint(mu, rho, G, kT[q], rec, None, spec_h, sd_ref, None, 10,
+                                       dmu if p[q] is not None else None, p[q], config=c, K=KR))
+                out[q].append((KR, None))
+            yield stage
+        return out
+
+    def eval_hessian_vector_products(self, m, dms, memory_fraction: float = 0.7):
+        """H dm for every perturbation in dms at the model m, the solves of all reference stations
+        and perturbations run together through the batch engine (eval_hessian_vector_product_2.m
+        per perturbation, its station loop :44-183 as the reference's parfor). Each result is
+        identical to eval_hessian_vector_product(m, dm). A program (a station with a group of
+        perturbations) starts when its arrays fit in memory_fraction of the available device memory
+        beside the programs running; when one station with all perturbations does not fit, the
+        perturbations are split into groups of equal size, each of which repeats the station's model
+        Green and correlation runs."""
+        from . import batch as B
+        torch, D, c, usr = self.torch, self.D, self.c, self.usr
+        Pm, mu, sd_ref = self._hvp_model(m)
+        probes = [self._hvp_perturbation(Pm, dm) for dm in dms]
+        Q, n_ref = len(probes), self.ref.shape[0]
+        if Q == 0:
+            return []
+        P, K, nr, N = c.nx * c.nz, c.n_sample, self.n_rec, c.n_corr
+        spec_b, g_snap, c_snap = 16 * P * K, 4 * P * c.n_fw_green(), 4 * P * c.n_fw_corr()
+        lib = max(B.solve_bytes(abi.FD2D_GREEN, c, 1, 1, 0, dmu=True),
+                  B.solve_bytes(abi.FD2D_CORRELATION, c, 1, 0, nr, True, 1, True, dmu=True),
+                  B.solve_bytes(abi.FD2D_ADJOINT, c, 13, nr, nr, True, 1, True, (N, nr, 1), dmu=True, fft_out=True))
+        # a station: G_fft, G, C, c_ref and the library buffers of one solve. A perturbation at its
+        # peak (YELLOW, or GRAY with BLACK): two spectra (dG_fft or kT, uT), dG, two correlation
+        # wavefields (dC and u_dag, or u_dag and p) plus the second Born correlation wavefield when
+        # it perturbs the sources, its traces and adjoint sources, four kernels and the library
+        # buffers of two solves
+        any_dsd = any(pr[3] for pr in probes)
+        shared = spec_b + g_snap + c_snap + 8 * nr * N + lib
+        per = (2 * spec_b + g_snap + (3 if any_dsd else 2) * c_snap + 2 * 16 * N * nr + 8 * 8 * nr * N
+               + 4 * 8 * P * (c.n_basis_fct + 1) + 2 * lib)
+        budget = int(memory_fraction * B.available_bytes())
+        q_per = int(max(1, min(Q, (budget - shared) // per)))
+        n_groups = -(-Q // q_per)
+        q_per = -(-Q // n_groups)  # groups of equal size
+        groups = [(i, q0) for i in range(n_ref) for q0 in range(0, Q, q_per)]
+        costs = [shared + len(probes[q0:q0 + q_per]) * per for _, q0 in groups]
+        self.last_hvp_plan = {"perturbations_per_group": q_per, "groups_per_station": n_groups,
+                              "programs_in_flight": max(1, budget // max(costs)),
+                              "bytes_station": shared, "bytes_perturbation": per, "budget": budget}
+        nbk = max(c.n_basis_fct, 1) + 1
+        H = [torch.zeros((nbk, c.nz, c.nx), dtype=torch.float64, device="cuda") for _ in range(Q)]
+        res = B.run_programs([self._hvp_station_program(i, mu, sd_ref, probes[q0:q0 + q_per]) for i, q0 in groups],
+                             costs=costs, budget=budget)
+        for (i, q0), kernels in zip(groups, res):  # station order, as the sequential loop
+            for dq, Ks in enumerate(kernels):
+                for Kq, part in Ks:
+                    if part == "structure":
+                        Kq[:nbk - 1].zero_()
+                    elif part == "source":
+                        Kq[nbk - 1].zero_()
+                    h = H[q0 + dq]
+                    axpby(1.0, h.view(-1), 1.0, Kq.view(-1), out=h.view(-1))
+        del res
+        return [self._hvp_finish(h) for h in H]
 
     def _hvp_reg_term(self, n):
         """alpha * weighting (source maps) and beta * weighting (structure map) on the nodes where
@@ -697,13 +882,15 @@ def optlib_wolfe(xj, s, stg, f, delta, theta, sig0, try_larger_steps, verbose, u
     return sig, {"m": xn, "gradient": gn, "objective": fn, "correlation": cn, "name": name}
 
 
-def optlib_lbfgs(m0, options: Options, usr_par: UsrPar, problem=None, rng=None):
+def optlib_lbfgs(m0, options: Options, usr_par: UsrPar, problem=None, rng=None, on_iteration=None):
     """[flag, mfinal, usr_par] = optlib_lbfgs(m0, options, usr_par) (optlib_lbfgs.m:1-210).
 
     ``problem`` supplies eval_objective_and_gradient / eval_objective / eval_grad_objective
     (default: NoiseProblem(usr_par)). Models left in usr_par.model by an earlier run warm-start the
     curvature pairs (optlib_lbfgs.m:28-57). flag 0: ||g|| <= tolerance*||g0||; 1: max_iterations;
-    2: misfit change over successive_iterations below successive_change."""
+    2: misfit change over successive_iterations below successive_change. ``on_iteration(it,
+    model)``, when given, runs after each new_iteration (it = 0 for the initial model), where the
+    reference's user edits new_iteration.m for per-iteration work."""
     import torch
     rng = np.random.default_rng() if rng is None else rng
     problem = NoiseProblem(usr_par) if problem is None else problem
@@ -716,6 +903,8 @@ def optlib_lbfgs(m0, options: Options, usr_par: UsrPar, problem=None, rng=None):
     normg0 = norm(g0)
     model["normg"] = normg0
     usr_par = new_iteration(it, model, usr_par)
+    if on_iteration is not None:
+        on_iteration(it, model)
     log(f"it={it}   j={j0:e}   ||g||={normg0:e}  \n")
 
     # optlib_restore_lbfgs_information.m:69-103
@@ -795,6 +984,8 @@ def optlib_lbfgs(m0, options: Options, usr_par: UsrPar, problem=None, rng=None):
             log("\tWarning: Bad angle of search direction. Update of BFGS curvature information skipped.\n")
         model = model_new
         usr_par = new_iteration(it, model, usr_par)
+        if on_iteration is not None:
+            on_iteration(it, model)
 
     if model["normg"] <= options.tolerance * normg0:
         log(f"Successful termination with ||g||<{options.tolerance:e}*min(1,||g0||):\n")

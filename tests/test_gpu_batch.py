@@ -155,6 +155,26 @@ def test_run_batch_matches_single_solves(torch_mod, mix):
               ref[q], got[q])
 
 
+def test_engines_one_after_another(torch_mod, mix):
+    """Engines created and destroyed one after another: each destroys its streams while the
+    library's caches keep the device blocks and phase tables that its solves released, and the next
+    engine takes them again. Every batch must match the single solves, and torch's own allocations
+    and launches between the batches must see no error left behind by the library."""
+    torch = torch_mod
+    from fd2d_noise_amd import batch as B
+    cases, ref = mix
+    for rep in range(4):
+        got = [_outs(torch, d) for _, _, _, d in cases]
+        B.run_batch([_spec(k, c, a, o) for (k, c, a, _), o in zip(cases, got)])
+        torch.cuda.synchronize()
+        for q, (kind, c, args, _) in enumerate(cases):
+            _same(torch, f"engine {rep}: {kind} {c.nx}x{c.nz}", ref[q], got[q])
+        del got
+        x = torch.zeros((1 << 26) + rep, dtype=torch.float64, device="cuda")
+        assert float(x.add_(1.0).sum()) == float(x.numel())
+        torch.cuda.synchronize()
+
+
 def test_engine_continuous_batching(torch_mod, mix):
     """Solves joining while others run (three waves, advanced by uneven round counts)."""
     torch = torch_mod

@@ -330,3 +330,43 @@ def test_noise_hessian_vector_product_device(orc, typ):
         err = rel_l2(H[sl], H_ref[sl])
         print(f"hvp {typ} {part}: {err:.2e}")
         assert err < 1e-4
+
+
+@pytest.mark.parametrize("typ", ["source", "structure", "joint"])
+def test_noise_hessian_vector_products_batched_equal_sequential(orc, typ):
+    """Several perturbations at once (NoiseProblem.eval_hessian_vector_products: every station's
+    model Green and correlation runs shared by the perturbations, all solves through one batch
+    engine) give eval_hessian_vector_product of each perturbation bit for bit, also when the
+    perturbations of a station are split into groups; the first perturbation is a Rademacher
+    (+-1) probe scaled to the parameters."""
+    import torch
+    from fd2d_noise_amd import inversion as V
+    from oracle import inversion as OI
+    from test_oracle_inversion import _noise_usr
+    usr, m_true, blob = _noise_usr(typ)
+    usr.array = np.array([[2.0e4, 3.0e4], [4.0e4, 3.5e4], [3.5e4, 2.0e4], [2.6e4, 4.0e4]])
+    usr.ref_stat = usr.array[:2].copy()
+    usr.measurement_source = "waveform_difference"
+    usr.reg_alpha, usr.reg_beta = 1e-7, 2e-7
+    c = usr.config
+    P = c.nx * c.nz
+    usr.c_data = np.zeros((6, c.n_corr))
+    _, _, usr.c_data = OI.eval_objective_and_gradient(m_true, usr, orc, only_objective=True)
+    m = torch.as_tensor(np.concatenate([np.ones(P), np.zeros(P)]), device="cuda")
+    rng = np.random.default_rng(7)
+    scale = np.concatenate([np.full(P, 0.1), np.full(P, 0.01 * usr.mu_0)])
+    dms = [rng.choice([-1.0, 1.0], 2 * P) * scale]
+    b = blob.ravel(order="F")
+    dms.append(np.concatenate([0.5 * b, np.zeros(P)]))                # source maps only
+    dms.append(np.concatenate([np.zeros(P), 0.02 * usr.mu_0 * b]))    # structure map only
+    prob = V.NoiseProblem(usr)
+    seq = [prob.eval_hessian_vector_product(m, dm) for dm in dms]
+    got = prob.eval_hessian_vector_products(m, dms)
+    print(f"{typ}: plan {prob.last_hvp_plan}")
+    split = prob.eval_hessian_vector_products(m, dms, memory_fraction=1e-9)
+    assert prob.last_hvp_plan["perturbations_per_group"] == 1
+    for q, (a, b_, s) in enumerate(zip(got, split, seq)):
+        print(f"{typ} probe {q}: |H| {float(s.norm()):.3e}")
+        assert torch.equal(a, s), f"probe {q}"
+        assert torch.equal(b_, s), f"probe {q} split"
+    assert prob.eval_hessian_vector_products(m, []) == []
