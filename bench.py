@@ -68,6 +68,8 @@ def parse():
                     help="Rademacher Hessian-vector products per L-BFGS iteration of the inversion workload")
     ap.add_argument("--rates", type=str, default="5,20,200", help="arrival rates [req/s] of the serving workload")
     ap.add_argument("--requests", type=int, default=48, help="requests per rate of the serving workload")
+    ap.add_argument("--serving-range", choices=["small", "full"], default="small",
+                    help="grid sizes of the serving workload: 200^2-400^2, or the full 200^2-4000^2 range")
     ap.add_argument("--stations", type=int, default=64, help="stations of the stations workload")
     ap.add_argument("--window", type=int, default=8, help="stations in flight in the batch engine")
     ap.add_argument("--plain", type=int, default=8, help="stations timed on the plain single-solve path")
@@ -404,18 +406,36 @@ def stations_bench(args):
 
 
 def serving_run(args):
-    """The serving workload at each arrival rate: throughput (requests/s, GP-updates/s) and p50/p99
-    latency (arrival to result); then the requests of the highest rate run one solve at a time
-    through the single-solve entry points (the plain path), and every result is compared."""
+    """The serving workload at each arrival rate: throughput (requests/s, GP-updates/s), p50/p99
+    latency (arrival to result) and admission (rejected and deferred requests); then the requests of
+    the highest rate run one solve at a time through the single-solve entry points (the plain path),
+    and every result is compared. With --serving-range full, the grids span 200^2 to 4000^2
+    (nt = 10 nx): the models whose kernel requests would not fit the budget with every step stored
+    store every fw_nth-th step (the smallest interval at which a kernel request fits half the
+    budget, or the whole budget on the largest grids), HVP requests on them are rejected, and
+    requests on the 200^2 models are held to the FP64 oracle."""
     import torch
     from fd2d_noise_amd import device as D, serving as S
-    from fd2d_noise_amd.workloads import serving_models, serving_stream
-    models = serving_models()
+    from fd2d_noise_amd.workloads import SERVING_RANGE_SIZES, serving_models, serving_stream
+    full = args.serving_range == "full"
+    models = serving_models(sizes=SERVING_RANGE_SIZES) if full else serving_models()
     rates = [float(r) for r in args.rates.split(",")]
     srv = S.Server(rounds=args.rounds)
     for name, m in models.items():
-        srv.add_model(name, m["config"], m["mu"], m["rho"], m["spectrum"], m["source_dist"])
-    print(f"# serving: {len(models)} models {sorted(models)}, budget {srv.budget / 1e9:.0f} GB", file=sys.stderr,
+        c = m["config"]
+        if full:
+            try:
+                f = S.kernel_fw_nth(c, 8, srv.budget // 2)
+            except ValueError:  # the grid's other arrays take more than half the budget
+                try:
+                    f = S.kernel_fw_nth(c, 8, srv.budget)
+                except ValueError:  # kernel requests on this model are rejected
+                    f = c.fw_nth
+            if f != c.fw_nth:
+                m["config"] = c = c.replace(fw_nth=f)
+        srv.add_model(name, c, m["mu"], m["rho"], m["spectrum"], m["source_dist"])
+    fw = {n: models[n]["config"].fw_nth for n in sorted(models)}
+    print(f"# serving: {len(models)} models, budget {srv.budget / 1e9:.0f} GB, fw_nth {fw}", file=sys.stderr,
           flush=True)
 
     def requests(stream):
@@ -424,22 +444,27 @@ def serving_run(args):
             c = models[name]["config"]
             obs = torch.zeros((c.n_corr, rec.shape[0]), dtype=torch.float64, device="cuda")
             dmu = None
-            if kind == "hvp":
+            if kind == "hvp" and c.fw_nth == 1:
                 sgn = np.sign(np.random.default_rng(seed).standard_normal((c.nx, c.nz)))
                 dmu = D.to_device(0.01 * models[name]["mu"] * sgn)
-            out.append((off, S.Request(kind, name, src, rec, observed=obs, dmu=dmu)))
+            out.append((off, S.Request(kind, name, src, rec, observed=obs, dmu=dmu, tag=(name, kind, seed))))
         return out
 
-    # warm-up, as a server does when it loads its models: one request of each kind on every model
-    # (first-use allocations, launch-shape tuning and phase tables of each grid stay out of the runs)
+    # warm-up, as a server does when it loads its models: one request of each kind on every model up
+    # to 543^2 (first-use allocations and phase tables of each grid stay out of the runs)
     warm = requests([(0.0, k, n, models[n]["stations"][:1], models[n]["stations"][1:4], 1)
-                     for n in sorted(models) for k in ("correlation", "kernel", "hvp")])
+                     for n in sorted(models) if models[n]["config"].nx <= 543
+                     for k in ("correlation", "kernel", "hvp")])
     srv.serve(warm)
     srv.done.clear()
+    srv.rejected.clear()
     runs = []
+    last = []
     for rate in rates:
-        stream = requests(serving_stream(models, rate, args.requests, seed=int(rate * 1000) + 7))
+        stream = requests(serving_stream(models, rate, args.requests, seed=int(rate * 1000) + 7,
+                                         log_uniform=(200, 4000) if full else None))
         srv.done.clear()
+        srv.rejected.clear()
         srv.gcache.clear()
         srv.gcache_bytes = 0
         gp0 = srv.gp_updates
@@ -447,12 +472,26 @@ def serving_run(args):
         srv.serve(stream)
         el = time.perf_counter() - t0
         lat = srv.latencies() * 1e3
+        wait = srv.waits() * 1e3
         kinds = {k: sum(1 for _, r in stream if r.kind == k) for k in ("correlation", "kernel", "hvp")}
+        reasons = {}
+        for r in srv.rejected:
+            key = "hvp needs fw_nth = 1" if "fw_nth = 1" in str(r.error) else "over budget"
+            reasons[key] = reasons.get(key, 0) + 1
+        sizes = {}
+        for d in srv.done:
+            sizes.setdefault(models[d.req.model]["config"].nx, []).append((d.t_done - d.t_arrival) * 1e3)
+        served = len(srv.done)
         runs.append({"rate_req_s": rate, "requests": len(stream), "mix": kinds, "wall_s": el,
-                     "throughput_req_s": len(stream) / el, "gp_updates_per_s": (srv.gp_updates - gp0) / el,
+                     "served": served, "rejected": len(srv.rejected), "rejected_reasons": reasons,
+                     "throughput_req_s": served / el, "gp_updates_per_s": (srv.gp_updates - gp0) / el,
                      "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+                     "admission_wait_ms": {"p50": float(np.percentile(wait, 50)), "p99": float(np.percentile(wait, 99)),
+                                           "deferred": int((wait > 1.0).sum())},
+                     "latency_ms_by_size": {str(k): float(np.median(v)) for k, v in sorted(sizes.items())},
                      "spectrum_cache_hits": srv.cache_hits})
-        print(f"# rate {rate}: {len(stream) / el:.2f} req/s, p50 {runs[-1]['latency_ms']['p50']:.0f} ms, "
+        print(f"# rate {rate}: {served} served, {len(srv.rejected)} rejected {reasons}, {served / el:.2f} req/s, "
+              f"{runs[-1]['gp_updates_per_s'] / 1e9:.1f} G GP/s, p50 {runs[-1]['latency_ms']['p50']:.0f} ms, "
               f"p99 {runs[-1]['latency_ms']['p99']:.0f} ms", file=sys.stderr, flush=True)
         last = list(srv.done)
     # the plain path on the requests of the highest rate, each compared with its served result
@@ -463,22 +502,68 @@ def serving_run(args):
         same &= all(torch.equal(alone[k], live.result[k]) for k in alone)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
+    print(f"# plain path: {t_plain:.1f} s, identical {same}", file=sys.stderr, flush=True)
+    oracle_check = serving_oracle_check(srv, models, requests) if full else None
     top = runs[-1]
     srv.close()
+    sizes_txt = ("grids 200^2-4000^2 (7 log-spaced sizes, request size log-uniform), dx 2 km, nt=10 nx, "
+                 "fw_nth=1 up to 894^2 and coarser above (kernel snapshots within half the budget)") if full else \
+        "grids 200^2-400^2 (dx 2 km, nt=10 nx), fw_nth=1"
     out = {
         "metric": "Gpt-updates/s, correlations+kernels/s, p99 latency; FP32 vs plain, L2<=1e-5",
         "value": top["gp_updates_per_s"], "unit": "GP-updates/s", "n_gpus": 1, "steps": top["requests"], "warmup": 1,
         "ms_per_step": top["wall_s"] / top["requests"] * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 fields, f64 DFT accumulators/kernels", "data": "synthetic",
         "config": {"workload": "serving: Poisson stream, 50% correlation / 35% correlation+structure kernel / 15% "
-                               "structure HVP, grids 200^2-400^2 (dx 2 km, nt=10 nx), order 4 or 8, fw_nth=1, "
-                               "121 frequencies, 8 receivers",
-                   "models": sorted(models), "rates": rates},
+                               f"structure HVP, {sizes_txt}, order 4 or 8, 121 frequencies, 8 receivers",
+                   "models": sorted(models), "fw_nth": fw, "rates": rates},
         "runs": runs,
         "plain_req_s": len(last) / t_plain, "served_vs_plain": t_plain / top["wall_s"],
         "identical_to_plain": bool(same),
+        "oracle_check": oracle_check,
     }
     return out
+
+
+def serving_oracle_check(srv, models, requests):
+    """One request of each kind on the 200^2 models (orders 4 and 8), served together, each
+    against the FP64 oracle restatement of its program on the host (relative L2 per output)."""
+    import torch
+    import oracle
+    from oracle import requests as OR
+    from fd2d_noise_amd import device as D
+    names = [n for n in sorted(models) if models[n]["config"].nx == 200]
+    picks = [(0.0, kind, names[q % len(names)], models[names[q % len(names)]]["stations"][q:q + 1],
+              models[names[q % len(names)]]["stations"][4:7], 100 + q)
+             for q, kind in enumerate(("correlation", "kernel", "hvp"))]
+    srv.done.clear()
+    srv.serve(requests(picks))
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    orc = oracle.solvers()
+    tol = {"correlations": 1e-5, "K": 1e-4, "H": 1e-4}
+    out = []
+    for live in srv.done:
+        req = live.req
+        m = srv.models[req.model]
+        c = m.config
+        args = (orc, c, D.to_host(m.mu), D.to_host(m.rho), m.spectrum_host, D.to_host(m.source_dist),
+                np.atleast_2d(req.src), np.asarray(req.rec))
+        t0 = time.perf_counter()
+        if req.kind == "correlation":
+            ref = OR.correlation(*args)
+        elif req.kind == "kernel":
+            ref = OR.kernel(*args, D.to_host(req.observed), req.measurement, req.veldis, req.window)
+        else:
+            ref = OR.hvp(*args, D.to_host(req.observed), D.to_host(req.dmu), req.measurement, req.veldis, req.window)
+        el = time.perf_counter() - t0
+        errs = {k: _rel(D.to_host(live.result[k]), r) for k, r in ref.items()}
+        ok = all(errs[k] <= tol[k] for k in errs)
+        out.append({"kind": req.kind, "model": req.model, "rel_l2": errs, "within_tolerance": ok, "oracle_s": el})
+        print(f"# oracle check {req.kind} on {req.model}: {errs} ({el:.1f} s on {threads} threads)", file=sys.stderr,
+              flush=True)
+    torch.cuda.synchronize()
+    return {"requests": out, "tolerance": tol, "threads": threads}
 
 
 def serving_bench(args):

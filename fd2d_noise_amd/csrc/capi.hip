@@ -666,15 +666,17 @@ struct Solver {
 };
 
 // Blocked frequency-domain helpers: time samples are staged in FP32 slabs of up to kSpecSlab slices
-// (fewer when device memory is short) and contracted on the matrix cores (csrc/spectral.hip).
+// and contracted on the matrix cores (csrc/spectral.hip). The slab length depends only on the grid
+// and the sample count (so a solve sums in the same order wherever and whenever it runs): kSpecSlab
+// samples, halved while a slab would exceed kSlabBytes (grids above about 2900^2).
 int spec_cols(int K) { return (2 * K + 255) / 256 * 256; }
+
+constexpr size_t kSlabBytes = size_t(32) << 30;
 
 int slab_len(size_t P, size_t n_samples)
 {
     size_t T = std::min<size_t>(kSpecSlab, std::max<size_t>(n_samples, 1));
-    size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
-        while (T > 64 && T * P * sizeof(float) > free_b / 4) T /= 2;
+    while (T > 64 && T * P * sizeof(float) > kSlabBytes) T /= 2;
     // several slabs: each starts on a 16-sample boundary (the phase tables are read 8 samples at a time)
     if (T < n_samples) T = std::max<size_t>(16, T / 16 * 16);
     return (int)T;
@@ -2113,7 +2115,7 @@ int64_t solve_bytes(const fd2d_solve* s)
     const int64_t n_src = numel(s->src) ? dim(s->src, 0) : 0, n_rec = numel(s->rec) ? dim(s->rec, 0) : 0;
     b += (int64_t)(g.nx / kTX + 2) * (g.nz / kTileZ + 2) * 8 + 8 * 9 * (n_src + n_rec);  // tile lists
     const int N = 2 * g.nt - 1;
-    auto slab = [&](int64_t n) { return std::min<int64_t>(kSpecSlab, std::max<int64_t>(n, 1)); };
+    auto slab = [&](int64_t n) { return (int64_t)slab_len((size_t)P, (size_t)std::max<int64_t>(n, 1)); };
     // slab, phase-table parts and the accumulators' compensation terms
     auto dft_bytes = [&](int64_t n) { return slab(n) * P * 4 + 3 * ((n + 15) / 16 * 16) * C_pad * 2 + P * K * 16; };
     auto syn_bytes = [&](int64_t n) {

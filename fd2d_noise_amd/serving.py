@@ -7,9 +7,12 @@ a small program over solves: a generator that yields the solves of its next stag
 together), does its FP64 caller work on the device between stages (adjoint sources, sums) and
 returns its result. The server
 
-  * admits requests first come, first served, against an HBM budget that counts what each request
-    holds at its peak: its wavefield and spectrum arrays (stored forward snapshots included) and the
-    library's internal buffers of its solves (fd2d_solve_device_bytes);
+  * admits requests against an HBM budget that counts what each request holds at its peak: its
+    wavefield and spectrum arrays (stored forward snapshots included) and the library's internal
+    buffers of its solves (fd2d_solve_device_bytes). A request that cannot fit even alone, or a
+    Hessian-vector product on a model that does not store every step, is rejected at submission;
+    one that fits but not now waits, and later requests that fit may start before it (backfill)
+    a bounded number of times;
   * keeps models (mu, rho) and Green-function spectra resident (the spectra in an LRU cache,
     counted in the budget), so a correlation request for a station whose G_fft is cached skips the
     Green run, as the reference reuses G_fft_<i>.mat (eval_objective_and_gradient.m:44-53);
@@ -181,8 +184,12 @@ PROGRAMS: Dict[str, Callable] = {"correlation": correlation_program, "kernel": k
 
 def request_bytes(srv: "Server", req: Request) -> int:
     """Peak device bytes of a request: its arrays plus the library buffers of its largest stage."""
-    c = srv.models[req.model].config
-    P, K, nr, N = c.nx * c.nz, c.n_sample, req.rec.shape[0], c.n_corr
+    return request_bytes_for(req.kind, srv.models[req.model].config, req.rec.shape[0])
+
+
+def request_bytes_for(kind: str, c: Config, nr: int) -> int:
+    """Peak device bytes of a request of this kind on a model with config c and nr receivers."""
+    P, K, N = c.nx * c.nz, c.n_sample, c.n_corr
     spec_b = 16 * P * K
     g_snap, c_snap = 4 * P * c.n_fw_green(), 4 * P * c.n_fw_corr()
     green = B.solve_bytes(abi.FD2D_GREEN, c, 1, 1, 0)
@@ -190,13 +197,31 @@ def request_bytes(srv: "Server", req: Request) -> int:
     adj1 = B.solve_bytes(abi.FD2D_ADJOINT, c, 1, nr, nr, False, 0, False, (nr, N, 1), fft_out=True)
     adj0 = B.solve_bytes(abi.FD2D_ADJOINT, c, 0, nr, nr, False, 1, True, (c.nx, c.nz, K))
     traces = 8 * nr * N * 4
-    if req.kind == "correlation":
+    if kind == "correlation":
         return spec_b + max(green, corr) + traces
-    if req.kind == "kernel":
+    if kind == "kernel":
         return 2 * spec_b + g_snap + c_snap + 16 * P * 2 + max(green, corr, adj1, adj0) + traces
     # hvp: G, dG, uT, kT spectra; G, dG snapshots; C, dC, u_dag, p snapshots; 4 kernels; two
     # solves of a stage at once
     return 4 * spec_b + 2 * g_snap + 4 * c_snap + 4 * 16 * P + 2 * max(green, corr, adj1, adj0) + traces
+
+
+def kernel_fw_nth(c: Config, nr: int, budget: int) -> int:
+    """The smallest snapshot interval fw_nth (at least c.fw_nth) at which a kernel request on this
+    model fits in `budget` bytes: the stored forward wavefields (4 B per grid point per stored step
+    of the Green and the correlation run) dominate a kernel request, and the reference's own knob
+    for them is fw_nth (input_parameters.m). Raises ValueError when no interval fits."""
+    f = max(1, int(c.fw_nth))
+    while f <= c.nt:
+        if request_bytes_for("kernel", c.replace(fw_nth=f), nr) <= budget:
+            return f
+        f += max(1, f // 8)
+    raise ValueError(f"no fw_nth lets a kernel request on a {c.nx} x {c.nz} grid fit in {budget / 1e9:.1f} GB")
+
+
+class RequestRejected(Exception):
+    """A request the server cannot run: it does not fit the budget even alone, or its model cannot
+    serve its kind (a Hessian-vector product needs fw_nth = 1)."""
 
 
 @dataclasses.dataclass(eq=False)
@@ -205,6 +230,8 @@ class _Live:
     gen: Any
     bytes: int
     t_arrival: float
+    bypassed: int = 0  # later requests admitted while this one waited at the head of the queue
+    t_admit: float = 0.0
     tickets: List[int] = dataclasses.field(default_factory=list)
     specs: List[Any] = dataclasses.field(default_factory=list)
     t_done: float = 0.0
@@ -214,11 +241,21 @@ class _Live:
 
 class Server:
     """The serving loop on torch's current stream. ``budget_bytes`` defaults to 80% of the free
-    device memory at start; ``rounds`` is the number of engine rounds between completion checks."""
+    device memory at start; ``rounds`` is the number of engine rounds between completion checks.
 
-    def __init__(self, budget_bytes: Optional[int] = None, rounds: int = 32, spectrum_cache_bytes: Optional[int] = None):
+    Admission: a request whose peak bytes exceed the budget, or that its model cannot serve, is
+    rejected at submission (it ends at once with RequestRejected and is listed in ``rejected``). A
+    request that fits the budget but not beside the running ones waits (deferred). Requests start
+    in arrival order, except that a later request that fits now may start while the head of the
+    queue waits (backfill), at most ``max_bypass`` times per head; after that nothing starts
+    before the head, so a large request is not starved by a stream of small ones."""
+
+    def __init__(self, budget_bytes: Optional[int] = None, rounds: int = 32, spectrum_cache_bytes: Optional[int] = None,
+                 max_bypass: int = 8):
         free, _ = torch.cuda.mem_get_info()
         self.budget = int(budget_bytes if budget_bytes is not None else 0.8 * free)
+        self.max_bypass = int(max_bypass)
+        self.rejected: List[_Live] = []
         self.cache_cap = int(spectrum_cache_bytes if spectrum_cache_bytes is not None else self.budget // 4)
         self.rounds = rounds
         self.models: Dict[str, Model] = {}
@@ -270,25 +307,45 @@ class Server:
         if req.model not in self.models:
             raise KeyError(f"model {req.model!r} is not resident")
         live = _Live(req, None, request_bytes(self, req), time.perf_counter() if t_arrival is None else t_arrival)
-        if live.bytes > self.budget:
-            raise ValueError(f"request needs {live.bytes / 1e9:.1f} GB, more than the budget {self.budget / 1e9:.1f} GB")
+        c = self.models[req.model].config
+        reason = None
+        if req.kind == "hvp" and c.fw_nth != 1:
+            reason = (f"a Hessian-vector product needs fw_nth = 1 (eval_hessian_vector_product_2.m:75,154); model "
+                      f"{req.model!r} stores every {c.fw_nth}th step")
+        elif live.bytes > self.budget:
+            reason = f"the request needs {live.bytes / 1e9:.1f} GB at its peak, more than the budget {self.budget / 1e9:.1f} GB"
+        if reason is not None:
+            live.error, live.t_done = RequestRejected(reason), live.t_arrival
+            self.rejected.append(live)
+            return live
         self.queue.append(live)
         return live
 
+    def _room_for(self, b: int) -> bool:
+        """Whether b more bytes fit beside the running requests, dropping cached spectra (oldest
+        first) to make room."""
+        while self.in_use + self.gcache_bytes + b > self.budget and self.gcache:
+            _, old = self.gcache.popitem(last=False)
+            self.gcache_bytes -= old.numel() * old.element_size()
+        return self.in_use + self.gcache_bytes + b <= self.budget
+
     def _admit(self) -> None:
-        # first come, first served: the head waits until its bytes (plus the cache) fit
-        while self.queue:
-            head = self.queue[0]
-            while self.in_use + self.gcache_bytes + head.bytes > self.budget and self.gcache:
-                _, old = self.gcache.popitem(last=False)  # make room: drop cached spectra first
-                self.gcache_bytes -= old.numel() * old.element_size()
-            if self.in_use + self.gcache_bytes + head.bytes > self.budget:
-                return
-            self.queue.popleft()
-            self.in_use += head.bytes
-            head.gen = PROGRAMS[head.req.kind](self, head.req)
-            self.live.append(head)
-            self._resume(head)
+        q = 0
+        while q < len(self.queue):
+            head, live = self.queue[0], self.queue[q]
+            if q > 0 and head.bypassed >= self.max_bypass:
+                return  # the head has waited long enough: nothing more starts before it
+            if not self._room_for(live.bytes):
+                q += 1  # deferred; a later request may fit
+                continue
+            del self.queue[q]
+            if q > 0:
+                head.bypassed += 1
+            self.in_use += live.bytes
+            live.t_admit = time.perf_counter()
+            live.gen = PROGRAMS[live.req.kind](self, live.req)
+            self.live.append(live)
+            self._resume(live)
 
     def _finish(self, live: _Live, result=None, error: Optional[BaseException] = None) -> None:
         live.result, live.error = result, error
@@ -377,7 +434,12 @@ class Server:
         torch.cuda.synchronize()
 
     def latencies(self) -> np.ndarray:
+        """Arrival to result of every finished request (rejected ones are not in done)."""
         return np.array([d.t_done - d.t_arrival for d in self.done])
+
+    def waits(self) -> np.ndarray:
+        """Arrival to admission of every finished request: the time it was deferred."""
+        return np.array([max(0.0, d.t_admit - d.t_arrival) for d in self.done if d.t_admit])
 
     def close(self) -> None:
         self.engine.close()

@@ -23,10 +23,14 @@ serving: a Poisson stream of mixed requests -- 50% correlation, 35% correlation 
          15% structure Hessian-vector product -- on resident models of three grid sizes (200, 283 and
          400 nodes square, log-spaced; dx = 2 km, nt = 10 nx) at FD order 4 or 8, vs = 4 km/s times
          (1 + 0.05 g) with seeded random fields g, forward fields stored every step (the HVP needs
-         it), 8 candidate reference stations per model and 8 receivers per request.
+         it), 8 candidate reference stations per model and 8 receivers per request. The full
+         range: seven sizes from 200 to 4000 nodes square (SERVING_RANGE_SIZES, log-spaced), each
+         request's size drawn log-uniform in [200, 4000] and sent to the nearest model.
 The taper is the reference's Gaussian taper (init_absbound.m:18).
 """
 from __future__ import annotations
+
+from typing import Optional, Tuple
 
 import numpy as np
 
@@ -134,15 +138,30 @@ def serving_models(seed: int = 5, sizes=(200, 283, 400), orders=(4, 8)) -> dict:
     return out
 
 
-def serving_stream(models: dict, rate: float, n_requests: int, seed: int = 6, mix=(0.5, 0.35, 0.15)):
-    """(arrival offset [s], kind, model name, src (1 x 2), rec (8 x 2), seed) tuples of a Poisson stream."""
+# grid sizes of the full serving range: 200 to 4000 points a side, log-spaced (200 * 20^(k/6))
+SERVING_RANGE_SIZES = (200, 330, 543, 894, 1474, 2428, 4000)
+
+
+def serving_stream(models: dict, rate: float, n_requests: int, seed: int = 6, mix=(0.5, 0.35, 0.15),
+                   log_uniform: Optional[Tuple[float, float]] = None):
+    """(arrival offset [s], kind, model name, src (1 x 2), rec (8 x 2), seed) tuples of a Poisson
+    stream. The model of a request is drawn uniformly from `models`; with log_uniform = (lo, hi)
+    its grid size is drawn log-uniform in [lo, hi] instead and the request goes to the model of the
+    nearest size (in log distance; orders drawn uniformly)."""
     rng = np.random.default_rng(seed)
     names = sorted(models)
     t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
     kinds = rng.choice(["correlation", "kernel", "hvp"], size=n_requests, p=list(mix))
+    sizes = np.array(sorted({models[n]["config"].nx for n in names}), dtype=np.float64)
     out = []
     for q in range(n_requests):
-        name = names[rng.integers(len(names))]
+        if log_uniform is None:
+            name = names[rng.integers(len(names))]
+        else:
+            nx = np.exp(rng.uniform(np.log(log_uniform[0]), np.log(log_uniform[1])))
+            size = int(sizes[np.argmin(np.abs(np.log(sizes) - np.log(nx)))])
+            cands = [n for n in names if models[n]["config"].nx == size]
+            name = cands[rng.integers(len(cands))]
         m = models[name]
         c = m["config"]
         w = c.absorb_width

@@ -97,9 +97,88 @@ def test_serving_budget_defers_admission(torch_mod):
     _check(torch, srv2, reqs2[:4], srv2.done)
     with pytest.raises(KeyError):
         srv2.submit(S.Request("correlation", "zzz", reqs2[0].src, reqs2[0].rec))
-    small = S.Server(budget_bytes=1024)
-    small.models = srv2.models
-    with pytest.raises(ValueError):
-        small.submit(reqs2[0])
-    small.close()
     srv2.close()
+
+
+def test_serving_rejects_infeasible_requests(torch_mod):
+    """A request that cannot fit the budget even alone, and a Hessian-vector product on a model
+    that does not store every step, are rejected at submission with the reason; the others run."""
+    torch = torch_mod
+    from fd2d_noise_amd import serving as S
+    srv, reqs = _server(torch)
+    m = srv.models["a"]
+    srv.add_model("a4", m.config.replace(fw_nth=4), m.mu, m.rho, m.spectrum, m.source_dist)
+    hv = reqs[2]
+    assert hv.kind == "hvp" and hv.model == "a"
+    bad = S.Request("hvp", "a4", hv.src, hv.rec, observed=hv.observed, dmu=hv.dmu, window=hv.window, tag="a4")
+    r1 = srv.submit(bad)
+    assert isinstance(r1.error, S.RequestRejected) and "fw_nth = 1" in str(r1.error)
+    small = S.Server(budget_bytes=1024)
+    small.models = srv.models
+    r2 = small.submit(reqs[0])
+    assert isinstance(r2.error, S.RequestRejected) and "budget" in str(r2.error)
+    assert small.rejected == [r2] and not small.busy()
+    small.close()
+    srv.serve([(0.0, reqs[0]), (0.0, S.Request("kernel", "a4", reqs[5].src, reqs[5].rec, observed=reqs[5].observed,
+                                                window=reqs[5].window, tag="k4"))])
+    assert srv.rejected == [r1] and len(srv.done) == 2 and all(d.error is None for d in srv.done)
+    # a kernel on the model that stores every 4th step is the reference's kernel at fw_nth = 4
+    _check(torch, srv, reqs, srv.done)
+    srv.close()
+
+
+@pytest.mark.parametrize("max_bypass", [8, 0])
+def test_serving_backfill(torch_mod, max_bypass):
+    """A kernel request that does not fit beside a running one waits; a correlation request that
+    arrives after it and fits starts first (backfill), unless max_bypass = 0."""
+    torch = torch_mod
+    from fd2d_noise_amd import serving as S
+    srv, reqs = _server(torch)
+    ka, kb, ca = reqs[5], reqs[1], reqs[0]
+    assert (ka.kind, kb.kind, ca.kind) == ("kernel", "kernel", "correlation")
+    b = [S.request_bytes(srv, r) for r in (ka, kb, ca)]
+    srv.close()
+    assert b[2] < b[1]
+    budget = max(b[1], b[0] + b[2])  # room for ka and ca, or for kb alone, not for ka and kb
+    srv2, reqs2 = _server(torch, budget=budget)
+    srv2.max_bypass = max_bypass
+    ka, kb, ca = reqs2[5], reqs2[1], reqs2[0]
+    lives = [srv2.submit(r) for r in (ka, kb, ca)]
+    srv2.run_until_idle()
+    t = {lv.req.tag: lv.t_admit for lv in lives}
+    print(f"max_bypass {max_bypass}: admitted order {sorted(t, key=t.get)}", flush=True)
+    assert t[ka.tag] < t[kb.tag]
+    assert (t[ca.tag] < t[kb.tag]) == (max_bypass > 0)
+    assert len(srv2.done) == 3 and srv2.in_use == 0
+    _check(torch, srv2, reqs2, srv2.done)
+    srv2.close()
+
+
+def test_serving_requests_match_oracle(torch_mod, orc):
+    """Every served request of the mix against the FP64 oracle restatement of its program
+    (oracle/requests.py): correlations within 1e-5 relative L2, kernels and Hessian-vector products
+    within 1e-4."""
+    torch = torch_mod
+    from fd2d_noise_amd import device as D
+    from oracle import requests as OR
+    from conftest import rel_l2
+    srv, reqs = _server(torch)
+    srv.serve([(0.0, r) for r in reqs])
+    tol = {"correlations": 1e-5, "K": 1e-4, "H": 1e-4}
+    for live in sorted(srv.done, key=lambda d: d.req.tag):
+        req = live.req
+        m = srv.models[req.model]
+        c = m.config
+        args = (orc, c, D.to_host(m.mu), D.to_host(m.rho), m.spectrum_host, D.to_host(m.source_dist),
+                np.atleast_2d(req.src), req.rec)
+        if req.kind == "correlation":
+            ref = OR.correlation(*args)
+        elif req.kind == "kernel":
+            ref = OR.kernel(*args, D.to_host(req.observed), req.measurement, req.veldis, req.window)
+        else:
+            ref = OR.hvp(*args, D.to_host(req.observed), D.to_host(req.dmu), req.measurement, req.veldis, req.window)
+        for k, r in ref.items():
+            e = rel_l2(D.to_host(live.result[k]), r)
+            print(f"request {req.tag} {req.kind} on {req.model}: {k} rel L2 {e:.2e}", flush=True)
+            assert e < tol[k], f"request {req.tag} {k}: {e:.3e}"
+    srv.close()
