@@ -68,6 +68,8 @@ def parse():
                     help="Rademacher Hessian-vector products per L-BFGS iteration of the inversion workload")
     ap.add_argument("--rates", type=str, default="5,20,200", help="arrival rates [req/s] of the serving workload")
     ap.add_argument("--requests", type=int, default=48, help="requests per rate of the serving workload")
+    ap.add_argument("--stations-grid", choices=["config3", "config1"], default="config3",
+                    help="grid of the stations workload: 1200x1200 (config3) or 300x300 (config1)")
     ap.add_argument("--serving-range", choices=["small", "full"], default="small",
                     help="grid sizes of the serving workload: 200^2-400^2, or the full 200^2-4000^2 range")
     ap.add_argument("--stations", type=int, default=64, help="stations of the stations workload")
@@ -314,19 +316,22 @@ def cpu_baseline_and_accuracy(w, nt_sample, threads, crop=1000):
     return cpu, acc
 
 
-def stations_run(args):
-    """Green + correlation per reference station on the shared 1200x1200 grid: the plain path (one
-    solve at a time) against the batch engine with `window` stations in flight, each station's
-    correlation submitted as soon as its Green run is done. Checks that the two give identical
-    correlation traces on the stations both ran."""
+def stations_run(args, grid: str = "config3"):
+    """Green + correlation per reference station on a shared grid (config3: 1200x1200, config1:
+    300x300): the plain path (one solve at a time) against the batch engine with `window` stations
+    in flight, each station's correlation submitted as soon as its Green run is done. Checks that
+    the two give identical correlation traces on the stations both ran."""
     import torch
     from fd2d_noise_amd import batch as B, device as D
-    from fd2d_noise_amd.workloads import config3_workload
-    w = config3_workload(nt=args.nt or 4000, n_stations=args.stations)
+    from fd2d_noise_amd.workloads import config1_stations_workload, config3_workload
+    if grid == "config1":
+        w = config1_stations_workload(n_stations=args.stations)
+    else:
+        w = config3_workload(nt=args.nt or 4000, n_stations=args.stations)
     c = w["config"]
     S, nr, P = len(w["stations"]), w["rec"].shape[0], c.nx * c.nz
     cd, f64 = torch.complex128, torch.float64
-    print(f"# workload config3 {c.nx}x{c.nz} order {c.order} nt {c.nt} K {c.n_sample} nb {c.n_basis_fct} "
+    print(f"# workload {w['name']} {c.nx}x{c.nz} order {c.order} nt {c.nt} K {c.n_sample} nb {c.n_basis_fct} "
           f"stations {S} receivers {nr}", file=sys.stderr, flush=True)
     mu, rho, rec = D.to_device(w["mu"]), D.to_device(w["rho"]), D.to_device(w["rec"])
     spec, sd = D.to_device(w["spectrum"]), D.to_device(w["source_dist"])
@@ -384,7 +389,7 @@ def stations_run(args):
         "value": gp / el, "unit": "GP-updates/s", "n_gpus": 1, "steps": S, "warmup": 1,
         "ms_per_step": el / S * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 fields, f64 DFT accumulators", "data": "synthetic",
-        "config": {"workload": f"config3: {S} reference stations x {c.n_basis_fct} basis functions, "
+        "config": {"workload": f"{w['name']}: {S} reference stations x {max(c.n_basis_fct, 1)} basis functions, "
                                f"{c.nx}x{c.nz} order-{c.order} nt={c.nt}, {c.n_sample} frequencies, {nr} receivers, "
                                f"Green+correlation per station, {W} stations in flight",
                    "nx": c.nx, "nz": c.nz, "order": c.order, "nt": c.nt, "n_sample": c.n_sample,
@@ -402,7 +407,7 @@ def stations_run(args):
 
 
 def stations_bench(args):
-    print(json.dumps(stations_run(args)), flush=True)
+    print(json.dumps(stations_run(args, grid=args.stations_grid)), flush=True)
 
 
 def serving_run(args):
@@ -853,16 +858,26 @@ def main():
         a.nt, a.stations, a.window, a.plain = None, 64, 8, 4
         st = stations_run(a)
         print(f"# stations: {st['batched_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
-        a.rates, a.requests = "200", 24
+        # the same flow on the 300x300 grid of the CPU-sized case, every station in flight
+        a.window = 64
+        st1 = stations_run(a, grid="config1")
+        print(f"# stations (300x300): {st1['batched_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
+        # a steady stream: long enough that the longest request no longer sets the wall time
+        a.rates, a.requests = "200", 96
         sv = serving_run(a)
         print(f"# serving: {sv['served_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
+
+        def brief(r):
+            return {"workload": r["config"]["workload"], "correlations_per_s": r["correlations_per_s"],
+                    "plain_correlations_per_s": r["plain_correlations_per_s"], "batched_vs_plain": r["batched_vs_plain"],
+                    "identical_to_plain": r["identical_to_plain"], "gp_updates_per_s": r["value"],
+                    "latency_ms": r["latency_ms"], "roofline_whole_job": r["roofline_whole_job"]}
         side = {"stations_vs_plain": round(st["batched_vs_plain"], 3),
+                "stations_300_vs_plain": round(st1["batched_vs_plain"], 3),
                 "served_vs_plain": round(sv["served_vs_plain"], 3),
                 "batched_runs": {
-                    "stations": {"workload": st["config"]["workload"], "correlations_per_s": st["correlations_per_s"],
-                                 "plain_correlations_per_s": st["plain_correlations_per_s"],
-                                 "identical_to_plain": st["identical_to_plain"],
-                                 "latency_ms": st["latency_ms"], "roofline_whole_job": st["roofline_whole_job"]},
+                    "stations": brief(st),
+                    "stations_300": brief(st1),
                     "serving": {"workload": sv["config"]["workload"], "rate_req_s": sv["runs"][-1]["rate_req_s"],
                                 "requests": sv["runs"][-1]["requests"],
                                 "throughput_req_s": sv["runs"][-1]["throughput_req_s"],

@@ -5,6 +5,8 @@ config1: the reference-sized CPU case -- 2-D SH on 2000 x 2000 km, 300 x 300 nod
          taper on all four sides, Gaussian noise spectrum centred at 0.05 Hz (sigma 0.02 Hz) on 91
          frequencies in 0.01-0.10 Hz, a Gaussian source blob (sigma 200 km) at a seeded location
          plus a 0.1 background, the reference station at the centre and 2 receivers 300-700 km away.
+config1-stations: the config1 model with 64 reference stations seeded inside the absorbing frame,
+         each correlated at the two config1 receivers (the batched form of the CPU-sized case).
 config2: the single large solve -- 2400 x 2400 nodes on 2000 km (dx = 0.834 km), order 8,
          dt = 0.06 s, nt = 8000, vs = 4 km/s times (1 + 0.1 g) with g a seeded Gaussian random field
          (30 km correlation length, max |g| = 1), rho constant, forward fields stored every 4th step,
@@ -77,6 +79,18 @@ def config1_workload(seed: int = 1) -> dict:
         src=centre[None, :], rec=_ring(rng, centre, 2, 300e3, 700e3),
         spectrum=np.exp(-((f - 0.05) ** 2) / (2 * 0.02 ** 2))[:, None],
         source_dist=_blob(rng, c))
+
+
+def config1_stations_workload(seed: int = 1, n_stations: int = 64) -> dict:
+    """The config1 model, source and receivers with n_stations reference stations seeded uniformly
+    inside the absorbing frame (the batched form of the reference-sized case)."""
+    w = config1_workload(seed)
+    c = w["config"]
+    rng = np.random.default_rng(seed + 100)
+    a = c.absorb_width
+    w["stations"] = np.stack([rng.uniform(a, c.Lx - a, n_stations), rng.uniform(a, c.Lz - a, n_stations)], axis=1)
+    w["name"] = "config1-stations"
+    return w
 
 
 def config2_workload(seed: int = 2, nt: int = 8000) -> dict:
