@@ -312,12 +312,17 @@ __global__ __launch_bounds__(512) void k_synth_bf16(float* __restrict__ out, con
                                                      const unsigned short* __restrict__ th,
                                                      const unsigned short* __restrict__ tm,
                                                      const unsigned short* __restrict__ tl, long P, int C_pad,
-                                                     int cnt)
+                                                     int cnt, int nxb, int njg, int per_xcd)
 {
     __shared__ u32x4 Bs[2 * kSNB * 3 * 64];  // [buf][node block][part][half][col]
+    // workgroup g runs on XCD g % 8; there the njg slice groups of a node block follow each other,
+    // so the block's spectrum rows come from HBM once and from that XCD's L2 for the other groups
+    const int g = blockIdx.x, k = g / kXCD;
+    const int xb = (g % kXCD) * per_xcd + k / njg, sg = k % njg;
+    if (xb >= nxb) return;  // workgroup-uniform
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const long x0 = (long)blockIdx.x * kSX;
-    const int j0 = blockIdx.y * kSJ + w * 32;
+    const long x0 = (long)xb * kSX;
+    const int j0 = sg * kSJ + w * 32;
     const bool active = j0 < cnt;  // wave-uniform
     const int half = lane >> 5, col = lane & 31;
     const int sn = tid & (kSX - 1), sr = tid >> 7;  // staging: node, group of 4 rows
@@ -365,9 +370,11 @@ void launch_synth_bf16(float* out, const float* a, const unsigned short* th, con
                        const unsigned short* tl, size_t P, int C_pad, int cnt, hipStream_t s)
 {
     if (cnt <= 0) return;
-    const unsigned nxb = (unsigned)((P + kSX - 1) / kSX);
-    const unsigned njg = (unsigned)((cnt + kSJ - 1) / kSJ);
-    hipLaunchKernelGGL(k_synth_bf16, dim3(nxb, njg), dim3(kSW * 64), 0, s, out, a, th, tm, tl, (long)P, C_pad, cnt);
+    const int nxb = (int)((P + kSX - 1) / kSX);
+    const int njg = (cnt + kSJ - 1) / kSJ;
+    const int per_xcd = (nxb + kXCD - 1) / kXCD;
+    hipLaunchKernelGGL(k_synth_bf16, dim3((unsigned)(kXCD * per_xcd * njg)), dim3(kSW * 64), 0, s, out, a, th, tm, tl,
+                       (long)P, C_pad, cnt, nxb, njg, per_xcd);
 }
 
 
