@@ -116,7 +116,7 @@ struct BlockCache {
             (void)hipGetLastError();
             if (ev) (void)hipEventDestroy(ev);
             lk.unlock();
-            (void)hipStreamSynchronize(st);
+            if (hipStreamSynchronize(st) != hipSuccess) (void)hipGetLastError();
             (void)hipFree(p);
             return;
         }
@@ -2081,8 +2081,8 @@ struct Engine {
     ~Engine()
     {
         for (int q = 0; q < kLanes; ++q)
-            if (lanes[q]) (void)hipStreamSynchronize(lanes[q]);
-        (void)hipStreamSynchronize(st);
+            if (lanes[q] && hipStreamSynchronize(lanes[q]) != hipSuccess) (void)hipGetLastError();
+        if (hipStreamSynchronize(st) != hipSuccess) (void)hipGetLastError();
         StreamScope scope(st);
         act.clear();
         draining.clear();
