@@ -14,9 +14,9 @@ Files and variables as the reference writes and reads them:
                                    inversion/optlib_callback_fct/new_iteration.m:28-46
 
 Arrays keep MATLAB's column-major shapes (a MATLAB nx x nz array is an (nx, nz) Fortran-ordered numpy
-array). ``solution.mat`` of start_inversion.m:215 is MAT v7.3 (HDF5); HDF5 needs h5py, which this
-image lacks, so ``save_solution`` writes MAT v5, which MATLAB reads the same way for variables below
-2 GB.
+array). ``solution.mat`` of start_inversion.m:215 is MAT v7.3 (HDF5): ``save_solution`` writes it
+through ``mat73`` (the HDF5 C library), or as MAT v5 with ``version="5"``, which MATLAB reads the same
+way for variables below 2 GB; ``load_solution`` reads either.
 
 ``calculate_data`` is code/calculate_data.m:180-223 on the GPU: the Green and correlation runs of all
 reference stations go through one batch engine instead of a parfor loop, and the two files are
@@ -114,10 +114,35 @@ def load_model(path: str) -> dict:
     return conv(d)
 
 
-def save_solution(path: str, flag: int, mfinal: np.ndarray, usr_par_fields: dict) -> None:
-    """save('solution.mat', 'flag', 'mfinal', 'usr_par', '-v7.3') (start_inversion.m:215) as MAT v5."""
-    _save(path, {"flag": float(flag), "mfinal": np.asarray(mfinal, dtype=np.float64).reshape(-1, 1),
-                 "usr_par": usr_par_fields})
+def save_solution(path: str, flag: int, mfinal: np.ndarray, usr_par_fields: dict, version: str = "7.3") -> None:
+    """save('solution.mat', 'flag', 'mfinal', 'usr_par', '-v7.3') (start_inversion.m:215); version "5"
+    writes MAT v5 instead."""
+    v = {"flag": float(flag), "mfinal": np.asarray(mfinal, dtype=np.float64).reshape(-1, 1), "usr_par": usr_par_fields}
+    if version == "7.3":
+        from . import mat73
+        mat73.save(path, v)
+    elif version == "5":
+        _save(path, v)
+    else:
+        raise ValueError(f"unknown MAT-file version {version!r} (7.3 or 5)")
+
+
+def load_solution(path: str) -> dict:
+    """flag, mfinal and usr_par of a solution file (MAT v7.3 or v5)."""
+    with open(path, "rb") as f:
+        head = f.read(20)
+    if head.startswith(b"MATLAB 7.3"):
+        from . import mat73
+        return mat73.load(path)
+    d = _load(path)
+
+    def conv(s):
+        if isinstance(s, np.ndarray) and s.dtype == object and s.size == 1:
+            s = s.flat[0]
+        if hasattr(s, "_fieldnames"):
+            return {f: conv(getattr(s, f)) for f in s._fieldnames}
+        return s
+    return {k: conv(v) for k, v in d.items()}
 
 
 def calculate_data(config, mu, rho, spectrum, source_dist, array: np.ndarray, ref_stat: np.ndarray,
