@@ -440,7 +440,8 @@ struct Solver {
     bool uform;
     PaddedLayout lay;
     DBuf<float> fields;  // two sets of v, sxy, szy (, u); displacement form: v0, v1, then u0, u1 in FP64
-    DBuf<float> coefs;   // bv, dtmu
+    static constexpr size_t kBinfo = 64;  // words after the two coefficient arrays: binfo
+    DBuf<float> coefs;   // bv, dtmu, binfo
     DBuf<float> ax_d, az_d, axp_d, azp_d;
     float *v[2], *sxy[2], *szy[2], *u[2];
     double* u64[2] = {nullptr, nullptr};  // displacement form: origin pointers of the FP64 u arrays
@@ -464,8 +465,8 @@ struct Solver {
         const int nf = uform ? 3 : with_u ? 4 : 3;
         fields.alloc(E * nf * 2);
         HIPCHK(hipMemsetAsync(fields.p, 0, E * nf * 2 * sizeof(float), st));
-        coefs.alloc(E * 2);
-        HIPCHK(hipMemsetAsync(coefs.p, 0, E * 2 * sizeof(float), st));
+        coefs.alloc(E * 2 + kBinfo);
+        HIPCHK(hipMemsetAsync(coefs.p, 0, (E * 2 + kBinfo) * sizeof(float), st));
         std::vector<float> ax, az;
         taper(cfg, g, ax, az);
         upload(ax_d, ax, st);
@@ -491,6 +492,7 @@ struct Solver {
         }
         a.bv = coefs.p + o;
         a.dtmu = coefs.p + E + o;
+        a.binfo = coefs.p + 2 * E;  // past the pool's buffer range (cbytes)
         a.ax = ax_d.p;
         a.az = az_d.p;
         a.axp = axp_d.p;
@@ -524,7 +526,7 @@ struct Solver {
     void material(const double* mu, const double* rho)
     {
         launch_material(mu, rho, coefs.p + lay.origin(), coefs.p + lay.elems() + lay.origin(), g.nx, g.nz,
-                        lay.pitch, g.dt, uform ? 1.0 : g.dt, st);
+                        lay.pitch, g.dt, uform ? 1.0 : g.dt, coefs.p + 2 * lay.elems(), st);
     }
     // the FP32 part of the pool a finiteness check reads (the FP64 halves of u are not FP32 values)
     const float* check_ptr() const { return fields.p; }

@@ -63,6 +63,7 @@ struct StepArgs {
     float* u_out;
     const float* bv;     // dt / rho
     const float* dtmu;   // dt * mu
+    const float* binfo;  // {dt/rho uniform, its value, mu uniform, its value}: see make_grid
     const float* ax;     // separable taper, nx values
     const float* az;     // nz values
     const float* axp;    // the taper padded like the fields: axp[x + kGhostX], azp[z + kGhost], zero
@@ -204,9 +205,10 @@ void launch_born_green(int R, const StepArgs& a, float* sxy, float* szy, const f
                        hipStream_t s);
 
 // Material coefficients: bv = dt/rho, dtmu = mu_scale*mu (dt*mu for the velocity-stress step, mu for the
-// displacement-form step) into padded arrays.
+// displacement-form step) into padded arrays, and binfo = {dt/rho uniform, its value, mu uniform, its
+// value, two int flags} (6 words).
 void launch_material(const double* mu, const double* rho, float* bv, float* dtmu, int nx, int nz,
-                     int pitch, double dt, double mu_scale, hipStream_t s);
+                     int pitch, double dt, double mu_scale, float* binfo, hipStream_t s);
 
 // Displacement-form steps (correlation and adjoint runs without Born terms, R = 3 or 4): the field
 // pool holds v (FP32) and u (FP64) only, o_u_in / o_u_out are FP32-element offsets of the FP64

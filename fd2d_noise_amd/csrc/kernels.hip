@@ -86,12 +86,15 @@ __device__ __forceinline__ void gst(T* p, int i, T v)
 struct Grid {
     int nx, nz, pitch;
     __amdgpu_buffer_rsrc_t rf, rc;
+    int o_b, o_m;    // offsets of the dt/rho and mu arrays in the coefficient pool
+    float cb, cm;    // added to every coefficient load (the value of a uniform coefficient, else 0)
     __device__ unsigned pofs(int arr, int xx, int zz) const
     {
         return (unsigned)(arr + (zz + kGhost) * pitch + xx + kGhostX) * 4u;
     }
     __device__ float field(int arr, int xx, int zz) const { return bload(rf, pofs(arr, xx, zz)); }
-    __device__ float coef(int arr, int xx, int zz) const { return bload(rc, pofs(arr, xx, zz)); }
+    __device__ float bcoef(int xx, int zz) const { return bload(rc, pofs(o_b, xx, zz)) + cb; }
+    __device__ float mcoef(int xx, int zz) const { return bload(rc, pofs(o_m, xx, zz)) + cm; }
     __device__ void put(int arr, int xx, int zz, float v) const { bstore(rf, pofs(arr, xx, zz), v); }
     // byte offset of (xx, zz) in a compact nx x nz array, or past its end outside the grid
     __device__ unsigned cofs(int xx, int zz) const
@@ -99,6 +102,20 @@ struct Grid {
         return (xx >= 0 && xx < nx && zz >= 0 && zz < nz) ? (unsigned)(zz * nx + xx) * 4u : kOutside;
     }
 };
+
+// The grid of a step. A coefficient with the same value at every node (a.binfo: {dt/rho uniform,
+// its value, mu uniform, its value}, written at set-up) is taken from binfo, and its array loads
+// point past the end of the coefficient pool, where buffer loads return 0 and move no data: a
+// homogeneous density or modulus costs no HBM traffic. The values are those the array holds, so the
+// results are the same bit for bit (outside the grid the 0/1 frames and the zero taper still give 0).
+__device__ __forceinline__ Grid make_grid(const StepArgs& a)
+{
+    const float* bi = a.binfo;
+    const bool ub = bi[0] != 0.f, um = bi[2] != 0.f;
+    const int past = (int)(a.cbytes >> 2);
+    return Grid{a.nx, a.nz, a.pitch, rsrc(a.fbase, a.fbytes), rsrc(a.cbase, a.cbytes), ub ? past : a.o_bv,
+                um ? past : a.o_dtmu, ub ? bi[1] : 0.f, um ? bi[3] : 0.f};
+}
 
 template <int R, int MODE>
 __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, int t)
@@ -122,7 +139,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x = i0 + lane;                 // this lane's own column
-    const Grid G{nx, nz, a.pitch, rsrc(a.fbase, a.fbytes), rsrc(a.cbase, a.cbytes)};
+    const Grid G = make_grid(a);
     const unsigned cb = a.compact_bytes;
     const bool has_syn = st.syn != nullptr;
     const auto r_syn = rsrc(has_syn ? (const void*)st.syn : (const void*)a.fbase, has_syn ? cb : 0u);
@@ -140,7 +157,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         const int r = w + TW * k, z = j0 - (R - 1) + r;
         msx[k] = G.field(a.o_sxy_in, x, z);
         mv[k] = G.field(a.o_v_in, x, z);
-        mb[k] = G.coef(a.o_bv, x, z);
+        mb[k] = G.bcoef(x, z);
         mu[k] = (HAS_U && ((r >= R - 1 && r < R - 1 + TZ) || kp)) ? G.field(a.o_u_in, x, z) : 0.f;
         msy[k] = has_syn ? bload(r_syn, G.cofs(x, z)) : 0.f;
     }
@@ -171,7 +188,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         const int c = h < R - 1 ? h : h + kTX;
         const int xx = i0 - (R - 1) + c, zz = j0 - (R - 1) + r;
         hv[k] = G.field(a.o_v_in, xx, zz);
-        hb[k] = G.coef(a.o_bv, xx, zz);
+        hb[k] = G.bcoef(xx, zz);
         hsy[k] = has_syn ? bload(r_syn, G.cofs(xx, zz)) : 0.f;
         ht[k] = gld(a.axp, xx + kGhostX) * gld(a.azp, zz + kGhost);
         hu[k] = (HAS_U && kp) ? G.field(a.o_u_in, xx, zz) : 0.f;
@@ -294,7 +311,7 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     // ---------------- phase 3
     float rm[NT];
 #pragma unroll
-    for (int k = 0; k < NT; ++k) rm[k] = G.coef(a.o_dtmu, x, j0 + w + TW * k);
+    for (int k = 0; k < NT; ++k) rm[k] = G.mcoef(x, j0 + w + TW * k);
     const float fxs = frame(x, R - 1, nx - R - 1);  // dx_v frame
     const int c = lane + R - 1;
     const bool born = st.born_dmu != nullptr;
@@ -461,7 +478,7 @@ __device__ __forceinline__ void pair_step(const StepArgs& a, const PairStep& ps,
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x = i0 + lane;
-    const Grid G{nx, nz, a.pitch, rsrc(a.fbase, a.fbytes), rsrc(a.cbase, a.cbytes)};
+    const Grid G = make_grid(a);
     const unsigned cb = a.compact_bytes;
     // the synthesised source of the pair belongs to step n (ps.syn) or to step n+1 (syn2)
     const float* syn = ps.syn ? ps.syn : ps.syn2;
@@ -499,7 +516,7 @@ __device__ __forceinline__ void pair_step(const StepArgs& a, const PairStep& ps,
         for (int k = 0; k < NM1; ++k) {
             const int r = w + PW * k, z = j0 - L1 + r;
             mv[k] = G.field(a.o_v_in, x, z);
-            mb[k] = G.coef(a.o_bv, x, z);
+            mb[k] = G.bcoef(x, z);
             mu[k] = (HAS_U && r >= L1 && r < L1 + TZ) ? G.field(a.o_u_in, x, z) : 0.f;
             msy[k] = has_syn ? bload(r_syn, G.cofs(x, z)) : 0.f;
         }
@@ -510,7 +527,7 @@ __device__ __forceinline__ void pair_step(const StepArgs& a, const PairStep& ps,
             const int c = h < L1 ? h : h + kTX;
             const int xx = i0 - L1 + c, zz = j0 - L1 + r;
             hv[k] = G.field(a.o_v_in, xx, zz);
-            hb[k] = G.coef(a.o_bv, xx, zz);
+            hb[k] = G.bcoef(xx, zz);
             hsy[k] = has_syn ? bload(r_syn, G.cofs(xx, zz)) : 0.f;
             ht[k] = gld(a.axp, xx + kGhostX) * gld(a.azp, zz + kGhost);
         }
@@ -576,20 +593,20 @@ __device__ __forceinline__ void pair_step(const StepArgs& a, const PairStep& ps,
     constexpr int NDX = (HS1X * HCX1 + PT - 1) / PT, NDZ = (HS1Z * HCZ1 + PT - 1) / PT;
     float dmm[NMD], dmx[NDX], dmz[NDZ];
 #pragma unroll
-    for (int k = 0; k < NMD; ++k) dmm[k] = G.coef(a.o_dtmu, x, j0 - (2 * R - 1) + w + PW * k);
+    for (int k = 0; k < NMD; ++k) dmm[k] = G.mcoef(x, j0 - (2 * R - 1) + w + PW * k);
 #pragma unroll
     for (int k = 0; k < NDX; ++k) {
         const int idx = min(tid + PT * k, HS1X * HCX1 - 1);
         const int r = idx / HCX1, h = idx - r * HCX1;
         const int xx = h < 2 * R - 1 ? i0 - (2 * R - 1) + h : i0 + kTX + (h - (2 * R - 1));
-        dmx[k] = G.coef(a.o_dtmu, xx, j0 - (R - 1) + r);
+        dmx[k] = G.mcoef(xx, j0 - (R - 1) + r);
     }
 #pragma unroll
     for (int k = 0; k < NDZ; ++k) {
         const int idx = min(tid + PT * k, HS1Z * HCZ1 - 1);
         const int r = idx / HCZ1, h = idx - r * HCZ1;
         const int xx = h < R - 1 ? i0 - (R - 1) + h : i0 + kTX + (h - (R - 1));
-        dmz[k] = G.coef(a.o_dtmu, xx, j0 - (2 * R - 1) + r);
+        dmz[k] = G.mcoef(xx, j0 - (2 * R - 1) + r);
     }
 
     const float fxl = frame(x, R, nx - R - 1);
@@ -821,7 +838,7 @@ __device__ __forceinline__ void utile_step(const StepArgs& a, const JobStep& st,
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int x = i0 + lane;
-    const Grid G{nx, nz, pitch, rsrc(a.fbase, a.fbytes), rsrc(a.cbase, a.cbytes)};
+    const Grid G = make_grid(a);
     const unsigned cb = a.compact_bytes;
     const bool has_syn = st.syn != nullptr;
     const auto r_syn = rsrc(has_syn ? (const void*)st.syn : (const void*)a.fbase, has_syn ? cb : 0u);
@@ -846,18 +863,18 @@ __device__ __forceinline__ void utile_step(const StepArgs& a, const JobStep& st,
         hu[k] = bload8(G.rf, pofs8(a.o_u_in, pitch, i0 - H + c, j0 + r));
     }
 #pragma unroll
-    for (int k = 0; k < NRZ; ++k) mz[k] = G.coef(a.o_dtmu, x, j0 - R + w + TW * k);
+    for (int k = 0; k < NRZ; ++k) mz[k] = G.mcoef(x, j0 - R + w + TW * k);
 #pragma unroll
     for (int k = 0; k < NHM; ++k) {
         const int idx = min(tid + TT * k, TZ * 2 * R - 1);
         const int r = idx / (2 * R), h = idx - r * 2 * R;
-        hm[k] = G.coef(a.o_dtmu, i0 - R + (h < R ? h : h + kTX), j0 + r);
+        hm[k] = G.mcoef(i0 - R + (h < R ? h : h + kTX), j0 + r);
     }
 #pragma unroll
     for (int k = 0; k < NT; ++k) {
         const int z = j0 + w + TW * k;
         mv[k] = G.field(a.o_v_in, x, z);
-        mb[k] = G.coef(a.o_bv, x, z);
+        mb[k] = G.bcoef(x, z);
         msy[k] = has_syn ? bload(r_syn, G.cofs(x, z)) : 0.f;
     }
     constexpr int NRF = (HF + TW - 1) / TW, NHF = (HF * HCF + TT - 1) / TT;
@@ -1280,23 +1297,48 @@ void launch_born_green(int R, const StepArgs& a, float* sxy, float* szy, const f
 // ---------------------------------------------------------------------------------------------
 // Set-up and element-wise kernels
 // ---------------------------------------------------------------------------------------------
+// binfo: {dt/rho uniform (1/0), its value, mu uniform, its value, (int) dt/rho differs, (int) mu differs}
+__global__ void k_binfo_init(float* binfo)
+{
+    int* d = (int*)(binfo + 4);
+    d[0] = 0;
+    d[1] = 0;
+}
+
 __global__ void k_material(const double* mu, const double* rho, float* bv, float* dtmu, int nx, int nz,
-                           int pitch, double dt, double mu_scale)
+                           int pitch, double dt, double mu_scale, float* binfo)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= (long)nx * nz) return;
     const int i = (int)(c % nx), j = (int)(c / nx);
     const long p = (long)j * pitch + i;
-    bv[p] = (float)(dt / rho[c]);
-    dtmu[p] = (float)(mu_scale * mu[c]);
+    const float b = (float)(dt / rho[c]), m = (float)(mu_scale * mu[c]);
+    bv[p] = b;
+    dtmu[p] = m;
+    // a value other than node 0's marks the coefficient as varying (one store per wavefront)
+    const bool db = !(b == (float)(dt / rho[0])), dm = !(m == (float)(mu_scale * mu[0]));
+    int* d = (int*)(binfo + 4);
+    if (__any(db) && (threadIdx.x & 63) == __builtin_ffsll(__ballot(1)) - 1) __hip_atomic_store(d, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (__any(dm) && (threadIdx.x & 63) == __builtin_ffsll(__ballot(1)) - 1) __hip_atomic_store(d + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_binfo_final(const double* mu, const double* rho, double dt, double mu_scale, float* binfo)
+{
+    const int* d = (const int*)(binfo + 4);
+    binfo[0] = d[0] ? 0.f : 1.f;
+    binfo[1] = (float)(dt / rho[0]);
+    binfo[2] = d[1] ? 0.f : 1.f;
+    binfo[3] = (float)(mu_scale * mu[0]);
 }
 
 void launch_material(const double* mu, const double* rho, float* bv, float* dtmu, int nx, int nz,
-                     int pitch, double dt, double mu_scale, hipStream_t s)
+                     int pitch, double dt, double mu_scale, float* binfo, hipStream_t s)
 {
     const long n = (long)nx * nz;
+    hipLaunchKernelGGL(k_binfo_init, dim3(1), dim3(1), 0, s, binfo);
     hipLaunchKernelGGL(k_material, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, mu, rho, bv, dtmu,
-                       nx, nz, pitch, dt, mu_scale);
+                       nx, nz, pitch, dt, mu_scale, binfo);
+    hipLaunchKernelGGL(k_binfo_final, dim3(1), dim3(1), 0, s, mu, rho, dt, mu_scale, binfo);
 }
 
 __global__ void k_scale_to_float(const double* in, float* out, size_t n, double scale)
