@@ -816,7 +816,13 @@ def main():
     steps_per_launch = d["steps"] / d["launches"]
     r = c.order // 2
     uform = best != "green" and r >= 3  # correlation and adjoint runs at orders 6 / 8: displacement form
-    ab = alg_bytes(best, c.fw_nth, best == "adjoint", uform)
+    # achieved: the per-GP-update algorithmic bytes of the reference's step (velocity-stress form) x
+    # the GP-updates of one launch / its duration. Beside it, the bytes this kernel itself must move:
+    # the displacement form's 32 B (+ snapshot, + K_mu read-modify-write) less the loads of a
+    # coefficient that is the same at every node (read from the set-up record instead)
+    ab = s8d_bytes(best, c.fw_nth)
+    kb = alg_bytes(best, c.fw_nth, best == "adjoint", uform)
+    kb -= 4.0 * bool(np.all(w["rho"] == w["rho"].flat[0])) + 4.0 * bool(np.all(w["mu"] == w["mu"].flat[0]))
     bytes_launch = ab * c.nx * c.nz * steps_per_launch
     achieved = bytes_launch / (launch_ms * 1e-3) / 1e9
     kname = "k_pair" if steps_per_launch > 1.5 else "k_tile"
@@ -836,7 +842,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kkey, c),
             "kernel": f"{kdesc} ({best} run, {steps_per_launch:.2f} time steps per launch)",
             "avg_launch_us": round(launch_ms * 1e3, 2), "steps_per_launch": round(steps_per_launch, 3),
-            "alg_bytes_per_gp": ab}
+            "alg_bytes_per_gp": ab, "kernel_bytes_per_gp": kb,
+            "kernel_GBps": round(kb * c.nx * c.nz * steps_per_launch / (launch_ms * 1e-3) / 1e9, 1)}
     # the whole pipeline against the reference's velocity-stress bytes per GP-update
     s8d = (c.nx * c.nz * (c.nt * s8d_bytes("green", c.fw_nth) + 2 * c.n_corr * s8d_bytes("correlation", c.fw_nth)
                           + c.nt * s8d_bytes("adjoint", c.fw_nth))) * args.steps / el / 1e9
