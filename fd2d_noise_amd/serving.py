@@ -240,8 +240,8 @@ class _Live:
 
 
 class Server:
-    """The serving loop on torch's current stream. ``budget_bytes`` defaults to 80% of the free
-    device memory at start; ``rounds`` is the number of engine rounds between completion checks.
+    """The serving loop on torch's current stream. ``budget_bytes`` defaults to 80% of the device
+    memory available at start; ``rounds`` is the number of engine rounds between completion checks.
 
     Admission: a request whose peak bytes exceed the budget, or that its model cannot serve, is
     rejected at submission (it ends at once with RequestRejected and is listed in ``rejected``). A
@@ -252,8 +252,9 @@ class Server:
 
     def __init__(self, budget_bytes: Optional[int] = None, rounds: int = 32, spectrum_cache_bytes: Optional[int] = None,
                  max_bypass: int = 8):
-        free, _ = torch.cuda.mem_get_info()
-        self.budget = int(budget_bytes if budget_bytes is not None else 0.8 * free)
+        # default: 80% of the device memory available at start (free, plus what torch's caching
+        # allocator holds unused, after the library has returned its cached blocks)
+        self.budget = int(budget_bytes if budget_bytes is not None else 0.8 * B.available_bytes())
         self.max_bypass = int(max_bypass)
         self.rejected: List[_Live] = []
         self.cache_cap = int(spectrum_cache_bytes if spectrum_cache_bytes is not None else self.budget // 4)

@@ -107,7 +107,7 @@ def test_serving_full_range_admission(torch_mod, orc):
     from fd2d_noise_amd.workloads import SERVING_RANGE_SIZES, serving_models
     from oracle import requests as OR
     models = serving_models(sizes=SERVING_RANGE_SIZES, orders=(8,))
-    # the budget is 80% of the free memory: return what earlier tests left cached
+    # the budget is 80% of the available memory: return what earlier tests left cached
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     from fd2d_noise_amd import _lib
