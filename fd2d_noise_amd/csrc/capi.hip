@@ -1173,7 +1173,7 @@ struct GreenJob : Job {
     double2* Gf = nullptr;
     float* Gout = nullptr;
     double* sg = nullptr;
-    DBuf<float> dtdmu;
+    DBuf<float> dtdmu, ndtdmu;  // dt * dmu and -dt * dmu (FP32)
     DftAccumulator dft;
 
     GreenJob(const fd2d_config* c, const fd2d_carray* mu, const fd2d_carray* rho, const fd2d_carray* src,
@@ -1212,6 +1212,8 @@ struct GreenJob : Job {
         if (born) {
             dtdmu.alloc(g.P);
             launch_scale_to_float((const double*)dmu->data, dtdmu.p, g.P, g.dt, st);
+            ndtdmu.alloc(g.P);
+            launch_scale_to_float((const double*)dmu->data, ndtdmu.p, g.P, -g.dt, st);
         }
         if (records) HIPCHK(hipMemsetAsync(sg, 0, sizeof(double) * (size_t)nr * g.nt, st));
         // sampled steps of :176 and their times t(n) = (n-1) dt (fft_coeff, :81-90)
@@ -1228,8 +1230,18 @@ struct GreenJob : Job {
             if (Gout) *sl.snap = Gout + (size_t)i_fw_out * g.P;
             ++i_fw_out;
         }
-        if (born) S->born_green(dtdmu.p, (const float*)G_in->data + (size_t)(n - 1) * g.P);
-        else if (!sid.empty()) S->point_sources(sub, nullptr, 1.0e9f);  // stf = 1e9 every step (:64)
+        if (born) {
+            // sigma += dt dmu dx_v(G_in(:,:,n)) before step n's divergence (:141-147): for n = 1 a
+            // launch of its own, for the later steps inside the previous step's stress update, which
+            // is the last change to sigma before it
+            if (n == 1) S->born_green(dtdmu.p, (const float*)G_in->data);
+            if (n < g.nt) {
+                S->a.born_dmu = ndtdmu.p;
+                S->a.born_in0 = (const float*)G_in->data + (size_t)n * g.P;
+            }
+        } else if (!sid.empty()) {
+            S->point_sources(sub, nullptr, 1.0e9f);  // stf = 1e9 every step (:64)
+        }
         if (sampled(n)) *sl.dsamp = dft.next_slot();
         if (records) S->receivers(sub, sg + (size_t)nr * (n - 1));
     }
@@ -2127,7 +2139,7 @@ int64_t solve_bytes(const fd2d_solve* s)
     const bool born = numel(s->dmu) != 0;
     if (s->solver == FD2D_GREEN) {
         if (numel(s->out[0]) && K) b += dft_bytes(g.nt / std::max(g.s, 1) + 1);
-        if (born) b += P * 4;
+        if (born) b += P * 8;  // dt dmu and -dt dmu
     } else if (s->solver == FD2D_CORRELATION) {
         b += syn_bytes(N / std::max(g.s, 1)) + K * (int64_t)std::max(cfg->n_basis_fct, 1) * 8 + K * 8;
         if (born) b += P * 4;

@@ -315,8 +315,10 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
     const float fxs = frame(x, R - 1, nx - R - 1);  // dx_v frame
     const int c = lane + R - 1;
     const bool born = st.born_dmu != nullptr;
+    // born_in1 is null for the Green run's Born term, which has one snapshot: its loads return 0
+    const bool born1 = born && st.born_in1 != nullptr;
     const auto r_b0 = rsrc(born ? (const void*)st.born_in0 : (const void*)a.fbase, born ? cb : 0u);
-    const auto r_b1 = rsrc(born ? (const void*)st.born_in1 : (const void*)a.fbase, born ? cb : 0u);
+    const auto r_b1 = rsrc(born1 ? (const void*)st.born_in1 : (const void*)a.fbase, born1 ? cb : 0u);
     const auto r_bd = rsrc(born ? (const void*)st.born_dmu : (const void*)a.fbase, born ? cb : 0u);
 #pragma unroll
     for (int k = 0; k < NT; ++k) {
@@ -333,7 +335,8 @@ __device__ __forceinline__ void tile_step(const StepArgs& a, const JobStep& st, 
         float snx = sx[r * WSX + c + R] + dm * (ex * fxs);
         float snz = sz[(r + R) * WV + c] + dm * (ez * fzs);
         if (born) {
-            // Born terms: - dmu * D(C_in(n) - C_in(n+1)) (FP32 snapshots, as the reference)
+            // Born terms: - dmu * D(C_in(n) - C_in(n+1)) (FP32 snapshots, as the reference); in the
+            // Green run + dt dmu D G_in(n+1), the next step's term (born_dmu = -dt dmu, no C_in(n+1))
             const unsigned co = G.cofs(x, z);
             auto bd = [&](unsigned o) { return bload(r_b0, o) - bload(r_b1, o); };
             float bx = 0.f, bz = 0.f;
