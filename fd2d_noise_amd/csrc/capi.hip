@@ -1231,7 +1231,7 @@ struct GreenJob : Job {
             ++i_fw_out;
         }
         if (born) {
-            // sigma += dt dmu dx_v(G_in(:,:,n)) before step n's divergence (:141-147): for n = 1 a
+            // sigma += dt dmu dx_v(G_in(:,:,n)) before step n's divergence (:140-144): for n = 1 a
             // launch of its own, for the later steps inside the previous step's stress update, which
             // is the last change to sigma before it
             if (n == 1) S->born_green(dtdmu.p, (const float*)G_in->data);
