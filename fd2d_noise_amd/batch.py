@@ -19,7 +19,7 @@ outputs). A spec keeps its tensors alive; the engine keeps each submitted spec u
 from __future__ import annotations
 
 import ctypes
-from typing import List, Optional, Sequence
+from typing import Optional, Sequence
 
 import numpy as np
 import torch

@@ -21,7 +21,7 @@ import ctypes
 import dataclasses
 import math
 import string
-from typing import Any, Callable, List, Optional, Sequence
+from typing import Any, List, Optional, Sequence
 
 import numpy as np
 
