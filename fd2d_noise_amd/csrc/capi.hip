@@ -818,24 +818,33 @@ std::string table_key(char kind, const fd2d_config* cfg, const std::vector<doubl
 
 struct DftAccumulator {
     double2* acc = nullptr;
-    DBuf<double2> comp;  // Neumaier compensation of acc: the sum is acc + comp
+    DBuf<double2> comp;  // Neumaier compensation of acc: the sum is acc + comp (series of several slabs)
     size_t P = 0;
     int K = 0, C_pad = 0, Jt = 0;
     DBuf<float> slab;
     // [C_pad][Jt]: parts of (Re, Im) of each sample's phase factor
     std::shared_ptr<const DeviceTable> tab;
-    int cnt = 0, done = 0, T = 0;
+    int cnt = 0, done = 0, T = 0, n_total = 0;
+    // where the last slab writes the finished sums: out = scale * (conj ? conj(sum) : sum)
+    double2* out = nullptr;
+    double out_scale = 1.0;
+    int out_conj = 0;
     hipStream_t st = nullptr;
-    // phase factors e^{sign i w_k t_j} * scale at the sample times ts
+    // phase factors e^{sign i w_k t_j} * scale at the sample times ts; the finished series goes to
+    // out_ (which may be acc_) as out_scale_ * (conj ? conj(sum) : sum)
     void init(double2* acc_, size_t P_, const fd2d_config* cfg, const std::vector<double>& ts, double sign,
-              double scale, hipStream_t s)
+              double scale, double2* out_, double out_scale_, int out_conj_, hipStream_t s)
     {
         acc = acc_;
         P = P_;
         K = cfg->n_sample;
         st = s;
+        out = out_;
+        out_scale = out_scale_;
+        out_conj = out_conj_;
         C_pad = spec_cols(K);
         const size_t n_samples = ts.size();
+        n_total = (int)n_samples;
         T = slab_len(P, n_samples);  // one slab, or a multiple of 16 samples
         Jt = (int)((n_samples + 15) / 16 * 16);
         slab.alloc((size_t)T * P);
@@ -848,22 +857,30 @@ struct DftAccumulator {
                     e.set(2 * k + 1, j, table[j * K + k].y);
                 }
         });
-        HIPCHK(hipMemsetAsync(acc, 0, P * (size_t)K * sizeof(double2), st));
-        comp.alloc(P * (size_t)K);
-        HIPCHK(hipMemsetAsync(comp.p, 0, P * (size_t)K * sizeof(double2), st));
+        // the first slab starts the sums without reading them, and a series of one slab keeps no
+        // compensation terms between slabs
+        if (n_total > T) comp.alloc(P * (size_t)K);
     }
     float* next_slot() { return slab.p + (size_t)cnt * P; }
     void commit()
     {
         if (++cnt == T) flush();
     }
+    // contract the staged samples; the slab that completes the series writes the finished sums
     void flush()
     {
         if (cnt == 0) return;
+        const bool last = done + cnt >= n_total;
         launch_dft_bf16(acc, comp.p, slab.p, tab->part[0] + done, tab->part[1] + done, tab->part[2] + done, P, K, C_pad,
-                        Jt, cnt, st);
+                        Jt, cnt, done == 0 ? 1 : 0, last ? out : nullptr, out_scale, out_conj, st);
         done += cnt;
         cnt = 0;
+    }
+    // after the series' last sample: the remaining samples, or zeros for a series without samples
+    void finish()
+    {
+        flush();
+        if (n_total == 0 && K > 0) HIPCHK(hipMemsetAsync(out, 0, P * (size_t)K * sizeof(double2), st));
     }
 };
 
@@ -1220,7 +1237,8 @@ struct GreenJob : Job {
         std::vector<double> ts;
         for (int n = 1; n <= g.nt; ++n)
             if ((n + g.nt - 1) % g.s == 0) ts.push_back((n - 1) * g.dt);
-        if (Gf) dft.init(Gf, g.P, cfg, ts, -1.0, 1.0, st);
+        // G_fft = conj(sum v * (2pi)^-1/2 e^{-iwt} dt)  (:178-180, :196-198)
+        if (Gf) dft.init(Gf, g.P, cfg, ts, -1.0, 1.0, Gf, kInvSqrt2Pi * g.dt, 1, st);
     }
     bool sampled(int n) const { return Gf != nullptr && (n + g.nt - 1) % g.s == 0; }
     void plan(int n, int sub) override
@@ -1253,11 +1271,7 @@ struct GreenJob : Job {
     }
     void finish() override
     {
-        if (Gf) {
-            dft.flush();
-            // G_fft = conj(sum v * (2pi)^-1/2 e^{-iwt} dt)  (:178-180, :196-198)
-            launch_finalize_dft(Gf, Gf, dft.comp.p, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 1, st);
-        }
+        if (Gf) dft.finish();
     }
 };
 
@@ -1489,7 +1503,10 @@ struct AdjointJob : Job {
             }
             std::vector<double> ts;
             for (int n = g.s; n <= nsteps; n += g.s) ts.push_back((double)(n - g.nt) * g.dt);
-            dft.init(accp, g.P, cfg, ts, -1.0, 1.0, st);
+            // with a source kernel, the series ends as the plain sums (read by the kernel, then
+            // scaled into stf_fft); else directly as stf_fft = sum u * (2pi)^-1/2 e^{-iwt} dt
+            if (src_kernel && Kd) dft.init(accp, g.P, cfg, ts, -1.0, 1.0, accp, 1.0, 0, st);
+            else dft.init(accp, g.P, cfg, ts, -1.0, 1.0, accp, kInvSqrt2Pi * g.dt, 0, st);
         }
         if (born) {
             dmu_f.alloc(g.P);
@@ -1554,15 +1571,15 @@ struct AdjointJob : Job {
         if (deferred) S->kmu(deferred, Kd + (size_t)nb_cols * g.P, (double)g.f);  // the last step's product
         deferred = nullptr;
         if (!need_dft) return;
-        dft.flush();
+        dft.finish();
         if (src_kernel && Kd) {
             // K_s(:,:,b) = sum_n Re(M_b(n) u(n)) = (2pi)^-1/2 dw sum_k I(k,b) spec_k Re(conj(G_k D_k)),
-            // D_k = sum_n u(n) e^{-i w_k t_n}
+            // D_k = sum_n u(n) e^{-i w_k t_n} (the finished sums in dft.acc)
             upload(sp_d, std::vector<double>(spec.begin(), spec.begin() + g.K), st);
-            launch_source_kernel(Kd, (const double2*)G->data, dft.acc, dft.comp.p, I_d.p, sp_d.p, g.P, g.K, nb_cols,
+            launch_source_kernel(Kd, (const double2*)G->data, dft.acc, nullptr, I_d.p, sp_d.p, g.P, g.K, nb_cols,
                                  kInvSqrt2Pi * g.dw, st);
+            if (sfft) launch_finalize_dft(sfft, sfft, nullptr, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 0, st);
         }
-        if (sfft) launch_finalize_dft(sfft, sfft, dft.comp.p, g.P * (size_t)g.K, kInvSqrt2Pi * g.dt, 0, st);
     }
 };
 

@@ -226,9 +226,12 @@ void launch_scale_to_float(const double* in, float* out, size_t n, double scale,
 // added into the FP64 accumulators with Neumaier compensation (comp holds the compensation terms;
 // the sum is acc + comp); e{h,m,l}[c][Jt] are the parts of Re e[j][k] (c = 2k) and Im e[j][k]
 // (c = 2k + 1), C_pad (a multiple of 256) rows, offset to the slab's first sample (16-byte aligned).
+// first != 0: the sums start from zero (acc and comp are not read). out != nullptr: the last slab
+// of the series; instead of acc and comp, out[k][x] = scale * (conj ? conj(sum) : sum) is written
+// (out may be acc; comp is not written and may be null when first != 0).
 void launch_dft_bf16(double2* acc, double2* comp, const float* samp, const unsigned short* eh,
                      const unsigned short* em, const unsigned short* el, size_t P, int K, int C_pad, int Jt,
-                     int cnt, hipStream_t s);
+                     int cnt, int first, double2* out, double scale, int conj, hipStream_t s);
 // out[j][x] = sum_c et[j][c] * a[c][x] for j < cnt, the same products: with a = (Re A, Im A) rows
 // and et = (Re, -Im) of the phase factors (parts t{h,m,l}[J_pad][C_pad], J_pad >= cnt rounded up to
 // 64) this is Re(A[k][x] * coef[j][k]) summed over k.
@@ -240,10 +243,11 @@ void split_bf16(float v, unsigned short& h, unsigned short& m, unsigned short& l
 // (FP64, rounded to FP32), dist(x,k) = sum_b I[k,b] sd[b][x]; zero rows 2K..C_pad-1
 void launch_source_spectrum(float* a32, const double2* Z, int conj_in, const double* sd, int n_sd,
                             const double* I, const double* spec_sum, size_t P, int K, int C_pad, hipStream_t s);
-// out[k][x] = scale * (conj ? conj(acc + comp) : acc + comp)  (out may be acc)
+// out[k][x] = scale * (conj ? conj(acc + comp) : acc + comp)  (out may be acc; comp may be null: 0)
 void launch_finalize_dft(double2* out, const double2* acc, const double2* comp, size_t n, double scale, int conj,
                          hipStream_t s);
-// Ks[b][x] = scale * sum_k I[k,b] spec[k] Re(conj(G[k][x] * D[k][x])), D = Dacc + Dcomp
+// Ks[b][x] = scale * sum_k I[k,b] spec[k] Re(conj(G[k][x] * D[k][x])), D = Dacc + Dcomp (Dcomp may be
+// null: 0)
 void launch_source_kernel(double* Ks, const double2* G, const double2* Dacc, const double2* Dcomp, const double* I,
                           const double* spec, size_t P, int K, int nb_cols, double scale,
                           hipStream_t s);

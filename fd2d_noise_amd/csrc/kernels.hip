@@ -1436,7 +1436,7 @@ __global__ void k_finalize_dft(double2* out, const double2* acc, const double2* 
 {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    const double2 a = acc[c], e = comp[c];
+    const double2 a = acc[c], e = comp ? comp[c] : make_double2(0.0, 0.0);
     const double re = a.x + e.x, im = a.y + e.y;
     out[c] = make_double2(scale * re, conj ? -(scale * im) : scale * im);
 }
@@ -1456,7 +1456,8 @@ __global__ void k_source_kernel(double* Ks, const double2* G, const double2* Dac
     for (int b = 0; b < nb_cols; ++b) {
         double s = 0.0;
         for (int k = 0; k < K; ++k) {
-            const double2 g = G[(size_t)k * P + x], da = Dacc[(size_t)k * P + x], dc = Dcomp[(size_t)k * P + x];
+            const double2 g = G[(size_t)k * P + x], da = Dacc[(size_t)k * P + x];
+            const double2 dc = Dcomp ? Dcomp[(size_t)k * P + x] : make_double2(0.0, 0.0);
             const double2 d = make_double2(da.x + dc.x, da.y + dc.y);
             s += I[k + (size_t)K * b] * spec[k] * (g.x * d.x - g.y * d.y);
         }

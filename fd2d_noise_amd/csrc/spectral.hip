@@ -181,7 +181,8 @@ __global__ __launch_bounds__(256, 2) void k_dft_bf16(double2* __restrict__ acc, 
                                                   const unsigned short* __restrict__ eh,
                                                   const unsigned short* __restrict__ em,
                                                   const unsigned short* __restrict__ el, long P, int K, int Jt,
-                                                  int cnt)
+                                                  int cnt, int first, double2* __restrict__ out, double scale,
+                                                  int conj)
 {
     __shared__ u32x4 Bs[2 * kNB * 3 * 64];  // [buf][node block][part][half][col]
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -234,7 +235,9 @@ __global__ __launch_bounds__(256, 2) void k_dft_bf16(double2* __restrict__ acc, 
     // rows come in (real, imaginary) pairs: column c0 + 32cb + row, row even. The slab sums are
     // exact in FP64 and go into the FP64 accumulators with Neumaier compensation (sum + its
     // compensation term, both read for one node block, clamped addresses, before any is written
-    // back).
+    // back). The first slab of a series starts from zero sums without reading them; the last one
+    // writes the finished value scale * (sum + compensation), conjugated if asked, to `out`
+    // instead of the accumulators (the operations of a separate finishing pass, in its order).
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
@@ -248,28 +251,34 @@ __global__ __launch_bounds__(256, 2) void k_dft_bf16(double2* __restrict__ acc, 
                 const int k = (c0 + 32 * cb + d_row(2 * r, lane)) >> 1;
                 ok[r] = x < P && k < K;
                 o[r] = ok[r] ? (long)k * P + x : 0;
-                v[r] = acc[o[r]];
-                cv[r] = comp[o[r]];
+                v[r] = first ? make_double2(0.0, 0.0) : acc[o[r]];
+                cv[r] = first ? make_double2(0.0, 0.0) : comp[o[r]];
             }
 #pragma unroll
             for (int r = 0; r < 8; ++r)
                 if (ok[r]) {
                     neumaier(v[r].x, cv[r].x, (double)d[cb][nb][2 * r]);
                     neumaier(v[r].y, cv[r].y, (double)d[cb][nb][2 * r + 1]);
-                    acc[o[r]] = v[r];
-                    comp[o[r]] = cv[r];
+                    if (out) {
+                        const double re = v[r].x + cv[r].x, im = v[r].y + cv[r].y;
+                        out[o[r]] = make_double2(scale * re, conj ? -(scale * im) : scale * im);
+                    } else {
+                        acc[o[r]] = v[r];
+                        comp[o[r]] = cv[r];
+                    }
                 }
         }
 }
 
 void launch_dft_bf16(double2* acc, double2* comp, const float* samp, const unsigned short* eh, const unsigned short* em,
-                     const unsigned short* el, size_t P, int K, int C_pad, int Jt, int cnt, hipStream_t s)
+                     const unsigned short* el, size_t P, int K, int C_pad, int Jt, int cnt, int first, double2* out,
+                     double scale, int conj, hipStream_t s)
 {
     if (cnt <= 0 || K <= 0) return;
     const unsigned nxb = (unsigned)((P + kFX - 1) / kFX);
     const unsigned ncg = (unsigned)(C_pad / kFC);
     hipLaunchKernelGGL(k_dft_bf16, dim3(nxb, ncg), dim3(256), 0, s, acc, comp, samp, eh, em, el, (long)P, K, Jt,
-                       cnt);
+                       cnt, first, out, scale, conj);
 }
 
 // ---------------------------------------------------------------------------------------------
