@@ -1848,16 +1848,23 @@ struct Engine {
         }
         std::stable_sort(cls.begin(), cls.end(),
                          [](const auto& x, const auto& y) { return x.first < y.first; });
+        // a class of n solves steps in ceil(n / kPairGroupMax) launches of near-equal size (64 solves:
+        // two launches of 32, not 36 + 24 + 4, whose smallest takes a third of the largest's time)
+        static_assert(kPairGroupMax <= kGroupMax, "a group must fit either grouped launch");
         for (size_t q = 0; q < cls.size();) {
             size_t r = q;
-            while (r < cls.size() && cls[r].first == cls[q].first && r - q < (size_t)kGroupMax) ++r;
-            if (r - q == 1) {
-                singles.push_back(cls[q].second);
-            } else {
+            while (r < cls.size() && cls[r].first == cls[q].first) ++r;
+            const size_t n = r - q, parts = (n + kPairGroupMax - 1) / kPairGroupMax;
+            for (size_t p = 0; p < parts; ++p) {
+                const size_t b = q + n * p / parts, e = q + n * (p + 1) / parts;
+                if (e - b == 1) {
+                    singles.push_back(cls[b].second);
+                    continue;
+                }
                 Group gr;
                 gr.R = cls[q].first.first;
                 gr.kind = cls[q].first.second;
-                for (size_t k = q; k < r; ++k) gr.members.push_back(cls[k].second);
+                for (size_t k = b; k < e; ++k) gr.members.push_back(cls[k].second);
                 groups.push_back(std::move(gr));
             }
             q = r;

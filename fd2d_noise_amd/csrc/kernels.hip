@@ -1408,12 +1408,25 @@ void launch_stf_real(float* amp, const double2* stf, int64_t d0, int nq, int nst
 // a[2k][x], a[2k+1][x] = Re, Im of A[k][x] = spec_sum[k] * dist(x,k) * (conj_in ? conj(Z) : Z)[k][x],
 // dist(x,k) = sum_b I[k,b] sd[b][x], formed in FP64 and rounded to FP32 as the synthesis reads them;
 // rows 2K..C_pad-1 are zero.
+// One thread per node and frequency block (kSpecKB frequencies of the padded C_pad / 2): the rows of
+// a node are written by C_pad / (2 kSpecKB) workgroups side by side, so every store is coalesced
+// and a small grid still fills the chip.
+constexpr int kSpecKB = 8;
 __global__ void k_source_spectrum(float* a32, const double2* Z, int conj_in, const double* sd, int n_sd,
                                   const double* I, const double* spec_sum, size_t P, int K, int C_pad)
 {
     const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= P) return;
-    for (int k = 0; k < K; ++k) {
+    const int k0 = blockIdx.y * kSpecKB;
+#pragma unroll
+    for (int q = 0; q < kSpecKB; ++q) {
+        const int k = k0 + q;
+        if (2 * k >= C_pad) break;
+        if (k >= K) {  // zero rows 2K .. C_pad - 1
+            a32[(size_t)(2 * k) * P + x] = 0.f;
+            a32[(size_t)(2 * k + 1) * P + x] = 0.f;
+            continue;
+        }
         double d = 0.0;
         for (int b = 0; b < n_sd; ++b) d += I[k + (size_t)K * b] * sd[x + P * b];
         const double a = spec_sum[k] * d;
@@ -1421,14 +1434,14 @@ __global__ void k_source_spectrum(float* a32, const double2* Z, int conj_in, con
         a32[(size_t)(2 * k) * P + x] = (float)(a * z.x);
         a32[(size_t)(2 * k + 1) * P + x] = (float)(conj_in ? -(a * z.y) : a * z.y);
     }
-    for (int r = 2 * K; r < C_pad; ++r) a32[(size_t)r * P + x] = 0.f;
 }
 
 void launch_source_spectrum(float* a32, const double2* Z, int conj_in, const double* sd, int n_sd,
                             const double* I, const double* spec_sum, size_t P, int K, int C_pad, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_source_spectrum, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, a32, Z, conj_in, sd,
-                       n_sd, I, spec_sum, P, K, C_pad);
+    const unsigned nkb = (unsigned)((C_pad / 2 + kSpecKB - 1) / kSpecKB);
+    hipLaunchKernelGGL(k_source_spectrum, dim3((unsigned)((P + 255) / 256), nkb), dim3(256), 0, s, a32, Z, conj_in,
+                       sd, n_sd, I, spec_sum, P, K, C_pad);
 }
 
 __global__ void k_finalize_dft(double2* out, const double2* acc, const double2* comp, size_t n, double scale,

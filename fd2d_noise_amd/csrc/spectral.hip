@@ -295,7 +295,7 @@ constexpr int kSX = 128;   // nodes per workgroup
 constexpr int kSNB = kSX / 32;
 constexpr int kSJ = 256;   // time slices per workgroup
 constexpr int kSW = kSJ / 32;  // wavefronts per workgroup
-constexpr int kPFS = 4;    // chunks of spectrum rows loaded ahead
+constexpr int kPFS = 1;    // chunks of spectrum rows loaded ahead (two workgroups per CU hide the rest)
 
 // d0 += A B0 and d1 += A B1 over the six part products, the two chains interleaved
 __device__ __forceinline__ void mfma6x2(f32x16& d0, f32x16& d1, const Split8& a, const Split8& b0, const Split8& b1)
@@ -317,7 +317,7 @@ __device__ __forceinline__ void mfma6x2(f32x16& d0, f32x16& d1, const Split8& a,
         }
 }
 
-__global__ __launch_bounds__(512) void k_synth_bf16(float* __restrict__ out, const float* __restrict__ a,
+__global__ __launch_bounds__(512, 4) void k_synth_bf16(float* __restrict__ out, const float* __restrict__ a,
                                                      const unsigned short* __restrict__ th,
                                                      const unsigned short* __restrict__ tm,
                                                      const unsigned short* __restrict__ tl, long P, int C_pad,
