@@ -1365,8 +1365,8 @@ __global__ __launch_bounds__(256) void k_nonfinite_partial(const float4* __restr
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += (size_t)kCheckBlocks * 256) {
         const float4 q = x[i];
         // a comparison with NaN is false; Inf exceeds the largest finite float
-        bad |= !(fabsf(q.x) <= FLT_MAX) | !(fabsf(q.y) <= FLT_MAX) | !(fabsf(q.z) <= FLT_MAX) |
-               !(fabsf(q.w) <= FLT_MAX);
+        bad |= (int)!(fabsf(q.x) <= FLT_MAX) | (int)!(fabsf(q.y) <= FLT_MAX) | (int)!(fabsf(q.z) <= FLT_MAX) |
+               (int)!(fabsf(q.w) <= FLT_MAX);
     }
     bad = __syncthreads_or(bad);
     if (threadIdx.x == 0) partial[blockIdx.x] = bad ? 1 : 0;
