@@ -878,8 +878,9 @@ def main():
         a.window = 64
         st1 = stations_run(a, grid="config1")
         print(f"# stations (300x300): {st1['batched_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
-        # a steady stream: long enough that the longest request no longer sets the wall time
-        a.rates, a.requests = "200", 96
+        # a steady stream: long enough that the longest request (a 400^2 Hessian-vector product of
+        # ~44,000 sequential steps) no longer sets most of the wall time
+        a.rates, a.requests = "200", 192
         sv = serving_run(a)
         print(f"# serving: {sv['served_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
 
