@@ -28,11 +28,27 @@ def to_host(t: torch.Tensor) -> np.ndarray:
     return t.detach().cpu().numpy().T
 
 
+def release_cached_memory() -> None:
+    """Returns the device blocks the library keeps for later solves and those torch's caching
+    allocator holds unused (fd2d_release_cached_memory, torch.cuda.empty_cache); waits for the device."""
+    _check(_lib.load().fd2d_release_cached_memory())
+    torch.cuda.empty_cache()
+
+
 def empty_like_dims(dims: Sequence[int], dtype, device="cuda") -> torch.Tensor:
     dims = [int(d) for d in dims]
     while len(dims) > 1 and dims[-1] == 1:
         dims.pop()
-    return torch.zeros(tuple(reversed(dims)), dtype=dtype, device=device)
+    shape = tuple(reversed(dims))
+    try:
+        return torch.zeros(shape, dtype=dtype, device=device)
+    except torch.OutOfMemoryError:
+        if torch.device(device).type != "cuda":
+            raise
+        # the free memory may sit in the library's cached blocks or in torch's, split into blocks
+        # of other sizes: return both to the device and try once more
+        release_cached_memory()
+        return torch.zeros(shape, dtype=dtype, device=device)
 
 
 def _mdims(t: torch.Tensor):

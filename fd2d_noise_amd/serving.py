@@ -234,6 +234,7 @@ class _Live:
     t_admit: float = 0.0
     tickets: List[int] = dataclasses.field(default_factory=list)
     specs: List[Any] = dataclasses.field(default_factory=list)
+    pending: List[Any] = dataclasses.field(default_factory=list)  # solves of the stage not yet submitted
     t_done: float = 0.0
     result: Any = None
     error: Optional[BaseException] = None  # set instead of result when the request failed or was cancelled
@@ -366,18 +367,42 @@ class Server:
             live.gen = None
             self._finish(live, fin.value)
             return
-        except abi.Fd2dError as e:  # the request's own caller work rejected its data
+        except (abi.Fd2dError, torch.OutOfMemoryError) as e:
+            # the request's own caller work rejected its data, or its arrays for the next stage do
+            # not fit the device memory free now: the request ends with the error, the others go on
             self._finish(live, error=e)
             return
-        live.specs = specs
-        live.tickets = [self.engine.submit(s) for s in specs]
+        live.specs, live.tickets, live.pending = specs, [], list(specs)
         self.gp_updates += sum(s.config.nx * s.config.nz * s.time_steps() for s in specs)
+        self._submit_pending(live)
+
+    def _submit_pending(self, live: _Live) -> bool:
+        """Submits the solves of the request's stage that are not yet running. A solve whose library
+        buffers do not fit the device memory free now (FD2D_ENOMEM: the budget counts what requests
+        hold, but torch's caching allocator may keep the freed arrays of finished requests, split
+        into blocks of other sizes) is retried once after torch returns its cached memory; if it still
+        does not fit, it waits for the next step, when running requests may have freed theirs. True
+        when every solve of the stage is submitted."""
+        emptied = False
+        while live.pending:
+            try:
+                live.tickets.append(self.engine.submit(live.pending[0]))
+            except abi.Fd2dError as e:
+                if e.code != abi.FD2D_ENOMEM:
+                    raise
+                if emptied:
+                    return False
+                torch.cuda.empty_cache()  # the library has returned its own cached blocks already
+                emptied = True
+                continue
+            live.pending.pop(0)
+        return True
 
     def _abandon(self, live: _Live, error: BaseException) -> None:
         """Stops the request's running solves and ends it with `error`; other requests go on."""
         for t in live.tickets:
             self.engine.cancel(t)
-        live.tickets = []
+        live.tickets, live.pending = [], []
         self._finish(live, error=error)
 
     def cancel(self, live: _Live) -> bool:
@@ -398,8 +423,15 @@ class Server:
         self._admit()
         if not self.live:
             return
+        waiting = [live for live in self.live if live.pending and not self._submit_pending(live)]
+        if waiting and len(waiting) == len(self.live) and self.engine.active == 0:
+            # nothing runs that could free memory: the first waiting request cannot run now
+            self._abandon(waiting[0], abi.Fd2dError(abi.FD2D_ENOMEM, "the device memory its solves need is not free"))
+            return
         self.engine.advance(self.rounds)
         for live in list(self.live):
+            if live.pending:
+                continue  # its stage waits for device memory
             states = [self.engine.state(t) for t in live.tickets]
             if abi.FD2D_SOLVE_FAILED in states:
                 q = states.index(abi.FD2D_SOLVE_FAILED)

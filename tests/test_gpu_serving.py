@@ -182,3 +182,70 @@ def test_serving_requests_match_oracle(torch_mod, orc):
             print(f"request {req.tag} {req.kind} on {req.model}: {k} rel L2 {e:.2e}", flush=True)
             assert e < tol[k], f"request {req.tag} {k}: {e:.3e}"
     srv.close()
+
+
+class _Scarce:
+    """The server's engine, except that a submission for which refuse(call number, solve) holds
+    fails with FD2D_ENOMEM, as when device memory the budget does not count (torch's cached
+    blocks, another process) is not free at that moment."""
+
+    def __init__(self, engine, refuse):
+        self._engine, self._refuse, self.calls, self.refused = engine, refuse, 0, 0
+
+    def submit(self, spec):
+        from fd2d_noise_amd import abi
+        self.calls += 1
+        if self._refuse(self.calls, spec):
+            self.refused += 1
+            raise abi.Fd2dError(abi.FD2D_ENOMEM, "device allocation refused")
+        return self._engine.submit(spec)
+
+    def __getattr__(self, name):
+        return getattr(self._engine, name)
+
+
+def test_serving_waits_for_device_memory(torch_mod):
+    """Solves refused for want of device memory, also after torch returns its cached blocks, wait
+    and are submitted on a later step; every request is still served, identical to alone."""
+    torch = torch_mod
+    srv, reqs = _server(torch)
+    eng = srv.engine
+    # while other solves run, a first try and its retry fail in turn (memory they hold is freed when
+    # they end, so with nothing running every submission succeeds)
+    scarce = _Scarce(eng, lambda n, spec: eng.active > 0 and n % 5 in (1, 2))
+    srv.engine = scarce
+    for r in reqs:
+        srv.submit(r)
+    waited = 0
+    while srv.busy():
+        srv.step()
+        waited += sum(1 for lv in srv.live if lv.pending)
+    torch.cuda.synchronize()
+    print(f"{scarce.refused} of {scarce.calls} submissions refused, {waited} request-steps waiting", flush=True)
+    assert scarce.refused > 0 and waited > 0
+    assert len(srv.done) == len(reqs) and all(d.error is None for d in srv.done) and srv.in_use == 0
+    srv.engine = scarce._engine
+    _check(torch, srv, reqs, srv.done)
+    srv.close()
+
+
+def test_serving_abandons_request_that_never_fits(torch_mod):
+    """A request whose solves are refused while nothing else runs (so nothing can free memory) ends
+    with FD2D_ENOMEM; the server goes on serving the next one."""
+    torch = torch_mod
+    from fd2d_noise_amd import abi
+    srv, reqs = _server(torch)
+    scarce = _Scarce(srv.engine, lambda n, spec: spec.config.nx == 72)  # every solve on model "b"
+    srv.engine = scarce
+    kb, ca = reqs[1], reqs[0]
+    assert (kb.model, ca.model) == ("b", "a")
+    lb = srv.submit(kb)
+    srv.run_until_idle()
+    assert isinstance(lb.error, abi.Fd2dError) and lb.error.code == abi.FD2D_ENOMEM and lb.result is None
+    assert srv.in_use == 0 and not srv.live
+    la = srv.submit(ca)
+    srv.run_until_idle()
+    assert la.error is None
+    srv.engine = scarce._engine
+    _check(torch, srv, reqs, [la])
+    srv.close()
