@@ -1816,7 +1816,10 @@ struct Engine {
     {
         groups.clear();
         singles.clear();
-        std::vector<std::pair<std::pair<int, int>, Entry*>> cls;
+        // class of a solve: its lane, FD order and field set. A grouped launch runs on its members'
+        // lane, which must be the lane of every member: their earlier steps, and the events that
+        // free their buffers, are queued there.
+        std::vector<std::pair<std::array<int, 3>, Entry*>> cls;
         std::vector<Entry*> big;  // the oldest large solves of one class
         double big_bytes = 0.0;
         for (auto& e : act) {
@@ -1828,14 +1831,15 @@ struct Engine {
             // round of workgroups that one solve leaves part-empty.
             if (j.S->n_tiles >= kSoloTiles) {
                 const double b = step_bytes(j);
-                if (big.empty() || (big_bytes + b <= kLastLevelCacheBytes && big[0]->job->g.R == j.g.R &&
+                if (big.empty() || (big_bytes + b <= kLastLevelCacheBytes && big[0]->lane == e->lane &&
+                                    big[0]->job->g.R == j.g.R &&
                                     field_kind(*big[0]->job) == field_kind(j) && !j.born && !big[0]->job->born)) {
                     big.push_back(e.get());
                     big_bytes += b;
                 }
                 continue;
             }
-            cls.push_back({{j.g.R, field_kind(j)}, e.get()});
+            cls.push_back({{e->lane, j.g.R, field_kind(j)}, e.get()});
         }
         if (big.size() == 1) {
             singles.push_back(big[0]);
@@ -1862,8 +1866,8 @@ struct Engine {
                     continue;
                 }
                 Group gr;
-                gr.R = cls[q].first.first;
-                gr.kind = cls[q].first.second;
+                gr.R = cls[q].first[1];
+                gr.kind = cls[q].first[2];
                 for (size_t k = b; k < e; ++k) gr.members.push_back(cls[k].second);
                 groups.push_back(std::move(gr));
             }
@@ -1910,6 +1914,8 @@ struct Engine {
     }
     void step_group(const Group& gr)
     {
+        for (const Entry* e : gr.members)
+            if (e->lane != gr.members[0]->lane) raise(FD2D_EINVAL, "internal: a grouped launch spans lanes");
         StreamScope scope(gr.members[0]->job->st);
         std::vector<Entry*> one, two;
         for (Entry* e : gr.members) (pairs_now(*e->job) ? two : one).push_back(e);
