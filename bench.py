@@ -470,6 +470,13 @@ def serving_run(args):
                      for n in sorted(models) if models[n]["config"].nx <= 543
                      for k in ("correlation", "kernel", "hvp")])
     srv.serve(warm)
+    if not full:
+        # and a stream like the timed ones (another seed): torch's caching allocator and the library's
+        # block cache then hold the blocks that a stream of this concurrency needs, as in a server
+        # that has been running. A cold one pays a hipMalloc for each, which took from 0.02 to 2.2 ms
+        # per call from one process to the next on the same box.
+        top = max(rates)
+        srv.serve(requests(serving_stream(models, top, args.requests, seed=int(top * 1000) + 8)))
     srv.done.clear()
     srv.rejected.clear()
     runs = []
@@ -481,6 +488,7 @@ def serving_run(args):
         srv.rejected.clear()
         srv.gcache.clear()
         srv.gcache_bytes = 0
+        srv.cache_hits = 0
         gp0 = srv.gp_updates
         t0 = time.perf_counter()
         srv.serve(stream)
