@@ -338,18 +338,24 @@ typedef struct fd2d_solve {
 } fd2d_solve;
 
 /*
- * An engine advances many solves together on one stream: each round advances every active solve by
- * one time step, with one launch for all solves of the same FD order and field set (Green runs
- * carry no displacement field, correlation and adjoint runs do) and one launch per Born solve.
- * Each solve's outputs equal those of its own entry point bit for bit. Solves may be submitted
- * between rounds (continuous batching); a solve's outputs are complete once
- * fd2d_engine_done() reports it. An engine is used by one host thread at a time.
+ * An engine advances many solves together: each round advances every active solve by one time
+ * step, or by two where its two-step launch applies, with one launch for up to 36 solves of the same
+ * FD order and field set (Green runs carry no displacement field, correlation and adjoint runs do),
+ * on streams of the engine's own that wait for the caller's stream at each submission. Each solve's
+ * outputs equal those of its own entry point bit for bit. Solves may be submitted between rounds
+ * (continuous batching); a solve's outputs are complete once fd2d_engine_done() reports it. An
+ * engine is used by one host thread at a time.
  */
 typedef struct fd2d_engine fd2d_engine;
 
 /* The stream must outlive the engine. */
 int fd2d_engine_create(void* stream, fd2d_engine** engine);
-/* Validates and sets up a solve (its device memory is allocated here); *ticket names it. */
+/*
+ * Validates and sets up a solve (its device memory is allocated here); *ticket names it. Returns
+ * FD2D_ENOMEM, with the engine unchanged, when the solve's buffers do not fit the device memory free
+ * now (after the library's own cached blocks have been returned) or the engine holds 1024 solves: it
+ * may be submitted again later, e.g. once running solves have completed.
+ */
 int fd2d_engine_submit(fd2d_engine* engine, const fd2d_solve* solve, int64_t* ticket);
 /* Runs up to `rounds` rounds; *n_active (may be NULL) = solves not yet complete afterwards. When no
  * solve is left running it waits for the finished ones, so advancing until *n_active == 0 leaves
