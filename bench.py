@@ -872,9 +872,16 @@ def main():
     n_rec, gp_step = p.nr, p.gp_updates()
     # free the pipeline's device memory for the runs below
     del p
-    torch.cuda.synchronize()
-    torch.cuda.empty_cache()
-    _lib.load().fd2d_release_cached_memory()
+
+    def fresh_caches():
+        # each side run starts with empty allocator caches, which its own warm-up fills: a library
+        # block cache still full of another workload's blocks frees every block returned to it with
+        # hipFree, which waits for the whole device
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        _lib.load().fd2d_release_cached_memory()
+
+    fresh_caches()
     side = {}
     if not args.no_side:
         import copy
@@ -884,11 +891,13 @@ def main():
         print(f"# stations: {st['batched_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
         # the same flow on the 300x300 grid of the CPU-sized case, every station in flight
         a.window = 64
+        fresh_caches()
         st1 = stations_run(a, grid="config1")
         print(f"# stations (300x300): {st1['batched_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
         # a steady stream: long enough that the longest request (a 400^2 Hessian-vector product of
         # ~44,000 sequential steps) no longer sets most of the wall time
         a.rates, a.requests = "200", 192
+        fresh_caches()
         sv = serving_run(a)
         print(f"# serving: {sv['served_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
 
