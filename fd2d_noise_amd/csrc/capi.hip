@@ -66,7 +66,8 @@ struct StreamScope {
 // released blocks are kept and handed out again for requests of at least 2/3 of their size. A
 // released block carries an event recorded on the stream that used it; the next user waits on the
 // event, so a block still read by queued work is never handed out early. At most `limit` bytes are
-// kept (a quarter of the device memory); fd2d_release_cached_memory() frees them all. A stream
+// kept (a quarter of the device memory), the blocks released longest ago freed first to make room;
+// fd2d_release_cached_memory() frees them all. A stream
 // that is about to be destroyed is detached first: the events recorded on it are waited for and
 // destroyed, so that no cached event refers to a stream that no longer exists.
 struct BlockCache {
@@ -74,10 +75,12 @@ struct BlockCache {
         void* p;
         hipEvent_t ev;
         hipStream_t st;  // the stream ev was recorded on
+        uint64_t seq;    // order of release: the cache evicts the least recently released first
     };
     std::mutex mu;
     std::multimap<size_t, Block> blocks;
     size_t held = 0, limit = 0;
+    uint64_t next_seq = 0;
     bool limit_set = false;
     void* take(size_t bytes, size_t* got)
     {
@@ -105,10 +108,26 @@ struct BlockCache {
             if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) limit = total_b / 4;
             limit_set = true;
         }
-        if (held + bytes > limit) {
+        if (bytes > limit) {
             lk.unlock();
             (void)hipFree(p);
             return;
+        }
+        // Full: make room by freeing the blocks released longest ago, so that the cache follows the
+        // sizes in use now. (Keeping the old blocks and freeing each new one instead held a cache
+        // full of one workload's sizes while the next workload paid a hipMalloc and a hipFree, which
+        // waits for the device, for every block.)
+        while (held + bytes > limit) {
+            auto old = blocks.begin();
+            for (auto it = blocks.begin(); it != blocks.end(); ++it)
+                if (it->second.seq < old->second.seq) old = it;
+            if (old->second.ev) {
+                if (hipEventSynchronize(old->second.ev) != hipSuccess) (void)hipGetLastError();
+                (void)hipEventDestroy(old->second.ev);
+            }
+            (void)hipFree(old->second.p);
+            held -= old->first;
+            blocks.erase(old);
         }
         hipEvent_t ev = nullptr;
         if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess ||
@@ -120,7 +139,7 @@ struct BlockCache {
             (void)hipFree(p);
             return;
         }
-        blocks.emplace(bytes, Block{p, ev, st});
+        blocks.emplace(bytes, Block{p, ev, st, next_seq++});
         held += bytes;
     }
     void detach(hipStream_t s)
