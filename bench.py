@@ -592,6 +592,21 @@ def serving_bench(args):
     print(json.dumps(serving_run(args)), flush=True)
 
 
+def serving_subprocess(rates: str, requests: int) -> dict:
+    """The serving workload (`--workload serving`) in a child process; its progress lines are passed
+    on to stderr and its JSON line is returned."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "serving", "--rates", rates,
+           "--requests", str(requests)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=900)
+    for line in r.stderr.splitlines():
+        if line.startswith("# "):
+            print(line, file=sys.stderr, flush=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"the serving run failed ({r.returncode}): {r.stderr[-2000:]}")
+    return json.loads([line for line in r.stdout.splitlines() if line.startswith("{")][-1])
+
+
 def inversion_bench(args):
     """Structure inversion of the 20-station ring: observed correlations from the true model
     (calculate_data, stations batched), one objective + gradient evaluation of the initial model on
@@ -895,10 +910,12 @@ def main():
         st1 = stations_run(a, grid="config1")
         print(f"# stations (300x300): {st1['batched_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
         # a steady stream: long enough that the longest request (a 400^2 Hessian-vector product of
-        # ~44,000 sequential steps) no longer sets most of the wall time
-        a.rates, a.requests = "200", 192
+        # ~44,000 sequential steps) no longer sets most of the wall time. It runs in a process of its
+        # own, as a server does: in this one, after the pipeline's 200 GB of device allocations have
+        # been freed, every small kernel runs slower (the same stream measured 4.5-4.7x the plain path
+        # here against 5.1-5.2x in a fresh process; allocating and freeing 200 GB alone does it).
         fresh_caches()
-        sv = serving_run(a)
+        sv = serving_subprocess(rates="200", requests=192)
         print(f"# serving: {sv['served_vs_plain']:.2f}x the plain path", file=sys.stderr, flush=True)
 
         def brief(r):
