@@ -67,12 +67,15 @@ struct StreamScope {
 // released block carries an event recorded on the stream that used it; the next user waits on the
 // event, so a block still read by queued work is never handed out early. At most `limit` bytes are
 // kept (a quarter of the device memory), the blocks released longest ago freed first to make room;
-// fd2d_release_cached_memory() frees them all.
+// fd2d_release_cached_memory() frees them all. A stream
+// that is about to be destroyed is detached first: the events recorded on it are waited for and
+// destroyed, so that no cached event refers to a stream that no longer exists.
 struct BlockCache {
     struct Block {
         void* p;
-        hipEvent_t ev;  // recorded behind the block's last use
-        uint64_t seq;   // order of release: the cache evicts the least recently released first
+        hipEvent_t ev;
+        hipStream_t st;  // the stream ev was recorded on
+        uint64_t seq;    // order of release: the cache evicts the least recently released first
     };
     std::mutex mu;
     std::multimap<size_t, Block> blocks;
@@ -81,7 +84,7 @@ struct BlockCache {
     bool limit_set = false;
     void* take(size_t bytes, size_t* got)
     {
-        Block blk{nullptr, nullptr, 0};
+        Block blk{nullptr, nullptr, nullptr};
         {
             std::lock_guard<std::mutex> lk(mu);
             auto it = blocks.lower_bound(bytes);
@@ -136,8 +139,21 @@ struct BlockCache {
             (void)hipFree(p);
             return;
         }
-        blocks.emplace(bytes, Block{p, ev, next_seq++});
+        blocks.emplace(bytes, Block{p, ev, st, next_seq++});
         held += bytes;
+    }
+    void detach(hipStream_t s)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& kv : blocks) {
+            Block& b = kv.second;
+            if (b.ev && b.st == s) {
+                if (hipEventSynchronize(b.ev) != hipSuccess) (void)hipGetLastError();
+                (void)hipEventDestroy(b.ev);
+                b.ev = nullptr;
+                b.st = nullptr;
+            }
+        }
     }
     void clear()
     {
@@ -712,7 +728,8 @@ struct DeviceTable {
     unsigned short* part[3] = {nullptr, nullptr, nullptr};
     std::vector<unsigned short> host[3];  // staging of the asynchronous upload
     size_t n = 0;                         // elements per part
-    hipEvent_t ready = nullptr;  // the upload is complete; null once it is known to be
+    hipEvent_t ready = nullptr;     // the upload is complete; null once it is known to be
+    hipStream_t ready_st = nullptr;  // the stream of the upload
     ~DeviceTable()
     {
         for (unsigned short* q : part)
@@ -771,6 +788,7 @@ std::shared_ptr<const DeviceTable> cached_table(const std::string& key, hipStrea
     }
     HIPCHK(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming));
     HIPCHK(hipEventRecord(t->ready, st));
+    t->ready_st = st;
     c.lru.emplace_back(key, t);
     c.bytes += 3 * t->n * sizeof(unsigned short);
     for (size_t q = 0; c.bytes > kTableCacheBytes && q + 1 < c.lru.size();) {
@@ -782,6 +800,24 @@ std::shared_ptr<const DeviceTable> cached_table(const std::string& key, hipStrea
         }
     }
     return t;
+}
+
+// Before the stream s is destroyed: the cached device blocks and tables whose events were
+// recorded on s lose those events (the work they mark is waited for first).
+void detach_stream(hipStream_t s)
+{
+    for (BlockCache& c : g_cache) c.detach(s);
+    std::lock_guard<std::mutex> lk(g_table_mu);
+    for (int d = 0; d < 64; ++d)
+        for (auto& kv : g_tables[d].lru) {
+            DeviceTable& t = *kv.second;
+            if (t.ready && t.ready_st == s) {
+                if (hipEventSynchronize(t.ready) != hipSuccess) (void)hipGetLastError();
+                (void)hipEventDestroy(t.ready);
+                t.ready = nullptr;
+                t.ready_st = nullptr;
+            }
+        }
 }
 
 // Key of the phase table e^{sign i w_k t_j} * scale over the times ts, in layout `kind` with slab
@@ -1703,52 +1739,8 @@ int lane_of(int order, int solver)
     return (solver == FD2D_GREEN ? 0 : 1) + 2 * ((R >> 1) & 1);  // orders 4 and 8 (and 2 and 6) apart
 }
 
-// Lane streams of engines, per device: an engine leases a set and gives it back when it is destroyed,
-// and the streams live until the process ends. HIP places streams on its few hardware queues as
-// they are created, and streams created after others were destroyed can put two busy lanes on one
-// queue, where they run one after the other (measured: a 192-request served stream took 7.8 s
-// instead of 5.3 s when one engine had been created and destroyed before the server's). Reused
-// streams keep the placement of the first set; engines alive at the same time lease distinct sets.
-struct LaneSet {
-    int dev;
-    std::array<hipStream_t, kLanes> s;
-};
-std::mutex g_lane_mu;
-std::vector<LaneSet>* g_free_lanes = new std::vector<LaneSet>;  // never destroyed, as the streams
-
-LaneSet lease_lanes()
-{
-    LaneSet set{0, {}};
-    HIPCHK(hipGetDevice(&set.dev));
-    {
-        std::lock_guard<std::mutex> lk(g_lane_mu);
-        for (auto it = g_free_lanes->begin(); it != g_free_lanes->end(); ++it)
-            if (it->dev == set.dev) {
-                set = *it;
-                g_free_lanes->erase(it);
-                return set;
-            }
-    }
-    for (int q = 0; q < kLanes; ++q) {
-        const hipError_t e = hipStreamCreateWithFlags(&set.s[q], hipStreamNonBlocking);
-        if (e != hipSuccess) {
-            for (int r = 0; r < q; ++r) (void)hipStreamDestroy(set.s[r]);
-            HIPCHK(e);
-        }
-    }
-    return set;
-}
-
-// The streams must have no work queued by the engine that gives them back.
-void give_back_lanes(const LaneSet& set)
-{
-    std::lock_guard<std::mutex> lk(g_lane_mu);
-    g_free_lanes->push_back(set);
-}
-
 struct Engine {
     hipStream_t st = nullptr;          // the caller's stream: inputs are ready on it at submission
-    LaneSet lane_set{};
     hipStream_t lanes[kLanes] = {};
     hipEvent_t submit_ev = nullptr;    // recorded on st at each submission, waited for by the lane
     std::vector<std::unique_ptr<Entry>> act;       // running
@@ -1782,19 +1774,12 @@ struct Engine {
     explicit Engine(hipStream_t s) : st(s)
     {
         StreamScope scope(st);
-        lane_set = lease_lanes();
-        for (int q = 0; q < kLanes; ++q) lanes[q] = lane_set.s[q];
-        try {
-            HIPCHK(hipEventCreateWithFlags(&submit_ev, hipEventDisableTiming));
-            slots_d.alloc((size_t)2 * kEngineSlots);
-            for (int q = 0; q < kLanes; ++q) chk_partial[q].alloc(kCheckBlocks);
-            chk_flag_d.alloc(kEngineSlots);
-            HIPCHK(hipHostMalloc((void**)&chk_flag_h, sizeof(int) * kEngineSlots, hipHostMallocDefault));
-        } catch (...) {
-            if (submit_ev) (void)hipEventDestroy(submit_ev);
-            give_back_lanes(lane_set);  // nothing was queued on them
-            throw;
-        }
+        for (int q = 0; q < kLanes; ++q) HIPCHK(hipStreamCreateWithFlags(&lanes[q], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&submit_ev, hipEventDisableTiming));
+        slots_d.alloc((size_t)2 * kEngineSlots);
+        for (int q = 0; q < kLanes; ++q) chk_partial[q].alloc(kCheckBlocks);
+        chk_flag_d.alloc(kEngineSlots);
+        HIPCHK(hipHostMalloc((void**)&chk_flag_h, sizeof(int) * kEngineSlots, hipHostMallocDefault));
         std::fill(chk_flag_h, chk_flag_h + kEngineSlots, 0);
         for (int q = kEngineSlots - 1; q >= 0; --q) free_slots.push_back(q);
     }
@@ -2169,7 +2154,11 @@ struct Engine {
         chk_flag_d.release();
         if (chk_flag_h) (void)hipHostFree(chk_flag_h);
         if (submit_ev) (void)hipEventDestroy(submit_ev);
-        give_back_lanes(lane_set);  // synchronised above
+        for (int q = 0; q < kLanes; ++q)
+            if (lanes[q]) {
+                detach_stream(lanes[q]);
+                (void)hipStreamDestroy(lanes[q]);
+            }
     }
 };
 
