@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=None,
                     help="time steps (default: the workload's, 8000 for the pipeline, 4000 for stations)")
-    ap.add_argument("--workload", choices=["pipeline", "stations", "serving", "inversion"], default="pipeline",
+    ap.add_argument("--workload", choices=["pipeline", "stations", "serving", "inversion", "greens"], default="pipeline",
                     help="pipeline: the single large solve (the benchmark line); stations: the batched "
                          "reference-station workload (1200x1200, order 4, 64 stations x 4 basis functions) "
                          "through the batch engine against the plain single-solve path; serving: a Poisson "
@@ -72,7 +72,9 @@ def parse():
                     help="grid of the stations workload: 1200x1200 (config3) or 300x300 (config1)")
     ap.add_argument("--serving-range", choices=["small", "full"], default="small",
                     help="grid sizes of the serving workload: 200^2-400^2, or the full 200^2-4000^2 range")
-    ap.add_argument("--stations", type=int, default=64, help="stations of the stations workload")
+    ap.add_argument("--stations", type=int, default=64, help="stations of the stations workload (solves of greens)")
+    ap.add_argument("--grid", type=int, default=200, help="nx = nz of the greens workload")
+    ap.add_argument("--order", type=int, default=4, choices=[2, 4, 6, 8], help="FD order of the greens workload")
     ap.add_argument("--window", type=int, default=8, help="stations in flight in the batch engine")
     ap.add_argument("--plain", type=int, default=8, help="stations timed on the plain single-solve path")
     ap.add_argument("--rounds", type=int, default=64, help="engine rounds between completion checks")
@@ -417,6 +419,57 @@ def stations_run(args, grid: str = "config3"):
 
 def stations_bench(args):
     print(json.dumps(stations_run(args, grid=args.stations_grid)), flush=True)
+
+
+def greens_run(args):
+    """--stations Green runs on one small grid (homogeneous model, dx = 2 km, dt = 0.2 s, sources drawn
+    in the central 40% of the domain, no receivers): one solve at a time
+    (fd2d_run_forward1_green_device, --plain of them after a warm-up solve) against all of them in
+    one batch (fd2d_run_batch_device, after a warm-up batch); G_fft of the solves both paths ran
+    compared bit for bit."""
+    import torch
+    from fd2d_noise_amd import Config, batch as B, device as D
+    n, nt, S = args.grid, args.nt or 10 * args.grid, args.stations
+    c = Config(Lx=2e3 * (n - 1), Lz=2e3 * (n - 1), nx=n, nz=n, dt=0.2, nt=nt, order=args.order, fw_nth=1)
+    mu = D.to_device(np.full((n, n), 4.8e10))
+    rho = D.to_device(np.full((n, n), 3000.0))
+    rng = np.random.default_rng(0)
+    srcs = [D.to_device(rng.uniform(0.3, 0.7, (1, 2)) * c.Lx) for _ in range(S)]
+    Gb = [D.empty_like_dims((n, n, c.n_sample), torch.complex128) for _ in range(S)]
+    n_plain = max(1, min(args.plain, S))
+    Gp = [D.empty_like_dims((n, n, c.n_sample), torch.complex128) for _ in range(n_plain)]
+    print(f"# workload greens {n}x{n} order {c.order} nt {nt} solves {S}", file=sys.stderr, flush=True)
+
+    def plain(i):
+        D.run_forward1_green(mu, rho, srcs[i], None, 0, None, None, config=c, G_fft=Gp[i])
+
+    plain(0)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(n_plain):
+        plain(i)
+    torch.cuda.synchronize()
+    t_plain = (time.perf_counter() - t0) / n_plain
+    specs = [B.green(mu, rho, srcs[i], None, 0, None, None, config=c, G_fft=Gb[i]) for i in range(S)]
+    B.run_batch(specs)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B.run_batch(specs)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    same = all(torch.equal(Gp[i], Gb[i]) for i in range(n_plain))
+    gp = S * n * n * nt
+    print(f"# plain {t_plain * 1e3:.1f} ms per solve, batched {el * 1e3:.1f} ms for {S}: "
+          f"{t_plain * S / el:.2f}x, identical {same}", file=sys.stderr, flush=True)
+    return {"metric": "Gpt-updates/s, correlations+kernels/s, p99 latency; FP32 vs plain, L2<=1e-5",
+            "value": gp / el, "unit": "GP-updates/s", "n_gpus": 1, "steps": S, "warmup": 1,
+            "ms_per_step": el / S * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 fields, f64 DFT accumulators", "data": "synthetic",
+            "config": {"workload": f"greens: {S} Green runs, {n}x{n} order-{c.order} nt={nt}, "
+                                   f"{c.n_sample} frequencies, all in one batch", "nx": n, "nz": n,
+                       "order": c.order, "nt": nt, "solves": S},
+            "plain_gp_updates_per_s": n * n * nt / t_plain, "batched_vs_plain": t_plain * S / el,
+            "identical_to_plain": bool(same), "solves_compared": n_plain}
 
 
 def serving_run(args):
@@ -778,6 +831,9 @@ def main():
         return
     if args.workload == "inversion":
         inversion_bench(args)
+        return
+    if args.workload == "greens":
+        print(json.dumps(greens_run(args)), flush=True)
         return
     w = config2_workload(nt=args.nt or 8000)
     c = w["config"]
