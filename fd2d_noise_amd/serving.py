@@ -381,15 +381,17 @@ class Server:
         buffers do not fit the device memory free now (FD2D_ENOMEM: the budget counts what requests
         hold, but torch's caching allocator may keep the freed arrays of finished requests, split
         into blocks of other sizes) is retried once after torch returns its cached memory; if it still
-        does not fit, it waits for the next step, when running requests may have freed theirs. True
-        when every solve of the stage is submitted."""
+        does not fit, it waits for the next step, when running requests may have freed theirs. A solve
+        the engine refuses for another reason (its arrays do not match its configuration) ends the
+        request with that error. True when nothing of the stage is left to submit."""
         emptied = False
         while live.pending:
             try:
                 live.tickets.append(self.engine.submit(live.pending[0]))
             except abi.Fd2dError as e:
                 if e.code != abi.FD2D_ENOMEM:
-                    raise
+                    self._abandon(live, e)
+                    return True
                 if emptied:
                     return False
                 torch.cuda.empty_cache()  # the library has returned its own cached blocks already

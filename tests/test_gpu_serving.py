@@ -249,3 +249,29 @@ def test_serving_abandons_request_that_never_fits(torch_mod):
     srv.engine = scarce._engine
     _check(torch, srv, reqs, [la])
     srv.close()
+
+
+def test_serving_ends_request_whose_solve_is_refused(torch_mod):
+    """A solve the engine refuses for a reason other than memory (here FD2D_EINVAL for every solve
+    on model "b") ends its request with that error; the other requests are served."""
+    torch = torch_mod
+    from fd2d_noise_amd import abi
+    srv, reqs = _server(torch)
+    eng = srv.engine
+
+    class _Refusing(_Scarce):
+        def submit(self, spec):
+            if spec.config.nx == 72:
+                raise abi.Fd2dError(abi.FD2D_EINVAL, "solve refused")
+            return self._engine.submit(spec)
+
+    srv.engine = _Refusing(eng, None)
+    kb, ca, cb = reqs[1], reqs[0], reqs[3]
+    assert (kb.model, ca.model, cb.model) == ("b", "a", "b")
+    lives = [srv.submit(r) for r in (kb, ca, cb)]
+    srv.run_until_idle()
+    assert [lv.error.code if lv.error else None for lv in lives] == [abi.FD2D_EINVAL, None, abi.FD2D_EINVAL]
+    assert srv.in_use == 0 and not srv.live
+    srv.engine = eng
+    _check(torch, srv, reqs, [lives[1]])
+    srv.close()
