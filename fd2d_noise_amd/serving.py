@@ -367,9 +367,10 @@ class Server:
             live.gen = None
             self._finish(live, fin.value)
             return
-        except (abi.Fd2dError, torch.OutOfMemoryError) as e:
-            # the request's own caller work rejected its data, or its arrays for the next stage do
-            # not fit the device memory free now: the request ends with the error, the others go on
+        except Exception as e:  # one request's failure must not stop the others
+            # the request's own caller work rejected its data, its arrays for the next stage do not
+            # fit the device memory free now, or its program failed: the request ends with the
+            # error, the others go on
             self._finish(live, error=e)
             return
         live.specs, live.tickets, live.pending = specs, [], list(specs)
