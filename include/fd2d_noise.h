@@ -423,8 +423,9 @@ int fd2d_resident_free(fd2d_resident* r);
 
 /*
  * Device memory of the solvers comes from a per-device cache: released buffers are kept (up to a
- * quarter of the device memory) and handed out again to later solves, which otherwise pay a
- * hipMalloc / hipFree of multi-GB buffers per solve. The split phase tables of the frequency-domain
+ * quarter of the device memory; when it is full, those released longest ago are freed to make room)
+ * and handed out again to later solves, which otherwise pay a hipMalloc / hipFree of multi-GB
+ * buffers per solve. The split phase tables of the frequency-domain
  * work are cached too (up to 256 MB per device), so solves of one configuration compute and upload
  * them once. The cached memory is not available to other allocators of the process (e.g.
  * PyTorch's); this call waits for the current device and frees every cached block and every phase
