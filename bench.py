@@ -967,6 +967,11 @@ def main():
             st1 = stations_run(a, grid="config1")
             print(f"# stations (300x300): {st1['batched_vs_plain']:.2f}x the plain path", file=sys.stderr,
                   flush=True)
+            # 64 Green runs on a 200x200 grid, all in one batch
+            g = copy.copy(args)
+            g.grid, g.order, g.nt, g.stations, g.plain = 200, 4, 2000, 64, 4
+            fresh_caches()
+            gr = greens_run(g)
             # a steady stream: long enough that the longest request (a 400^2 Hessian-vector product
             # of ~44,000 sequential steps) no longer sets most of the wall time. It runs in a process
             # of its own, as a server does: in this one, after the pipeline's 200 GB of device
@@ -985,10 +990,15 @@ def main():
                         "roofline_whole_job": r["roofline_whole_job"]}
             side = {"stations_vs_plain": round(st["batched_vs_plain"], 3),
                     "stations_300_vs_plain": round(st1["batched_vs_plain"], 3),
+                    "greens_200_vs_plain": round(gr["batched_vs_plain"], 3),
                     "served_vs_plain": round(sv["served_vs_plain"], 3),
                     "batched_runs": {
                         "stations": brief(st),
                         "stations_300": brief(st1),
+                        "greens_200": {"workload": gr["config"]["workload"], "gp_updates_per_s": gr["value"],
+                                       "plain_gp_updates_per_s": gr["plain_gp_updates_per_s"],
+                                       "batched_vs_plain": gr["batched_vs_plain"],
+                                       "identical_to_plain": gr["identical_to_plain"]},
                         "serving": {"workload": sv["config"]["workload"],
                                     "rate_req_s": sv["runs"][-1]["rate_req_s"],
                                     "requests": sv["runs"][-1]["requests"],
