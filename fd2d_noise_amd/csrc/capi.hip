@@ -2042,23 +2042,26 @@ struct Engine {
         }
         reap();
     }
-    // A class that already fills a grouped launch on its lane (kPairGroupMax running solves of the
-    // same FD order there) takes further solves on the partner lane, the lane of the same solver kind
-    // for the other FD orders, while no solve of another order runs there: the class's launches of a
-    // round then run side by side on the device, each filling the other's last, part-empty round of
-    // workgroups, instead of one after the other (64 Green runs at 200^2: 12.8x the plain path
-    // against 9.3x). A partner lane busy with its own orders is left alone (a served mix that
-    // spilled onto it ran 2% slower).
+    // A large class (kSpillFrom or more running solves of one FD order on its lane) is shared with
+    // the partner lane, the lane of the same solver kind for the other FD orders, while no solve of
+    // another order runs there: each new solve goes to whichever of the two holds fewer of the class.
+    // The class's launches of a round then run side by side on the device, each filling the other's
+    // last, part-empty round of workgroups, instead of one after the other (64 Green runs at 200^2:
+    // 13.6x the plain path with 32 + 32 side by side, 12.8x with 36 + 28, 9.3x with two launches of
+    // 32 in turn). A partner lane busy with its own orders is left alone (a served mix that spilled
+    // onto it ran 2% slower).
+    static constexpr int kSpillFrom = 32;
     int spill_lane(int lane, int R) const
     {
         const int partner = lane ^ 2;
-        int n = 0;
+        int n = 0, m = 0;
         bool partner_busy = false;
         for (const auto& e : act) {
             if (e->lane == lane && e->job->g.R == R) ++n;
+            if (e->lane == partner && e->job->g.R == R) ++m;
             if (e->lane == partner && e->job->g.R != R) partner_busy = true;
         }
-        return n >= kPairGroupMax && !partner_busy ? partner : lane;
+        return !partner_busy && n >= kSpillFrom && m < n ? partner : lane;
     }
     int64_t submit(const fd2d_solve* s)
     {
