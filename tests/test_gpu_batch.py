@@ -341,3 +341,30 @@ def test_engine_cancel(torch_mod, mix):
     torch.cuda.synchronize()
     for k in (0, 2):
         _same(torch, f"beside a cancelled solve: case {sel[k]}", ref[sel[k]], got[k])
+
+
+def test_large_class_shared_with_partner_lane(torch_mod):
+    """40 Green runs of one FD order in one batch: 32 run on their class's lane and the other 8 on
+    the partner lane (the lane of the other FD orders, idle here), whose launches run side by side
+    with the first lane's. Every G_fft and seismogram equals its single solve's, bit for bit."""
+    torch = torch_mod
+    from fd2d_noise_amd import batch as B, device as D
+    c = Config(order=4, nx=48, nz=40, nt=90, fw_nth=1, freq_samp=3)
+    rng = np.random.default_rng(907)
+    mu, rho = hetero_model(rng, c)
+    mu_d, rho_d = D.to_device(mu), D.to_device(rho)
+    rec = D.to_device(interior_points(rng, c, 2))
+    n = 40
+    srcs = [D.to_device(interior_points(rng, c, 1)) for _ in range(n)]
+    G_b = [D.empty_like_dims((c.nx, c.nz, c.n_sample), torch.complex128) for _ in range(n)]
+    s_b = [D.empty_like_dims((2, c.nt), torch.float64) for _ in range(n)]
+    B.run_batch([B.green(mu_d, rho_d, srcs[i], rec, 0, None, None, config=c, G_fft=G_b[i], seismograms=s_b[i])
+                 for i in range(n)])
+    torch.cuda.synchronize()
+    for i in range(n):
+        G = D.empty_like_dims((c.nx, c.nz, c.n_sample), torch.complex128)
+        s = D.empty_like_dims((2, c.nt), torch.float64)
+        D.run_forward1_green(mu_d, rho_d, srcs[i], rec, 0, None, None, config=c, G_fft=G, seismograms=s)
+        torch.cuda.synchronize()
+        assert torch.equal(G, G_b[i]) and torch.equal(s, s_b[i]), f"solve {i}"
+        assert float(G.abs().max()) > 0.0
