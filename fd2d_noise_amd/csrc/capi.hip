@@ -2042,33 +2042,40 @@ struct Engine {
         }
         reap();
     }
-    // A large class (kSpillFrom or more running solves of one FD order on its lane) is shared with
-    // the partner lane, the lane of the same solver kind for the other FD orders, while no solve of
-    // another order runs there: each new solve goes to whichever of the two holds fewer of the class.
-    // The class's launches of a round then run side by side on the device, each filling the other's
-    // last, part-empty round of workgroups, instead of one after the other (64 Green runs at 200^2:
-    // 13.6x the plain path with 32 + 32 side by side, 12.8x with 36 + 28, 9.3x with two launches of
-    // 32 in turn). A partner lane busy with its own orders is left alone (a served mix that spilled
-    // onto it ran 2% slower).
-    static constexpr int kSpillFrom = 32;
-    int spill_lane(int lane, int R) const
+    // A class of small solves (fewer than kSoloTiles tiles each) whose running solves of one FD order
+    // hold kSpillTiles or more tiles on its lane is shared with the partner lane, the lane of the
+    // same solver kind for the other FD orders, while no solve of another order runs there: each new
+    // solve goes to whichever of the two holds fewer of the class's tiles. The class's launches then
+    // run side by side on the device, each filling the other's last, part-empty round of workgroups,
+    // instead of one launch, or two in turn, that ends with one (64 Green runs at 200^2: 14.7x the
+    // plain path against 9.3x with two launches of 32 in turn; 16 at 300^2: 7.9x against 5.9x with
+    // one launch). Below kSpillTiles one launch is all the device holds at once anyway, and the
+    // second launch only costs (8 Green runs at 200^2, 224 tiles: 5.7x split against 5.8x). A
+    // partner lane busy with its own orders is left alone (a served mix that spilled onto it ran 2%
+    // slower), and large solves keep their lane: they step in groups the last-level cache holds.
+    static constexpr int kSpillTiles = 384;
+    int spill_lane(int lane, int R, int tiles) const
     {
+        if (tiles >= kSoloTiles) return lane;
         const int partner = lane ^ 2;
-        int n = 0, m = 0;
+        int64_t n = 0, m = 0;
         bool partner_busy = false;
         for (const auto& e : act) {
-            if (e->lane == lane && e->job->g.R == R) ++n;
-            if (e->lane == partner && e->job->g.R == R) ++m;
-            if (e->lane == partner && e->job->g.R != R) partner_busy = true;
+            const Job& j = *e->job;
+            if (j.S->n_tiles >= kSoloTiles) continue;
+            if (e->lane == lane && j.g.R == R) n += j.S->n_tiles;
+            if (e->lane == partner && j.g.R == R) m += j.S->n_tiles;
+            if (e->lane == partner && j.g.R != R) partner_busy = true;
         }
-        return !partner_busy && n >= kSpillFrom && m < n ? partner : lane;
+        return !partner_busy && n >= kSpillTiles && m < n ? partner : lane;
     }
     int64_t submit(const fd2d_solve* s)
     {
         check_solve(s);
         if (free_slots.empty()) raise(FD2D_ENOMEM, "the engine holds %d solves already", kEngineSlots);
         // the solve's set-up and steps run on its lane, after the work queued on the caller's stream
-        const int lane = spill_lane(lane_of(s->cfg->order, s->solver), s->cfg->order / 2);
+        const int tiles = (s->cfg->nx + kTX - 1) / kTX * ((s->cfg->nz + kTileZ - 1) / kTileZ);
+        const int lane = spill_lane(lane_of(s->cfg->order, s->solver), s->cfg->order / 2, tiles);
         const hipStream_t ls = lanes[lane];
         HIPCHK(hipEventRecord(submit_ev, st));
         HIPCHK(hipStreamWaitEvent(ls, submit_ev, 0));

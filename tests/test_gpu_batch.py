@@ -344,12 +344,13 @@ def test_engine_cancel(torch_mod, mix):
 
 
 def test_large_class_shared_with_partner_lane(torch_mod):
-    """40 Green runs of one FD order in one batch: 32 run on their class's lane and the other 8 on
-    the partner lane (the lane of the other FD orders, idle here), whose launches run side by side
-    with the first lane's. Every G_fft and seismogram equals its single solve's, bit for bit."""
+    """40 Green runs of one FD order in one batch, 20 tiles each: once 20 of them (400 tiles) run on
+    their class's lane, the other 20 go to the partner lane (the lane of the other FD orders, idle
+    here), whose launches run side by side with the first lane's. Every G_fft and seismogram equals
+    its single solve's, bit for bit."""
     torch = torch_mod
     from fd2d_noise_amd import batch as B, device as D
-    c = Config(order=4, nx=48, nz=40, nt=90, fw_nth=1, freq_samp=3)
+    c = Config(order=4, nx=200, nz=150, nt=60, fw_nth=1, freq_samp=3, Lx=199e3, Lz=149e3)
     rng = np.random.default_rng(907)
     mu, rho = hetero_model(rng, c)
     mu_d, rho_d = D.to_device(mu), D.to_device(rho)
