@@ -341,7 +341,8 @@ typedef struct fd2d_solve {
  * An engine advances many solves together: each round advances every active solve by one time
  * step, or by two where its two-step launch applies, with one launch for up to 36 solves of the same
  * FD order and field set (Green runs carry no displacement field, correlation and adjoint runs do),
- * on streams of the engine's own that wait for the caller's stream at each submission. Each solve's
+ * on streams of the engine's own that wait for the caller's stream at each submission (a large class
+ * of small solves is shared between two of them, whose launches then run side by side). Each solve's
  * outputs equal those of its own entry point bit for bit. Solves may be submitted between rounds
  * (continuous batching); a solve's outputs are complete once fd2d_engine_done() reports it. An
  * engine is used by one host thread at a time.
